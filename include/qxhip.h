@@ -1,0 +1,131 @@
+/*
+ * qxhip.h -- C ABI of libqxhip.so, the MI355X (gfx950) executor for QXTools'
+ * tensor-network contraction path.
+ *
+ * It replaces, for one GPU, the arithmetic reached from these reference seams:
+ *   - single_amplitude(tnc, plan, bitstring)         src/simulation.jl:86-91
+ *     (add_output! + contract_tn! + output[1], lines 88-90)
+ *   - the compute graph built by build_compute_graph src/compute_graph/compute_graph.jl:15-98
+ *     in its saved .qx DSL form                      docs/src/users_guide.md:37-170
+ *   - QXContexts.execute(dsl, input, param, ...)     bin/qxrun.jl:83-87
+ *     (per bitstring, per slice assignment: load / view / ncon, then sum)
+ *
+ * All arguments are plain pointers and sizes.  Complex numbers are interleaved
+ * (re, im) doubles, which is the memory layout of Julia's ComplexF64 arrays; tensor
+ * data is flattened in Julia's column-major order (leg 1 fastest), as
+ * reshape(data, prod(dim)) does in src/compute_graph/tensor_cache.jl:53.
+ *
+ * Errors never abort: every call returns a qx_status; qx_last_error() gives the
+ * message of the last failing call on the calling thread.
+ *
+ * Threading: a qx_ctx owns one HIP stream; calls on one ctx (and its plans) must
+ * be serialised by the caller.  Separate contexts are independent.
+ */
+#ifndef QXHIP_H
+#define QXHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t qx_status;
+
+#define QX_OK 0
+#define QX_ERR_ARGUMENT (-1)    /* null pointer, size or range error            */
+#define QX_ERR_PARSE (-2)       /* malformed DSL line                           */
+#define QX_ERR_LABELS (-3)      /* ncon labels outside the H/K/M/N classes      */
+#define QX_ERR_DATA (-4)        /* missing tensor data or wrong element count   */
+#define QX_ERR_MEMORY (-5)      /* arena does not fit the device / the cap      */
+#define QX_ERR_DEVICE (-6)      /* HIP runtime error                            */
+#define QX_ERR_STATE (-7)       /* call made in the wrong plan state            */
+#define QX_ERR_UNSUPPORTED (-8) /* valid DSL outside what this build executes   */
+
+/* Arithmetic of a plan.  QX_C128: complex128 storage and arithmetic.  The C64
+ * variants store complex64; their products are formed as 4 real products (4M),
+ * 3 real products (3M, Gauss), or, on large steps, as three bf16 MFMA products of
+ * a hi/lo split (BF16X3); accumulation is FP32, the cross-slice sum FP64. */
+enum qx_precision {
+  QX_C128 = 0,
+  QX_C64_4M = 1,
+  QX_C64_3M = 2,
+  QX_C64_BF16X3 = 3
+};
+
+typedef struct qx_ctx qx_ctx;   /* device + stream */
+typedef struct qx_plan qx_plan; /* lowered, device-resident plan */
+
+/* Create a context on HIP device `device`.  No reference counterpart: the
+ * reference selects the GPU inside QXContexts (bin/qxrun.jl:47-49, --gpu). */
+qx_status qx_ctx_create(int device, qx_ctx **out);
+void qx_ctx_destroy(qx_ctx *ctx);
+
+/* Parse and validate a .qx program (docs/src/users_guide.md:95-165): load, view,
+ * ncon, output, save.  Replaces QXContexts' DSL parse ("Parse input files",
+ * docs/src/distributed.md:97). */
+qx_status qx_plan_from_dsl(qx_ctx *ctx, const char *dsl, size_t len, int precision,
+                           qx_plan **out);
+
+/* Provide the values of one data key named by `load` lines (the JLD2 entries
+ * written by save_cache, src/compute_graph/tensor_cache.jl:90-106).  `z` holds
+ * `nelem` interleaved complex128 values in column-major order; it is copied. */
+qx_status qx_plan_set_tensor(qx_plan *plan, const char *key, const double *z, int64_t nelem);
+
+/* Plan options, set before qx_plan_finalize:
+ *   "batch"       bitstrings contracted together (power of two, default 256)
+ *   "arena_mb"    device arena cap in MiB (default: 90% of free memory)
+ *   "graph"       1 = replay each slice's launches from a captured hipGraph
+ *   "profile"     1 = time every step with HIP events (qx_plan_stats) */
+qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value);
+
+/* Check that every data key is present, lower the program (label classes,
+ * physical layouts, batched bitstring leg), compute the arena reuse schedule and
+ * upload the initial tensors. */
+qx_status qx_plan_finalize(qx_plan *plan);
+
+/* Sizes a caller needs: number of bitstring characters per string (the largest
+ * qubit named by an `output` line), complex values per bitstring in the result
+ * (product of the saved tensor's open dims; 1 for a closed network), number of
+ * slices (product of the slice variables' dims), number of distinct data keys
+ * and the total complex elements over all keys. */
+qx_status qx_plan_info(const qx_plan *plan, int64_t *num_chars, int64_t *result_elems,
+                       int64_t *num_slices, int64_t *num_keys, int64_t *key_elems);
+
+/* Name and element count of data key `i` (order of first `load` in the DSL);
+ * `name` receives up to `cap` bytes including the terminating NUL. */
+qx_status qx_plan_key(const qx_plan *plan, int64_t i, char *name, size_t cap, int64_t *nelem);
+
+/* Amplitudes of `nb` bitstrings (nb * num_chars characters in {'0','1','+','-'},
+ * character q = qubit q as in docs/src/basics.md:55-63), summed over slices
+ * [slice_begin, slice_end) (slice 0 = all variables at their first value, first
+ * variable fastest).  out: nb * result_elems interleaved complex128.
+ * Replaces single_amplitude (src/simulation.jl:86-91) for a batch of bitstrings
+ * and the slice loop + sum of QXContexts.execute (bin/qxrun.jl:83-87). */
+qx_status qx_amplitudes(qx_plan *plan, const char *bits, int64_t nb, int64_t slice_begin,
+                        int64_t slice_end, double *out);
+
+/* Vector-Jacobian product of qx_amplitudes over all slices: given amp_cot (nb *
+ * result_elems complex), writes for every data key (qx_plan_key order, keys
+ * concatenated) grad = sum_b amp_cot_b * conj(d amp_b / d T_key), summed over all
+ * load nodes that use the key.  This is the pullback ChainRulesCore uses for a
+ * holomorphic map; it has no counterpart in the reference (ComplexF64 forward
+ * only).  grad: key_elems interleaved complex128. */
+qx_status qx_amplitudes_vjp(qx_plan *plan, const char *bits, int64_t nb, const double *amp_cot,
+                            double *grad);
+
+/* JSON with per-step shapes (B, M, N, K), algorithmic flops and bytes, kernel
+ * choice and, when "profile" is on, the accumulated time of every step. */
+qx_status qx_plan_stats(const qx_plan *plan, char *json, size_t cap);
+
+void qx_plan_destroy(qx_plan *plan);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *qx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QXHIP_H */

@@ -1,0 +1,280 @@
+// Lowering of a parsed program for one bitstring batch size:
+//   * every bitstring-dependent tensor (downstream of an `output` line) gets an
+//     extra leg of size `batch`, laid out fastest, so one launch serves a batch of
+//     bitstrings and the batch leg makes every such step coalesced;
+//   * views are strided views: a base offset plus (slice variable * stride), the
+//     viewed leg becoming size 1 (users_guide.md:107-122) -- no copies;
+//   * an ncon result is laid out in its DSL label order (leg 1 fastest), size-1
+//     legs taking no space;
+//   * steps are split into sections run once per call (bitstring- and
+//     slice-independent), once per slice, and once per bitstring batch;
+//   * arena offsets come from a first-fit over liveness intervals.
+#include <algorithm>
+#include <numeric>
+
+#include "qx_internal.hpp"
+
+namespace qx {
+
+static bool pow2(int64_t d) { return d > 0 && (d & (d - 1)) == 0; }
+static int ilog2(int64_t d) {
+  int r = 0;
+  while ((int64_t(1) << r) < d) ++r;
+  return r;
+}
+
+int64_t first_fit(const std::vector<int64_t> &start, const std::vector<int64_t> &end,
+                  const std::vector<int64_t> &size, int esize, std::vector<int64_t> &off) {
+  const int64_t align = std::max<int64_t>(1, 256 / esize);
+  size_t n = start.size();
+  std::vector<size_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    if (start[a] != start[b]) return start[a] < start[b];
+    return size[a] > size[b];
+  });
+  off.assign(n, -1);
+  std::vector<size_t> placed;
+  int64_t total = 0;
+  for (size_t i : order) {
+    // live intervals overlapping [start, end) in time, sorted by offset
+    std::vector<std::pair<int64_t, int64_t>> busy;
+    for (size_t j : placed)
+      if (start[j] < end[i] && start[i] < end[j]) busy.push_back({off[j], off[j] + size[j]});
+    std::sort(busy.begin(), busy.end());
+    int64_t cand = 0;
+    for (auto &b : busy) {
+      if (cand + size[i] <= b.first) break;
+      cand = std::max(cand, (b.second + align - 1) / align * align);
+    }
+    off[i] = cand;
+    placed.push_back(i);
+    total = std::max(total, cand + size[i]);
+  }
+  return (total + align - 1) / align * align;
+}
+
+// Host-side proof that no kernel can leave its buffers: the largest offset every
+// step can form in A, B and C must lie inside that operand's storage.
+static void check_bounds(const Program &p, const Lowered &L, const std::vector<int> &root) {
+  std::vector<int64_t> key_elems;
+  for (auto &k : p.keys) key_elems.push_back(p.key_elems.at(k));
+  auto extent = [&](int node) -> std::pair<int64_t, int64_t> {  // (max offset, storage elems)
+    const Value &v = L.values[node];
+    int64_t mx = v.static_off;
+    for (auto &t : v.var_terms) mx += (p.var_dims[t.first] - 1) * t.second;
+    int64_t cap = v.buf == Buf::Data ? key_elems[v.data_key] : L.values[root[node]].elems;
+    return {mx, cap};
+  };
+  for (const Step &st : L.steps) {
+    int64_t ma = 0, mb = 0, mc = 0;
+    for (auto &g : st.cl) {
+      ma += (g.dim - 1) * g.sa;
+      mb += (g.dim - 1) * g.sb;
+      mc += (g.dim - 1) * g.sc;
+    }
+    for (auto &g : st.kl) {
+      ma += (g.dim - 1) * g.sa;
+      mb += (g.dim - 1) * g.sb;
+    }
+    auto ea = extent(st.in_a), eb = extent(st.in_b), ec = extent(st.node);
+    if (ea.first + ma >= ea.second || eb.first + mb >= eb.second || ec.first + mc >= ec.second ||
+        mc + 1 != st.c_elems)
+      throw Error(QX_ERR_STATE, "internal: step '" + p.nodes[st.node].name + "' addresses outside its operands");
+    for (auto &g : st.cl)
+      if (g.sa < 0 || g.sb < 0 || g.sc < 0) throw Error(QX_ERR_STATE, "internal: negative stride");
+  }
+}
+
+Lowered lower(const Program &p, int64_t batch) {
+  if (!pow2(batch)) throw Error(QX_ERR_ARGUMENT, "batch must be a power of two");
+  Lowered L;
+  L.batch = batch;
+  const size_t nn = p.nodes.size();
+  L.values.resize(nn);
+  std::map<std::string, int> key_index;
+  for (size_t i = 0; i < p.keys.size(); ++i) key_index[p.keys[i]] = (int)i;
+
+  for (size_t i = 0; i < nn; ++i) {
+    const Node &n = p.nodes[i];
+    Value &v = L.values[i];
+    switch (n.op) {
+      case Op::Load: {
+        v.buf = Buf::Data;
+        v.data_key = key_index.at(n.key);
+        v.dims = n.shape;
+        int64_t s = 1;
+        for (int64_t d : n.shape) {
+          v.strides.push_back(s);
+          s *= d;
+        }
+        v.elems = s;
+        break;
+      }
+      case Op::Output:
+        v.buf = Buf::Arena;
+        v.dims = n.shape;
+        v.strides = {batch};
+        v.dep = true;
+        v.beta_stride = 1;
+        v.elems = 2 * batch;
+        L.out_elems = v.elems;
+        break;
+      case Op::View: {
+        v = L.values[n.ia];
+        v.slot = -1;
+        v.var_terms.push_back({n.var_id, v.strides[n.pos]});
+        v.dims[n.pos] = 1;
+        break;
+      }
+      case Op::Ncon: {
+        v.buf = Buf::Arena;
+        v.dims = n.shape;
+        v.dep = n.dep;
+        int64_t s = 1;
+        if (n.dep) {
+          v.beta_stride = 1;
+          s = batch;
+        }
+        for (int64_t d : n.shape) {
+          v.strides.push_back(d > 1 ? s : 0);
+          if (d > 1) s *= d;
+        }
+        v.elems = s;
+        break;
+      }
+      case Op::Save:
+        v = L.values[n.ia];
+        break;
+    }
+  }
+
+  // steps
+  for (size_t i = 0; i < nn; ++i) {
+    const Node &n = p.nodes[i];
+    if (n.op != Op::Ncon) continue;
+    const Value &A = L.values[n.ia], &B = L.values[n.ib], &C = L.values[i];
+    Step st;
+    st.node = (int)i;
+    st.in_a = n.ia;
+    st.in_b = n.ib;
+    std::map<int, std::pair<int64_t, int64_t>> la, lb;  // label -> (dim, stride)
+    for (size_t j = 0; j < n.alab.size(); ++j) la[n.alab[j]] = {A.dims[j], A.strides[j]};
+    for (size_t j = 0; j < n.blab.size(); ++j) lb[n.blab[j]] = {B.dims[j], B.strides[j]};
+    if (C.dep) {
+      st.cl.push_back({batch, A.dep ? A.beta_stride : 0, B.dep ? B.beta_stride : 0, 1});
+      if (A.dep && B.dep) st.H *= batch;
+      else if (A.dep) st.M *= batch;
+      else st.N *= batch;
+    }
+    for (size_t j = 0; j < n.clab.size(); ++j) {
+      int l = n.clab[j];
+      int64_t d = C.dims[j];
+      bool ina = la.count(l), inb = lb.count(l);
+      if (ina && inb) st.H *= d;
+      else if (ina) st.M *= d;
+      else st.N *= d;
+      if (d == 1) continue;
+      st.cl.push_back({d, ina ? la[l].second : 0, inb ? lb[l].second : 0, C.strides[j]});
+    }
+    for (size_t j = 0; j < n.alab.size(); ++j) {
+      int l = n.alab[j];
+      if (!lb.count(l) || std::find(n.clab.begin(), n.clab.end(), l) != n.clab.end()) continue;
+      int64_t d = A.dims[j];
+      st.K *= d;
+      if (d == 1) continue;
+      st.kl.push_back({d, la[l].second, lb[l].second, 0});
+    }
+    st.c_elems = C.elems;
+    bool all_pow2 = true;
+    int nbc = 0, nbk = 0;
+    for (auto &g : st.cl) {
+      all_pow2 &= pow2(g.dim);
+      nbc += ilog2(g.dim);
+    }
+    for (auto &g : st.kl) {
+      all_pow2 &= pow2(g.dim);
+      nbk += ilog2(g.dim);
+    }
+    st.kern = (all_pow2 && nbc <= kMaxBits && nbk <= kMaxBits) ? Kern::Bits : Kern::Radix;
+    if (st.kern == Kern::Radix && ((int)st.cl.size() > kMaxLegs || (int)st.kl.size() > kMaxLegs))
+      throw Error(QX_ERR_UNSUPPORTED, "step with more than 48 legs of non power-of-two dims");
+    st.section = C.dep ? 2 : (n.sdep ? 1 : 0);
+    L.steps.push_back(std::move(st));
+  }
+
+  // result elements per bitstring
+  const Node &sv = p.nodes[p.save];
+  L.result_elems = 1;
+  for (int64_t d : sv.shape) L.result_elems *= d;
+
+  // liveness: schedule positions = sections 0, 1, then output fill, then section 2
+  std::vector<int> order;
+  for (int sec = 0; sec < 3; ++sec)
+    for (size_t s = 0; s < L.steps.size(); ++s)
+      if (L.steps[s].section == sec) order.push_back((int)s);
+  std::vector<int64_t> pos_of_step(L.steps.size());
+  int64_t fill_pos = 0;
+  {
+    int64_t pos = 0;
+    bool filled = false;
+    for (int s : order) {
+      if (!filled && L.steps[s].section == 2) {
+        fill_pos = pos++;
+        filled = true;
+      }
+      pos_of_step[s] = pos++;
+    }
+    if (!filled) fill_pos = pos++;
+  }
+  const int64_t end_pos = (int64_t)order.size() + 2;
+  // arena root of each node's value (views resolve to what they view)
+  std::vector<int> root(nn, -1);
+  for (size_t i = 0; i < nn; ++i) {
+    const Node &n = p.nodes[i];
+    if (n.op == Op::View || n.op == Op::Save) root[i] = root[n.ia];
+    else if (n.op == Op::Ncon || n.op == Op::Output) root[i] = (int)i;
+  }
+  std::vector<int64_t> st_pos(nn, -1), en_pos(nn, -1);
+  std::vector<int> sec_of(nn, 2);
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    int ni = L.steps[s].node;
+    st_pos[ni] = pos_of_step[s];
+    en_pos[ni] = pos_of_step[s] + 1;
+    sec_of[ni] = L.steps[s].section;
+  }
+  for (int o : p.outputs) {
+    st_pos[o] = fill_pos;
+    en_pos[o] = fill_pos + 1;
+    sec_of[o] = 2;
+  }
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    for (int in : {L.steps[s].in_a, L.steps[s].in_b}) {
+      int r = root[in];
+      if (r < 0) continue;
+      int64_t e = pos_of_step[s] + 1;
+      if (L.steps[s].section > sec_of[r]) e = end_pos;
+      en_pos[r] = std::max(en_pos[r], e);
+    }
+  }
+  int rs = root[p.save];
+  if (rs >= 0) en_pos[rs] = end_pos;
+  std::vector<int> arena_nodes;
+  std::vector<int64_t> S, E, Z, off;
+  for (size_t i = 0; i < nn; ++i)
+    if (root[i] == (int)i) {
+      arena_nodes.push_back((int)i);
+      S.push_back(st_pos[i]);
+      E.push_back(std::max(en_pos[i], st_pos[i] + 1));
+      Z.push_back(L.values[i].elems);
+    }
+  L.arena_elems = first_fit(S, E, Z, 8, off);
+  for (size_t j = 0; j < arena_nodes.size(); ++j) L.values[arena_nodes[j]].slot = off[j];
+  // propagate slots to views and save
+  for (size_t i = 0; i < nn; ++i)
+    if (root[i] >= 0 && root[i] != (int)i) L.values[i].slot = L.values[root[i]].slot;
+  check_bounds(p, L, root);
+  return L;
+}
+
+}  // namespace qx

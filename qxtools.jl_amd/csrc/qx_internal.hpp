@@ -1,0 +1,160 @@
+// Internal structures of libqxhip: parsed program, lowered steps, kernel launch
+// interface.  Nothing here crosses the C ABI.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qxhip.h"
+
+namespace qx {
+
+// Internal error carrying an ABI status code; caught at the C boundary.
+struct Error : std::runtime_error {
+  qx_status code;
+  Error(qx_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+// ----------------------------------------------------------------------------
+// Parsed .qx program (docs/src/users_guide.md:95-165)
+enum class Op { Load, Output, View, Ncon, Save };
+
+struct Node {
+  Op op = Op::Load;
+  std::string name;
+  int line = 0;
+  // load
+  std::string key;
+  std::vector<int64_t> dims;  // logical dims (load), output: {dim}
+  // output
+  int qubit = 0;
+  // view
+  std::string src, var;
+  int pos = 0;  // 0-based leg
+  int64_t vdim = 0;
+  // ncon
+  std::string a, b;
+  std::vector<int> clab, alab, blab;
+  // resolved inputs (node indices)
+  int ia = -1, ib = -1;
+  // derived
+  std::vector<int64_t> shape;  // logical shape of this node's tensor
+  bool dep = false;            // depends on output (bitstring) tensors
+  bool sdep = false;           // depends on a slice variable
+  int var_id = -1;             // view: slice variable index
+};
+
+struct Program {
+  std::vector<Node> nodes;                // topological order as written
+  std::map<std::string, int> by_name;     // tensor name -> node
+  int save = -1;                          // node index of the saved tensor
+  std::vector<std::string> vars;          // slice variables, sorted v1..vk
+  std::vector<int64_t> var_dims;
+  std::vector<std::string> keys;          // data keys, order of first load
+  std::map<std::string, int64_t> key_elems;
+  int num_chars = 0;                      // largest output qubit
+  std::vector<int> outputs;               // output node indices
+};
+
+Program parse_program(const char *text, size_t len);
+
+// ----------------------------------------------------------------------------
+// Lowered form for one batch size.
+enum class Buf : int8_t { None, Data, Arena };
+
+// Where a tensor value lives and how its logical legs map to memory.
+struct Value {
+  Buf buf = Buf::None;
+  int data_key = -1;               // Buf::Data
+  int64_t slot = -1;               // Buf::Arena: element offset in the arena
+  int64_t static_off = 0;          // element offset added to the base
+  std::vector<std::pair<int, int64_t>> var_terms;  // (slice var, stride): offset += v*stride
+  std::vector<int64_t> dims;       // logical dims (views: 1 at the sliced leg)
+  std::vector<int64_t> strides;    // element stride of each logical leg
+  bool dep = false;                // carries the batch leg (stride beta_stride)
+  int64_t beta_stride = 0;
+  int64_t elems = 0;               // elements of storage (arena values)
+};
+
+// One leg of a step in canonical form: dims and element strides in A, B and C.
+struct Leg {
+  int64_t dim;
+  int64_t sa, sb, sc;
+};
+
+enum class Kern : int8_t { Bits, Radix, Gemm };
+
+struct Step {
+  int node = -1;           // ncon node
+  int in_a = -1, in_b = -1;
+  std::vector<Leg> cl;     // C legs in C's memory order (fastest first)
+  std::vector<Leg> kl;     // contracted legs
+  int64_t H = 1, M = 1, N = 1, K = 1;  // class products (beta counted in its class)
+  int64_t c_elems = 1;
+  Kern kern = Kern::Bits;
+  int section = 0;         // 0 shared slice-free, 1 shared per slice, 2 per bitstring batch
+};
+
+struct Lowered {
+  int64_t batch = 0;
+  std::vector<Value> values;        // per node
+  std::vector<Step> steps;          // ncon nodes in order
+  int64_t arena_elems = 0;          // forward arena size (elements)
+  int64_t out_elems = 0;            // elements of one output tensor incl. batch
+  int64_t result_elems = 1;         // complex values per bitstring
+};
+
+Lowered lower(const Program &p, int64_t batch);
+
+// First-fit offsets for [start, end) intervals of given sizes (elements), 256-byte
+// aligned for element size `esize`.  Returns the total size; offsets written out.
+int64_t first_fit(const std::vector<int64_t> &start, const std::vector<int64_t> &end,
+                  const std::vector<int64_t> &size, int esize, std::vector<int64_t> &off);
+
+// ----------------------------------------------------------------------------
+// Kernel launch interface (kernels.hip).
+constexpr int kMaxBits = 48;   // bits of a C index or a K index
+constexpr int kMaxLegs = 48;
+
+struct BitArgs {
+  int64_t wa[kMaxBits], wb[kMaxBits];   // weight of each C-index bit in A and B
+  int64_t ka[kMaxBits], kb[kMaxBits];   // weight of each K-index bit in A and B
+  int nbc = 0, nbk = 0;
+  int conj_a = 0, conj_b = 0;
+};
+
+struct RadixArgs {
+  int64_t cd[kMaxLegs], ca[kMaxLegs], cb[kMaxLegs];
+  int64_t kd[kMaxLegs], ka[kMaxLegs], kb[kMaxLegs];
+  int ncl = 0, nkl = 0;
+  int conj_a = 0, conj_b = 0;
+};
+
+// C[m, n] (+)= sum_k A[m, k] B[k, n] per batch h, every index mapped through
+// bit weights (all dims powers of two).
+struct GemmArgs {
+  int64_t ha[kMaxBits], hb[kMaxBits], hc[kMaxBits];
+  int64_t ma[kMaxBits], mc[kMaxBits];
+  int64_t nb[kMaxBits], nc[kMaxBits];
+  int64_t ka[kMaxBits], kb[kMaxBits];
+  int nbh = 0, nbm = 0, nbn = 0, nbk = 0;
+  int conj_a = 0, conj_b = 0;
+};
+
+void launch_contract_bits(int prec, const void *A, const void *B, void *C, int64_t c_elems,
+                          const BitArgs &args, void *stream);
+void launch_contract_radix(int prec, const void *A, const void *B, void *C, int64_t c_elems,
+                           const RadixArgs &args, void *stream);
+void launch_fill_outputs(int prec, void *arena, const int64_t *d_out_off, int n_out,
+                         const int *d_out_q, const char *d_bits, int64_t nvalid, int64_t batch,
+                         int64_t num_chars, void *stream);
+void launch_accumulate(int prec, const void *root, int root_dep, int64_t batch, int64_t nvalid,
+                       int64_t relems, const int64_t *d_relem_off, double *acc, int64_t row0,
+                       void *stream);
+void launch_finalize(const double *acc, double *out, int64_t n, void *stream);
+void launch_zero(void *p, int64_t bytes, void *stream);
+
+}  // namespace qx

@@ -1,0 +1,523 @@
+// C ABI of libqxhip: contexts, plans, the slice/batch walk, statistics.
+//
+// The walk replaces QXContexts' executor loop (for each bitstring, for each slice
+// assignment: load / view / ncon, then sum; bin/qxrun.jl:83-87,
+// docs/src/users_guide.md:11-20) and single_amplitude (src/simulation.jl:86-91):
+//   once per call   : steps independent of bitstrings and slices
+//   per slice       : bitstring-independent steps that read a view
+//   per batch       : fill the output tensors of `batch` bitstrings, run the
+//                     bitstring-dependent steps, add the saved tensor into FP64
+//                     Neumaier accumulators on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <sstream>
+
+#include "qx_internal.hpp"
+
+struct qx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw qx::Error(QX_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+qx_status guard(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return QX_OK;
+  } catch (const qx::Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return QX_ERR_MEMORY;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return QX_ERR_ARGUMENT;
+  }
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t n) {
+    if (n <= bytes) return;
+    release();
+    if (n == 0) return;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw qx::Error(QX_ERR_MEMORY, "device allocation of " + std::to_string(n) + " bytes failed");
+    }
+    bytes = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+int ilog2(int64_t d) {
+  int r = 0;
+  while ((int64_t(1) << r) < d) ++r;
+  return r;
+}
+
+}  // namespace
+
+struct qx_plan {
+  qx_ctx *ctx = nullptr;
+  int prec = QX_C128;
+  qx::Program prog;
+  std::map<std::string, std::vector<double>> host_data;
+  int64_t opt_batch = 256;
+  int64_t opt_arena_mb = -1;
+  int64_t opt_graph = 0;
+  int64_t opt_profile = 0;
+  bool finalized = false;
+
+  std::vector<DevBuf> data;                 // per key
+  std::map<int64_t, qx::Lowered> lowered;   // per batch size
+  std::map<int64_t, std::vector<qx::BitArgs>> bitargs;
+  std::map<int64_t, std::vector<qx::RadixArgs>> radixargs;
+  DevBuf arena, out_off, out_q, bits, acc, outbuf, relem_off;
+  int64_t arena_cap_bytes = 0;
+  std::vector<double> step_ms;
+  std::vector<int64_t> step_calls;
+  std::vector<hipEvent_t> events;           // pool: 2 per profiled launch
+  std::vector<int> event_step;              // step of each recorded pair this call
+
+  size_t esize() const { return prec == QX_C128 ? 16 : 8; }
+
+  ~qx_plan() {
+    for (auto &d : data) d.release();
+    arena.release();
+    out_off.release();
+    out_q.release();
+    bits.release();
+    acc.release();
+    outbuf.release();
+    relem_off.release();
+    for (auto e : events) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+void build_args(qx_plan *P, int64_t batch) {
+  const qx::Lowered &L = P->lowered.at(batch);
+  auto &ba = P->bitargs[batch];
+  auto &ra = P->radixargs[batch];
+  ba.assign(L.steps.size(), qx::BitArgs{});
+  ra.assign(L.steps.size(), qx::RadixArgs{});
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    const qx::Step &st = L.steps[s];
+    if (st.kern == qx::Kern::Bits) {
+      qx::BitArgs &a = ba[s];
+      int b = 0;
+      for (auto &g : st.cl)
+        for (int i = 0; i < ilog2(g.dim); ++i, ++b) {
+          a.wa[b] = g.sa << i;
+          a.wb[b] = g.sb << i;
+        }
+      a.nbc = b;
+      b = 0;
+      for (auto &g : st.kl)
+        for (int i = 0; i < ilog2(g.dim); ++i, ++b) {
+          a.ka[b] = g.sa << i;
+          a.kb[b] = g.sb << i;
+        }
+      a.nbk = b;
+    } else {
+      qx::RadixArgs &a = ra[s];
+      a.ncl = (int)st.cl.size();
+      a.nkl = (int)st.kl.size();
+      for (int j = 0; j < a.ncl; ++j) {
+        a.cd[j] = st.cl[j].dim;
+        a.ca[j] = st.cl[j].sa;
+        a.cb[j] = st.cl[j].sb;
+      }
+      for (int j = 0; j < a.nkl; ++j) {
+        a.kd[j] = st.kl[j].dim;
+        a.ka[j] = st.kl[j].sa;
+        a.kb[j] = st.kl[j].sb;
+      }
+    }
+  }
+}
+
+const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch) {
+  auto it = P->lowered.find(batch);
+  if (it != P->lowered.end()) return it->second;
+  qx::Lowered L = qx::lower(P->prog, batch);
+  size_t need = (size_t)L.arena_elems * P->esize();
+  if (P->arena_cap_bytes > 0 && (int64_t)need > P->arena_cap_bytes)
+    throw qx::Error(QX_ERR_MEMORY, "arena of " + std::to_string(need >> 20) + " MiB for batch " +
+                                       std::to_string(batch) + " exceeds the cap of " +
+                                       std::to_string(P->arena_cap_bytes >> 20) + " MiB");
+  P->lowered.emplace(batch, std::move(L));
+  build_args(P, batch);
+  return P->lowered.at(batch);
+}
+
+// device pointer of a value for the given slice-variable assignment
+void *value_ptr(qx_plan *P, const qx::Value &v, const std::vector<int64_t> &varval) {
+  int64_t off = v.static_off;
+  for (auto &t : v.var_terms) off += varval[t.first] * t.second;
+  char *base = nullptr;
+  if (v.buf == qx::Buf::Data) base = (char *)P->data[v.data_key].p;
+  else if (v.buf == qx::Buf::Arena) base = (char *)P->arena.p + v.slot * (int64_t)P->esize();
+  else throw qx::Error(QX_ERR_STATE, "value without storage");
+  return base + off * (int64_t)P->esize();
+}
+
+void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int64_t> &varval) {
+  const qx::Step &st = L.steps[s];
+  void *A = value_ptr(P, L.values[st.in_a], varval);
+  void *B = value_ptr(P, L.values[st.in_b], varval);
+  void *C = value_ptr(P, L.values[st.node], varval);
+  hipStream_t stream = P->ctx->stream;
+  size_t ev = 0;
+  if (P->opt_profile) {
+    ev = P->event_step.size();
+    while (P->events.size() < 2 * (ev + 1)) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      P->events.push_back(e);
+    }
+    P->event_step.push_back((int)s);
+    hip_check(hipEventRecord(P->events[2 * ev], stream), "hipEventRecord");
+  }
+  if (st.kern == qx::Kern::Radix)
+    qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, P->radixargs[L.batch][s], stream);
+  else
+    qx::launch_contract_bits(P->prec, A, B, C, st.c_elems, P->bitargs[L.batch][s], stream);
+  if (P->opt_profile) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
+}
+
+// After the stream has drained: add every recorded launch's time to its step.
+void collect_profile(qx_plan *P, const qx::Lowered &L) {
+  if (!P->opt_profile) return;
+  if (P->step_ms.size() != L.steps.size()) {
+    P->step_ms.assign(L.steps.size(), 0.0);
+    P->step_calls.assign(L.steps.size(), 0);
+  }
+  for (size_t i = 0; i < P->event_step.size(); ++i) {
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, P->events[2 * i], P->events[2 * i + 1]), "hipEventElapsedTime");
+    P->step_ms[P->event_step[i]] += ms;
+    P->step_calls[P->event_step[i]] += 1;
+  }
+  P->event_step.clear();
+}
+
+void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1, double *out) {
+  if (!P->finalized) throw qx::Error(QX_ERR_STATE, "plan is not finalized");
+  if (nb <= 0) throw qx::Error(QX_ERR_ARGUMENT, "need at least one bitstring");
+  if (!bits || !out) throw qx::Error(QX_ERR_ARGUMENT, "null bitstring or output pointer");
+  int64_t nslices = 1;
+  for (int64_t d : P->prog.var_dims) nslices *= d;
+  if (s0 < 0 || s1 > nslices || s0 >= s1) throw qx::Error(QX_ERR_ARGUMENT, "slice range out of bounds");
+  const int64_t nch = P->prog.num_chars;
+  for (int64_t i = 0; i < nb * nch; ++i) {
+    char c = bits[i];
+    if (c != '0' && c != '1' && c != '+' && c != '-')
+      throw qx::Error(QX_ERR_ARGUMENT, "bitstring characters must be 0, 1, + or -");
+  }
+  int64_t batch = 1;
+  while (batch < nb && batch < P->opt_batch) batch <<= 1;
+  const qx::Lowered &L = ensure_lowered(P, batch);
+  hipStream_t stream = P->ctx->stream;
+  const size_t es = P->esize();
+  P->arena.reserve(std::max<size_t>((size_t)L.arena_elems * es, 256));
+
+  // output tables
+  std::vector<int64_t> ooff;
+  std::vector<int> oq;
+  for (int o : P->prog.outputs) {
+    ooff.push_back(L.values[o].slot);
+    oq.push_back(P->prog.nodes[o].qubit);
+  }
+  const int nout = (int)ooff.size();
+  P->out_off.reserve(std::max<size_t>(ooff.size() * 8, 16));
+  P->out_q.reserve(std::max<size_t>(oq.size() * 4, 16));
+  if (nout) {
+    hip_check(hipMemcpyAsync(P->out_off.p, ooff.data(), ooff.size() * 8, hipMemcpyHostToDevice, stream), "copy");
+    hip_check(hipMemcpyAsync(P->out_q.p, oq.data(), oq.size() * 4, hipMemcpyHostToDevice, stream), "copy");
+  }
+  // root element offsets (logical column-major order of the saved tensor)
+  const qx::Value &rv = L.values[P->prog.save];
+  const int64_t R = L.result_elems;
+  std::vector<int64_t> roff(R);
+  for (int64_t e = 0; e < R; ++e) {
+    int64_t r = e, o = 0;
+    for (size_t j = 0; j < rv.dims.size(); ++j) {
+      int64_t d = r % rv.dims[j];
+      r /= rv.dims[j];
+      o += d * rv.strides[j];
+    }
+    roff[e] = o;
+  }
+  P->relem_off.reserve(std::max<size_t>(R * 8, 16));
+  hip_check(hipMemcpyAsync(P->relem_off.p, roff.data(), R * 8, hipMemcpyHostToDevice, stream), "copy");
+  size_t nbits = (size_t)nb * nch;
+  P->bits.reserve(std::max<size_t>(nbits, 16));
+  if (nbits) hip_check(hipMemcpyAsync(P->bits.p, bits, nbits, hipMemcpyHostToDevice, stream), "copy");
+  P->acc.reserve((size_t)nb * R * 32);
+  qx::launch_zero(P->acc.p, nb * R * 32, stream);
+  P->outbuf.reserve((size_t)nb * R * 16);
+
+  P->event_step.clear();
+
+  const size_t nv = P->prog.vars.size();
+  std::vector<int64_t> varval(nv, 0);
+  auto run_section = [&](int sec) {
+    for (size_t s = 0; s < L.steps.size(); ++s)
+      if (L.steps[s].section == sec) run_step(P, L, s, varval);
+  };
+  run_section(0);
+  for (int64_t sl = s0; sl < s1; ++sl) {
+    int64_t r = sl;
+    for (size_t j = 0; j < nv; ++j) {
+      varval[j] = r % P->prog.var_dims[j];
+      r /= P->prog.var_dims[j];
+    }
+    run_section(1);
+    for (int64_t row0 = 0; row0 < nb; row0 += batch) {
+      int64_t nvalid = std::min<int64_t>(batch, nb - row0);
+      qx::launch_fill_outputs(P->prec, P->arena.p, (const int64_t *)P->out_off.p, nout, (const int *)P->out_q.p,
+                              (const char *)P->bits.p + row0 * nch, nvalid, batch, nch, stream);
+      run_section(2);
+      qx::launch_accumulate(P->prec, value_ptr(P, rv, varval), rv.dep ? 1 : 0, batch, nvalid, R,
+                            (const int64_t *)P->relem_off.p, (double *)P->acc.p, row0, stream);
+    }
+  }
+  qx::launch_finalize((const double *)P->acc.p, (double *)P->outbuf.p, nb * R, stream);
+  hip_check(hipMemcpyAsync(out, P->outbuf.p, (size_t)nb * R * 16, hipMemcpyDeviceToHost, stream), "copy out");
+  hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  collect_profile(P, L);
+}
+
+std::string stats_json(const qx_plan *P) {
+  std::ostringstream os;
+  os.precision(17);
+  os << "{\"precision\":" << P->prec << ",\"batches\":[";
+  bool first = true;
+  for (auto &kv : P->lowered) {
+    const qx::Lowered &L = kv.second;
+    if (!first) os << ",";
+    first = false;
+    os << "{\"batch\":" << L.batch << ",\"arena_bytes\":" << L.arena_elems * (int64_t)P->esize()
+       << ",\"result_elems\":" << L.result_elems << ",\"steps\":[";
+    for (size_t s = 0; s < L.steps.size(); ++s) {
+      const qx::Step &st = L.steps[s];
+      if (s) os << ",";
+      double flops = 8.0 * st.H * st.M * st.N * st.K;
+      double bytes = (double)P->esize() * st.H * (st.M * st.K + st.K * st.N + st.M * st.N);
+      os << "{\"name\":\"" << P->prog.nodes[st.node].name << "\",\"section\":" << st.section
+         << ",\"B\":" << st.H << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K
+         << ",\"flops\":" << flops << ",\"bytes\":" << bytes << ",\"kernel\":\""
+         << (st.kern == qx::Kern::Bits ? "bits" : st.kern == qx::Kern::Radix ? "radix" : "gemm") << "\"";
+      if (P->opt_profile && L.batch == P->lowered.rbegin()->first && s < P->step_ms.size())
+        os << ",\"ms\":" << P->step_ms[s] << ",\"calls\":" << P->step_calls[s];
+      os << "}";
+    }
+    os << "]}";
+  }
+  os << "]}";
+  return os.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+qx_status qx_ctx_create(int device, qx_ctx **out) {
+  return guard([&] {
+    if (!out) throw qx::Error(QX_ERR_ARGUMENT, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) throw qx::Error(QX_ERR_ARGUMENT, "no HIP device " + std::to_string(device));
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    auto *c = new qx_ctx;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      hip_check(e, "hipStreamCreate");
+    }
+    *out = c;
+  });
+}
+
+void qx_ctx_destroy(qx_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+qx_status qx_plan_from_dsl(qx_ctx *ctx, const char *dsl, size_t len, int precision, qx_plan **out) {
+  return guard([&] {
+    if (!ctx || !out) throw qx::Error(QX_ERR_ARGUMENT, "null context or output pointer");
+    *out = nullptr;
+    if (precision < QX_C128 || precision > QX_C64_BF16X3) throw qx::Error(QX_ERR_ARGUMENT, "unknown precision");
+    auto P = std::make_unique<qx_plan>();
+    P->ctx = ctx;
+    P->prec = precision;
+    P->prog = qx::parse_program(dsl, len);
+    *out = P.release();
+  });
+}
+
+qx_status qx_plan_set_tensor(qx_plan *plan, const char *key, const double *z, int64_t nelem) {
+  return guard([&] {
+    if (!plan || !key || (!z && nelem > 0)) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
+    if (plan->finalized) throw qx::Error(QX_ERR_STATE, "plan already finalized");
+    auto it = plan->prog.key_elems.find(key);
+    if (it == plan->prog.key_elems.end())
+      throw qx::Error(QX_ERR_DATA, std::string("data key '") + key + "' is not loaded by the program");
+    if (it->second != nelem)
+      throw qx::Error(QX_ERR_DATA, std::string("data key '") + key + "' needs " + std::to_string(it->second) +
+                                       " elements, got " + std::to_string(nelem));
+    plan->host_data[key].assign(z, z + 2 * nelem);
+  });
+}
+
+qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
+  return guard([&] {
+    if (!plan || !name) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
+    std::string n(name);
+    if (n == "batch") {
+      if (value <= 0 || (value & (value - 1)) || value > (1 << 20))
+        throw qx::Error(QX_ERR_ARGUMENT, "batch must be a power of two in [1, 2^20]");
+      plan->opt_batch = value;
+    } else if (n == "arena_mb") {
+      plan->opt_arena_mb = value;
+      plan->arena_cap_bytes = value > 0 ? (value << 20) : 0;
+    } else if (n == "graph") {
+      plan->opt_graph = value;
+    } else if (n == "profile") {
+      plan->opt_profile = value;
+    } else {
+      throw qx::Error(QX_ERR_ARGUMENT, "unknown option '" + n + "'");
+    }
+  });
+}
+
+qx_status qx_plan_finalize(qx_plan *plan) {
+  return guard([&] {
+    if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
+    if (plan->finalized) throw qx::Error(QX_ERR_STATE, "plan already finalized");
+    for (auto &k : plan->prog.keys)
+      if (!plan->host_data.count(k)) throw qx::Error(QX_ERR_DATA, "data key '" + k + "' was not provided");
+    hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+    if (plan->arena_cap_bytes == 0) {
+      size_t fr = 0, tot = 0;
+      hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+      plan->arena_cap_bytes = (int64_t)(fr * 0.9);
+    }
+    // lower once up front so label or memory errors surface here
+    ensure_lowered(plan, plan->opt_batch);
+    plan->data.resize(plan->prog.keys.size());
+    const size_t es = plan->esize();
+    for (size_t i = 0; i < plan->prog.keys.size(); ++i) {
+      const auto &h = plan->host_data[plan->prog.keys[i]];
+      size_t n = h.size() / 2;
+      plan->data[i].reserve(std::max<size_t>(n * es, 16));
+      if (plan->prec == QX_C128) {
+        hip_check(hipMemcpy(plan->data[i].p, h.data(), n * 16, hipMemcpyHostToDevice), "upload");
+      } else {
+        std::vector<float> f(h.begin(), h.end());
+        hip_check(hipMemcpy(plan->data[i].p, f.data(), n * 8, hipMemcpyHostToDevice), "upload");
+      }
+    }
+    plan->finalized = true;
+  });
+}
+
+qx_status qx_plan_info(const qx_plan *plan, int64_t *num_chars, int64_t *result_elems, int64_t *num_slices,
+                       int64_t *num_keys, int64_t *key_elems) {
+  return guard([&] {
+    if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
+    int64_t R = 1;
+    for (int64_t d : plan->prog.nodes[plan->prog.save].shape) R *= d;
+    int64_t ns = 1;
+    for (int64_t d : plan->prog.var_dims) ns *= d;
+    int64_t ke = 0;
+    for (auto &k : plan->prog.keys) ke += plan->prog.key_elems.at(k);
+    if (num_chars) *num_chars = plan->prog.num_chars;
+    if (result_elems) *result_elems = R;
+    if (num_slices) *num_slices = ns;
+    if (num_keys) *num_keys = (int64_t)plan->prog.keys.size();
+    if (key_elems) *key_elems = ke;
+  });
+}
+
+qx_status qx_plan_key(const qx_plan *plan, int64_t i, char *name, size_t cap, int64_t *nelem) {
+  return guard([&] {
+    if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
+    if (i < 0 || i >= (int64_t)plan->prog.keys.size()) throw qx::Error(QX_ERR_ARGUMENT, "key index out of range");
+    const std::string &k = plan->prog.keys[i];
+    if (name && cap) {
+      size_t n = std::min(cap - 1, k.size());
+      std::memcpy(name, k.data(), n);
+      name[n] = '\0';
+    }
+    if (nelem) *nelem = plan->prog.key_elems.at(k);
+  });
+}
+
+qx_status qx_amplitudes(qx_plan *plan, const char *bits, int64_t nb, int64_t slice_begin, int64_t slice_end,
+                        double *out) {
+  return guard([&] {
+    if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
+    hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+    amplitudes(plan, bits, nb, slice_begin, slice_end, out);
+  });
+}
+
+qx_status qx_amplitudes_vjp(qx_plan *plan, const char *bits, int64_t nb, const double *amp_cot, double *grad) {
+  return guard([&] {
+    (void)bits;
+    (void)nb;
+    (void)amp_cot;
+    (void)grad;
+    if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
+    throw qx::Error(QX_ERR_UNSUPPORTED, "vjp not available in this build");
+  });
+}
+
+qx_status qx_plan_stats(const qx_plan *plan, char *json, size_t cap) {
+  return guard([&] {
+    if (!plan || !json || cap == 0) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
+    std::string s = stats_json(plan);
+    if (s.size() + 1 > cap)
+      throw qx::Error(QX_ERR_ARGUMENT, "stats need " + std::to_string(s.size() + 1) + " bytes");
+    std::memcpy(json, s.c_str(), s.size() + 1);
+  });
+}
+
+void qx_plan_destroy(qx_plan *plan) {
+  if (!plan) return;
+  (void)hipSetDevice(plan->ctx->device);
+  delete plan;
+}
+
+const char *qx_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

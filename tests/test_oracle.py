@@ -1,0 +1,185 @@
+"""Pins of the CPU FP64 oracle (runs without a GPU).
+
+Each check is chosen so that a plausible mistake in the oracle (dropped term,
+wrong leg order, transposed operand, wrong slice arithmetic) fails it:
+* reference known-answer vectors: GHZ-3 statevector (test/test_tn_conversion.jl:38-40,
+  test/test_contraction_planning.jl:60) and GHZ-5 amplitudes (test/test_simulation.jl:24-25);
+* an independent FP64 statevector simulator, itself pinned by closed forms;
+* np.einsum on random steps of every label class;
+* Sum |amp|^2 = 1 and the slice-sum identity.
+"""
+
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import qxoracle as O
+import qxtest
+from qxhip import circuits, simulation
+
+S2 = 1 / math.sqrt(2)
+
+
+# --------------------------------------------------------------------- statevector pins
+def test_statevector_closed_forms():
+    n = 4
+    # X on qubits 1 and 3: basis state 1010
+    psi = O.statevector(n, [(circuits.X, [1]), (circuits.X, [3])])
+    assert O.sv_amplitude(psi, "1010") == 1.0
+    assert O.sv_amplitude(psi, "1000") == 0.0
+    # Hadamard layer: uniform 2^{-n/2}
+    psi = O.statevector(n, [(circuits.H, [q]) for q in range(1, n + 1)])
+    for b in simulation.amplitudes_all(n):
+        assert O.sv_amplitude(psi, b) == pytest.approx(0.25, abs=1e-15)
+    # CX with [target, ctrl] order: H(2) CX(target 1, ctrl 2) -> (|00> + |11>)/sqrt2
+    psi = O.statevector(2, [(circuits.H, [2]), (circuits.CX, [1, 2])])
+    assert O.sv_amplitude(psi, "11") == pytest.approx(S2)
+    assert O.sv_amplitude(psi, "01") == 0.0
+    # a CX read with the opposite qubit order would give |01>+|11> instead
+    psi_wrong = O.statevector(2, [(circuits.H, [2]), (circuits.CX, [2, 1])])
+    assert O.sv_amplitude(psi_wrong, "11") == 0.0
+
+
+def test_statevector_plus_minus_and_norm():
+    circ = circuits.create_rqc_circuit(2, 2, 5, seed=1)
+    psi = O.statevector(4, qxtest.circuit_gates(circ))
+    assert np.sum(np.abs(psi) ** 2) == pytest.approx(1.0, abs=1e-14)
+    a0, a1 = O.sv_amplitude(psi, "0010"), O.sv_amplitude(psi, "1010")
+    assert O.sv_amplitude(psi, "+010") == pytest.approx(S2 * (a0 + a1))
+    assert O.sv_amplitude(psi, "-010") == pytest.approx(S2 * (a0 - a1))
+
+
+def test_gate_matrices():
+    for g in (circuits.SQRT_X, circuits.SQRT_Y, circuits.SQRT_W):
+        assert np.allclose(g @ g.conj().T, np.eye(2))
+    assert np.allclose(circuits.SQRT_X @ circuits.SQRT_X, circuits.X)
+    assert np.allclose(circuits.SQRT_Y @ circuits.SQRT_Y, circuits.Y)
+    assert np.allclose(circuits.SQRT_W @ circuits.SQRT_W, circuits.W)
+    th = 0.37
+    h = 1e-6
+    assert np.allclose((circuits.rz(th + h) - circuits.rz(th - h)) / (2 * h), circuits.drz(th), atol=1e-9)
+    assert np.allclose((circuits.ry(th + h) - circuits.ry(th - h)) / (2 * h), circuits.dry(th), atol=1e-9)
+
+
+# --------------------------------------------------------------------- reference KATs
+@pytest.mark.parametrize("decompose", [False, True])
+@pytest.mark.parametrize("planner", ["greedy", "sweep"])
+def test_ghz3_kat(decompose, planner):
+    circ = circuits.create_test_circuit()
+    t, plan, g = qxtest.graph_for(circ, planner=planner, decompose=decompose, no_output=True)
+    out = O.amplitudes(g.to_dsl(), g.data, [""])[0].reshape(-1, order="F")
+    ref = np.zeros(8)
+    ref[[0, 7]] = S2
+    assert np.allclose(out, ref, atol=1e-15)
+
+
+def test_ghz5_kat():
+    circ = circuits.create_ghz_circuit(5)
+    t, plan, g = qxtest.graph_for(circ)
+    bits = list(simulation.amplitudes_all(5))
+    amps = O.amplitudes(g.to_dsl(), g.data, bits)
+    assert amps[0] == pytest.approx(S2) and amps[-1] == pytest.approx(S2)
+    assert np.allclose(amps[1:-1], 0)
+
+
+# --------------------------------------------------------------------- executor vs statevector
+@pytest.mark.parametrize("planner", ["sweep", "greedy"])
+def test_oracle_matches_statevector_rqc(planner):
+    circ = circuits.create_rqc_circuit(3, 3, 8, seed=42)
+    t, plan, g = qxtest.graph_for(circ, planner=planner)
+    bits = simulation.amplitudes_uniform(9, 4242, 24) + ["+-+-+-+-0"]
+    amps = O.amplitudes(g.to_dsl(), g.data, bits)
+    sv = qxtest.statevector_amps(circ, bits)
+    assert np.max(np.abs(amps - sv)) < 1e-14
+
+
+def test_norm_over_all_bitstrings():
+    circ = circuits.create_rqc_circuit(2, 3, 6, seed=5)
+    t, plan, g = qxtest.graph_for(circ)
+    amps = O.amplitudes(g.to_dsl(), g.data, list(simulation.amplitudes_all(6)))
+    assert np.sum(np.abs(amps) ** 2) == pytest.approx(1.0, abs=1e-13)
+
+
+def test_planted_transpose_fails_statevector_pin():
+    """Transposing one gate's tensor in the saved data must break agreement with the
+    statevector: the pin sees leg-order mistakes."""
+    circ = circuits.create_rqc_circuit(2, 2, 4, seed=2)
+    t, plan, g = qxtest.graph_for(circ)
+    bits = list(simulation.amplitudes_all(4))
+    sv = qxtest.statevector_amps(circ, bits)
+    data = dict(g.data)
+    key = next(k for k, v in data.items() if v.shape == (2, 2) and not np.allclose(v, v.T))
+    data[key] = data[key].T
+    amps = O.amplitudes(g.to_dsl(), data, bits)
+    assert np.max(np.abs(amps - sv)) > 1e-3
+
+
+# --------------------------------------------------------------------- ncon vs einsum
+def _einsum_ref(A, al, B, bl, cl):
+    letters = {l: chr(97 + i) for i, l in enumerate(sorted(set(al + bl)))}
+    spec = "".join(letters[l] for l in al) + "," + "".join(letters[l] for l in bl) + "->" + \
+        "".join(letters[l] for l in cl)
+    return np.einsum(spec, A, B)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_ncon_vs_einsum(seed):
+    rng = np.random.default_rng(seed)
+    nl = int(rng.integers(1, 8))
+    labs = list(range(1, nl + 1))
+    dims = {l: int(rng.choice([1, 2, 3])) for l in labs}
+    cls = {l: rng.choice(["H", "K", "M", "N"]) for l in labs}
+    al = [l for l in labs if cls[l] in "HKM"]
+    bl = [l for l in labs if cls[l] in "HKN"]
+    rng.shuffle(al)
+    rng.shuffle(bl)
+    cl = [l for l in labs if cls[l] in "HMN"]
+    rng.shuffle(cl)
+    A = rng.normal(size=[dims[l] for l in al]) + 1j * rng.normal(size=[dims[l] for l in al])
+    B = rng.normal(size=[dims[l] for l in bl]) + 1j * rng.normal(size=[dims[l] for l in bl])
+    C = O.ncon(np.asarray(A), al, np.asarray(B), bl, cl)
+    R = _einsum_ref(np.asarray(A), al, np.asarray(B), bl, cl)
+    assert C.shape == R.shape
+    assert np.allclose(C, R, rtol=1e-13, atol=1e-13)
+
+
+def test_ncon_rejects_invalid_labels():
+    A = np.ones((2, 2))
+    with pytest.raises(ValueError):
+        O.ncon(A, [1, 1], A, [1, 2], [2])          # repeated label
+    with pytest.raises(ValueError):
+        O.ncon(A, [1, 2], A, [2, 3], [3])          # label 1 summed out of A alone
+
+
+# --------------------------------------------------------------------- slicing and summation
+def test_slice_sum_identity_and_enumeration():
+    circ = circuits.create_rqc_circuit(3, 3, 8, seed=9)
+    t, plan, g = qxtest.graph_for(circ, planner="greedy", nslice=3)
+    t0, plan0, g0 = qxtest.graph_for(circ, planner="greedy", nslice=0)
+    dsl = g.to_dsl()
+    cmds = O.parse(dsl)
+    vs = O.slice_vars(cmds)
+    assert [v for v, _ in vs] == ["v1", "v2", "v3"]
+    assigns = list(O.slice_assignments(cmds))
+    assert len(assigns) == 8
+    assert [a["v1"] for a in assigns[:4]] == [0, 1, 0, 1]     # first variable fastest
+    assert [a["v3"] for a in assigns] == [0] * 4 + [1] * 4
+    bits = simulation.amplitudes_uniform(9, 3, 6)
+    sliced = O.amplitudes(dsl, g.data, bits)
+    full = O.amplitudes(g0.to_dsl(), g0.data, bits)
+    assert np.allclose(sliced, full, rtol=1e-13, atol=1e-16)
+    per = O.per_slice(dsl, g.data, bits[0])
+    assert abs(sum(per) - sliced[0]) < 1e-15
+
+
+def test_neumaier_recovers_cancellation():
+    acc = O.Neumaier()
+    for x in [1.0, 1e100, 1.0, -1e100]:
+        acc.add(x)
+    assert acc.value() == 2.0
+    acc = O.Neumaier((2,))
+    for x in itertools.repeat(np.array([0.1 + 0.1j, 1e-17]), 10):
+        acc.add(x)
+    assert acc.value()[0] == pytest.approx(1 + 1j, abs=1e-16)
