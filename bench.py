@@ -1,0 +1,158 @@
+"""Benchmark: amplitudes/s of the tensor-network contraction path on one MI355X.
+
+Workload (default): 5x5-grid random circuit, depth 20, complex128, saved plan with
+no sliced bonds, 256 seeded bitstrings (qxtools.jl_amd/workloads/rqc_5x5_d20.*).
+A step is one `qx_amplitudes` call over all 256 bitstrings, the plan's tensors
+already resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                    [--precision c128|c64_4m|c64_3m] [--cpu-seconds S | --no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "qxtools.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"c128": 78.6, "c64_4m": 157.3, "c64_3m": 157.3, "c64_bf16x3": 2500.0}
+ESIZE = {"c128": 16, "c64_4m": 8, "c64_3m": 8, "c64_bf16x3": 8}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_summary(stats: dict, prec: str) -> dict:
+    """Per kernel family: launches, time, algorithmic bytes and flops (per-step
+    counts from the plan; bytes = e*B*(MK+KN+MN), flops = 8*B*M*N*K)."""
+    fam: dict = {}
+    batch = max(stats["batches"], key=lambda b: b["batch"])
+    for st in batch["steps"]:
+        calls = st.get("calls", 0)
+        if not calls:
+            continue
+        f = fam.setdefault(st["kernel"], {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
+        f["launches"] += calls
+        f["ms"] += st["ms"]
+        f["bytes"] += st["bytes"] * calls
+        f["flops"] += st["flops"] * calls
+    return fam
+
+
+def plan_roofline(stats: dict, prec: str, ms_per_step: float, steps: int) -> float:
+    """sum_s max(flops_s/P, bytes_s/BW) over the launches of one step / measured."""
+    batch = max(stats["batches"], key=lambda b: b["batch"])
+    t = 0.0
+    for st in batch["steps"]:
+        calls = st.get("calls", 0) / max(steps, 1)
+        t += calls * max(st["flops"] / (PEAK_TFLOPS[prec] * 1e12), st["bytes"] / (HBM_PEAK_GBS * 1e9))
+    return t / (ms_per_step * 1e-3)
+
+
+def cpu_baseline(dsl: str, data: dict, bits: list, seconds: float) -> dict:
+    """The oracle (numpy TTGT, OpenBLAS zgemm) contracting whole amplitudes one
+    bitstring at a time, as the reference's per-bitstring loop does."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import qxoracle
+    from threadpoolctl import threadpool_limits
+
+    cores = min(os.cpu_count() or 1, 16)
+    done = 0
+    t0 = time.perf_counter()
+    with threadpool_limits(limits=cores, user_api="blas"):
+        while done == 0 or (time.perf_counter() - t0 < seconds and done < len(bits)):
+            qxoracle.amplitudes(dsl, data, [bits[done]])
+            done += 1
+            log(f"cpu baseline: {done} amplitude(s) in {time.perf_counter() - t0:.1f} s")
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "amplitudes/s", "cores": cores, "kind": "port",
+            "sample": f"{done} of the workload's bitstrings, each contracted over the whole plan "
+                      f"by the oracle ({el:.1f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rqc_5x5_d20")
+    ap.add_argument("--precision", default=None)
+    ap.add_argument("--batch", type=int, default=None)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build runs the contraction path on a single "
+              "MI355X (multi-GPU slice sharding is a design, not a measured path)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from qxhip import workloads
+    from qxhip.simulation import Executor
+
+    w = workloads.WORKLOADS[a.workload]
+    prec = a.precision or w.precision
+    dsl, data, bits = w.load()
+    log(f"workload {w.name}: {len(bits)} bitstrings, precision {prec}")
+    ex = Executor(dsl, data, precision=prec, device=0, batch=a.batch or min(256, len(bits)))
+    for i in range(a.warmup):
+        ex.amplitudes(bits)
+        log(f"warmup {i + 1}/{a.warmup} done")
+    ex.plan.set_option("profile", 1)   # per-launch HIP events on the library's stream
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        amps = ex.amplitudes(bits)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    log(f"timed {a.steps} steps in {el:.3f} s")
+    stats = ex.stats()
+    ms_step = 1e3 * el / a.steps
+    value = len(bits) * a.steps / el
+
+    fam = kernel_summary(stats, prec)
+    dom_name, dom = max(fam.items(), key=lambda kv: kv[1]["ms"])
+    t_hbm = dom["bytes"] / (HBM_PEAK_GBS * 1e9)
+    t_mfma = dom["flops"] / (PEAK_TFLOPS[prec] * 1e12)
+    if t_hbm >= t_mfma:
+        ach = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
+    else:
+        ach = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
+                "frac": ach / PEAK_TFLOPS[prec]}
+    roof.update({"traffic": None, "kernel": f"contract_{dom_name}", "launches": dom["launches"],
+                 "avg_launch_us": 1e3 * dom["ms"] / dom["launches"],
+                 "algorithmic_bytes_per_launch": dom["bytes"] / dom["launches"],
+                 "algorithmic_flops_per_launch": dom["flops"] / dom["launches"]})
+    kernels = {k: {"launches": v["launches"], "ms_per_step": v["ms"] / a.steps} for k, v in fam.items()}
+
+    out = {
+        "metric": "amplitudes/s", "value": value, "unit": "amplitudes/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128" if prec == "c128" else "c64",
+        "data": "synthetic: seeded random grid circuit and seeded bitstrings",
+        "config": {"workload": w.name, "qubits": w.qubits, "depth": w.depth, "bitstrings": len(bits),
+                   "sliced_bonds": w.sliced, "precision": prec, "batch": ex.plan.stats()["batches"][-1]["batch"]},
+        "roofline": roof,
+        "plan_roofline_frac": plan_roofline(stats, prec, ms_step, a.steps),
+        "kernels": kernels,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(dsl, data, bits, a.cpu_seconds)
+    _ = np.asarray(amps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -137,6 +137,225 @@ __global__ __launch_bounds__(kThreads) void k_contract_radix(const typename Vec<
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Tiled complex GEMM over bit-weight index maps: C[h, m, n] = sum_k A[h,m,k] B[h,k,n].
+// A TM x TK and a TK x TN tile are gathered into LDS per K chunk, lanes walking
+// whichever of the tile's index bits has the smaller stride in that operand (so the
+// loads are coalesced whatever the permutation), each lane accumulates an RM x RN
+// register micro-tile, and the C tile is staged through LDS so the stores walk C's
+// fastest tile bits.
+template <int TM, int TN, int TK>
+struct GemmTile {
+  static constexpr int NT = 256;
+  static constexpr int RM = 4;
+  static constexpr int RN = (TM * TN) / (NT * RM);
+  static constexpr int TX = TM / RM;
+  static constexpr int TY = TN / RN;
+  static_assert(TX * TY == NT, "tile shape");
+  static constexpr int LDS_AB = TK * (TM + TN);
+  static constexpr int LDS_C = TM * TN;
+  static constexpr int LDS = LDS_AB > LDS_C ? LDS_AB : LDS_C;
+};
+
+__device__ __forceinline__ int ilog2_dev(int x) { return 31 - __clz(x); }
+
+template <typename T, int TM, int TN, int TK>
+__global__ __launch_bounds__(256) void k_contract_gemm(const typename Vec<T>::type *__restrict__ A,
+                                                       const typename Vec<T>::type *__restrict__ B,
+                                                       typename Vec<T>::type *__restrict__ C, GemmArgs a) {
+  using V = typename Vec<T>::type;
+  using G = GemmTile<TM, TN, TK>;
+  __shared__ V lds[G::LDS];
+  __shared__ int64_t t_am[TM], t_cm[TM], t_bn[TN], t_cn[TN], t_ak[TK], t_bk[TK];
+  V *sA = lds;              // [TK][TM]
+  V *sB = lds + TK * TM;    // [TK][TN]
+  const int tid = threadIdx.x;
+  const int lm = ilog2_dev(TM), ln = ilog2_dev(TN), lk = ilog2_dev(TK);
+  const int rows = a.nbm < lm ? (1 << a.nbm) : TM;
+  const int cols = a.nbn < ln ? (1 << a.nbn) : TN;
+  const int kk = a.nbk < lk ? (1 << a.nbk) : TK;
+  const int64_t tiles_m = a.nbm > lm ? (int64_t(1) << (a.nbm - lm)) : 1;
+  const int64_t tiles_n = a.nbn > ln ? (int64_t(1) << (a.nbn - ln)) : 1;
+  const int64_t chunks = a.nbk > lk ? (int64_t(1) << (a.nbk - lk)) : 1;
+  const int64_t ntiles = tiles_m * tiles_n * (int64_t(1) << a.nbh);
+
+  // low-bit offset tables of this tile shape
+  for (int i = tid; i < TM; i += 256) {
+    int64_t oa = 0, oc = 0;
+    for (int b = 0; b < lm && b < a.nbm; ++b)
+      if ((i >> b) & 1) oa += a.ma[b], oc += a.mc[b];
+    t_am[i] = oa;
+    t_cm[i] = oc;
+  }
+  for (int j = tid; j < TN; j += 256) {
+    int64_t ob = 0, oc = 0;
+    for (int b = 0; b < ln && b < a.nbn; ++b)
+      if ((j >> b) & 1) ob += a.nb[b], oc += a.nc[b];
+    t_bn[j] = ob;
+    t_cn[j] = oc;
+  }
+  for (int l = tid; l < TK; l += 256) {
+    int64_t oa = 0, ob = 0;
+    for (int b = 0; b < lk && b < a.nbk; ++b)
+      if ((l >> b) & 1) oa += a.ka[b], ob += a.kb[b];
+    t_ak[l] = oa;
+    t_bk[l] = ob;
+  }
+  __syncthreads();
+
+  // XCD-aware order: consecutive tile ids land on the same XCD when the grid allows
+  const int64_t nblk = gridDim.x;
+  int64_t bid = blockIdx.x;
+  if (nblk % 8 == 0) bid = (bid % 8) * (nblk / 8) + bid / 8;
+
+  const int tx = tid % G::TX, ty = tid / G::TX;
+  for (int64_t tile = bid; tile < ntiles; tile += nblk) {
+    int64_t nt = tile % tiles_n;
+    int64_t rest = tile / tiles_n;
+    int64_t mt = rest % tiles_m;
+    int64_t h = rest / tiles_m;
+    int64_t oa = 0, ob = 0, oc = 0;
+    for (int b = 0; b < a.nbh; ++b)
+      if ((h >> b) & 1) oa += a.ha[b], ob += a.hb[b], oc += a.hc[b];
+    for (int b = lm; b < a.nbm; ++b)
+      if (((mt << lm) >> b) & 1) oa += a.ma[b], oc += a.mc[b];
+    for (int b = ln; b < a.nbn; ++b)
+      if (((nt << ln) >> b) & 1) ob += a.nb[b], oc += a.nc[b];
+
+    V acc[G::RM][G::RN];
+#pragma unroll
+    for (int r = 0; r < G::RM; ++r)
+#pragma unroll
+      for (int c = 0; c < G::RN; ++c) acc[r][c] = V{0, 0};
+
+    for (int64_t kc = 0; kc < chunks; ++kc) {
+      int64_t hka = 0, hkb = 0;
+      for (int b = lk; b < a.nbk; ++b)
+        if (((kc << lk) >> b) & 1) hka += a.ka[b], hkb += a.kb[b];
+      const V *Ab = A + oa + hka;
+      const V *Bb = B + ob + hkb;
+      for (int e = tid; e < TM * TK; e += 256) {
+        int i, l;
+        if (a.a_kfast) i = e / TK, l = e % TK;
+        else l = e / TM, i = e % TM;
+        V v{0, 0};
+        if (i < rows && l < kk) {
+          v = Ab[t_am[i] + t_ak[l]];
+          if (a.conj_a) v.y = -v.y;
+        }
+        sA[l * TM + i] = v;
+      }
+      for (int e = tid; e < TN * TK; e += 256) {
+        int j, l;
+        if (a.b_kfast) j = e / TK, l = e % TK;
+        else l = e / TN, j = e % TN;
+        V v{0, 0};
+        if (j < cols && l < kk) {
+          v = Bb[t_bn[j] + t_bk[l]];
+          if (a.conj_b) v.y = -v.y;
+        }
+        sB[l * TN + j] = v;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int l = 0; l < TK; ++l) {
+        V av[G::RM], bv[G::RN];
+#pragma unroll
+        for (int r = 0; r < G::RM; ++r) av[r] = sA[l * TM + tx + r * G::TX];
+#pragma unroll
+        for (int c = 0; c < G::RN; ++c) bv[c] = sB[l * TN + ty + c * G::TY];
+#pragma unroll
+        for (int r = 0; r < G::RM; ++r)
+#pragma unroll
+          for (int c = 0; c < G::RN; ++c) cmac<T>(acc[r][c], av[r], bv[c], 0, 0);
+      }
+      __syncthreads();
+    }
+    // stage the C tile, then store along C's fastest tile bits
+    V *sC = lds;  // [TM][TN]
+#pragma unroll
+    for (int r = 0; r < G::RM; ++r)
+#pragma unroll
+      for (int c = 0; c < G::RN; ++c) sC[(tx + r * G::TX) * TN + ty + c * G::TY] = acc[r][c];
+    __syncthreads();
+    V *Cb = C + oc;
+    for (int e = tid; e < TM * TN; e += 256) {
+      int i, j;
+      if (a.c_nfast) i = e / TN, j = e % TN;
+      else j = e / TM, i = e % TM;
+      if (i < rows && j < cols) Cb[t_cm[i] + t_cn[j]] = sC[i * TN + j];
+    }
+    __syncthreads();
+  }
+}
+
+// One wave per C element, the K range split over lanes, shuffle reduction.
+template <typename T>
+__global__ __launch_bounds__(256) void k_contract_reduce(const typename Vec<T>::type *__restrict__ A,
+                                                         const typename Vec<T>::type *__restrict__ B,
+                                                         typename Vec<T>::type *__restrict__ C,
+                                                         int64_t c_elems, BitArgs a) {
+  using V = typename Vec<T>::type;
+  const int lane = threadIdx.x & 63;
+  const int64_t nk = int64_t(1) << a.nbk;
+  // per-lane offsets of the low 6 k bits
+  int64_t la = 0, lb = 0;
+  for (int b = 0; b < 6 && b < a.nbk; ++b)
+    if ((lane >> b) & 1) la += a.ka[b], lb += a.kb[b];
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  for (int64_t idx = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); idx < c_elems; idx += waves) {
+    int64_t oa = 0, ob = 0;
+    for (int b = 0; b < a.nbc; ++b)
+      if ((idx >> b) & 1) oa += a.wa[b], ob += a.wb[b];
+    V acc{0, 0};
+    for (int64_t k0 = 0; k0 < nk; k0 += 64) {
+      if (k0 + lane >= nk) break;
+      int64_t ha = 0, hb = 0;
+      for (int b = 6; b < a.nbk; ++b)
+        if ((k0 >> b) & 1) ha += a.ka[b], hb += a.kb[b];
+      cmac<T>(acc, A[oa + ha + la], B[ob + hb + lb], a.conj_a, a.conj_b);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      acc.x += __shfl_xor(acc.x, off, 64);
+      acc.y += __shfl_xor(acc.y, off, 64);
+    }
+    if (lane == 0) C[idx] = acc;
+  }
+}
+
+
+// Row kernel for small K and N on many rows: each lane owns one row r (H and M
+// bits, lanes along C's fastest bits), loads its NK values of A once, and writes
+// all NN outputs of the row; B (NK x NN per h) is read through the cache.
+template <typename T, int NK, int NN>
+__global__ __launch_bounds__(256) void k_contract_rows(const typename Vec<T>::type *__restrict__ A,
+                                                       const typename Vec<T>::type *__restrict__ B,
+                                                       typename Vec<T>::type *__restrict__ C, RowArgs a) {
+  using V = typename Vec<T>::type;
+  constexpr int LB = 8;  // row bits resolved per lane
+  const int64_t nrows = int64_t(1) << a.nbr;
+  int64_t la = 0, lb = 0, lc = 0;
+  for (int b = 0; b < LB && b < a.nbr; ++b)
+    if ((threadIdx.x >> b) & 1) la += a.ra[b], lb += a.rb[b], lc += a.rc[b];
+  for (int64_t base = (int64_t)blockIdx.x * 256; base < nrows; base += (int64_t)gridDim.x * 256) {
+    if (base + threadIdx.x >= nrows) break;
+    int64_t oa = la, ob = lb, oc = lc;
+    for (int b = LB; b < a.nbr; ++b)
+      if ((base >> b) & 1) oa += a.ra[b], ob += a.rb[b], oc += a.rc[b];
+    V av[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) av[k] = A[oa + a.ka[k]];
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+      V acc{0, 0};
+#pragma unroll
+      for (int k = 0; k < NK; ++k) cmac<T>(acc, av[k], B[ob + a.kb[k] + a.nb[n]], a.conj_a, a.conj_b);
+      C[oc + a.nc[n]] = acc;
+    }
+  }
+}
+
 // Output tensors <b_q| for a batch: O_i[beta + batch*x], beta fastest.
 template <typename T>
 __global__ void k_fill_outputs(typename Vec<T>::type *arena, const int64_t *__restrict__ out_off, int n_out,
@@ -212,7 +431,84 @@ void check_launch(const char *what) {
   if (e != hipSuccess) throw Error(QX_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+
+template <int TM, int TN, int TK>
+void gemm_launch(int prec, const void *A, const void *B, void *C, const GemmArgs &a, hipStream_t s) {
+  int lm = 0, ln = 0;
+  while ((1 << lm) < TM) ++lm;
+  while ((1 << ln) < TN) ++ln;
+  int64_t tiles = (a.nbm > lm ? (int64_t(1) << (a.nbm - lm)) : 1) * (a.nbn > ln ? (int64_t(1) << (a.nbn - ln)) : 1) *
+                  (int64_t(1) << a.nbh);
+  int g = (int)(tiles > (int64_t(1) << 20) ? (int64_t(1) << 20) : tiles);
+  if (prec == QX_C128)
+    hipLaunchKernelGGL((k_contract_gemm<double, TM, TN, TK>), dim3(g), dim3(256), 0, s, (const double2 *)A,
+                       (const double2 *)B, (double2 *)C, a);
+  else
+    hipLaunchKernelGGL((k_contract_gemm<float, TM, TN, TK>), dim3(g), dim3(256), 0, s, (const float2 *)A,
+                       (const float2 *)B, (float2 *)C, a);
+}
+
+
+template <int NK, int NN>
+void rows_launch(int prec, const void *A, const void *B, void *C, const RowArgs &a, hipStream_t s) {
+  int g = grid_for(int64_t(1) << a.nbr, 256);
+  if (prec == QX_C128)
+    hipLaunchKernelGGL((k_contract_rows<double, NK, NN>), dim3(g), dim3(256), 0, s, (const double2 *)A,
+                       (const double2 *)B, (double2 *)C, a);
+  else
+    hipLaunchKernelGGL((k_contract_rows<float, NK, NN>), dim3(g), dim3(256), 0, s, (const float2 *)A,
+                       (const float2 *)B, (float2 *)C, a);
+}
+
+template <int NK>
+void rows_launch_n(int prec, const void *A, const void *B, void *C, const RowArgs &a, hipStream_t s) {
+  switch (a.nn) {
+    case 1: rows_launch<NK, 1>(prec, A, B, C, a, s); break;
+    case 2: rows_launch<NK, 2>(prec, A, B, C, a, s); break;
+    case 4: rows_launch<NK, 4>(prec, A, B, C, a, s); break;
+    default: rows_launch<NK, 8>(prec, A, B, C, a, s); break;
+  }
+}
+
 }  // namespace
+
+void launch_contract_rows(int prec, const void *A, const void *B, void *C, const RowArgs &args, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  switch (args.nk) {
+    case 1: rows_launch_n<1>(prec, A, B, C, args, s); break;
+    case 2: rows_launch_n<2>(prec, A, B, C, args, s); break;
+    case 4: rows_launch_n<4>(prec, A, B, C, args, s); break;
+    default: rows_launch_n<8>(prec, A, B, C, args, s); break;
+  }
+  check_launch("contract_rows");
+}
+
+namespace {
+
+}  // namespace
+
+void launch_contract_gemm(int prec, const void *A, const void *B, void *C, const GemmArgs &args, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  switch (args.tile) {
+    case 0: gemm_launch<128, 16, 16>(prec, A, B, C, args, s); break;
+    case 1: gemm_launch<128, 32, 16>(prec, A, B, C, args, s); break;
+    default: gemm_launch<64, 64, 16>(prec, A, B, C, args, s); break;
+  }
+  check_launch("contract_gemm");
+}
+
+void launch_contract_reduce(int prec, const void *A, const void *B, void *C, int64_t c_elems, const BitArgs &args,
+                            void *stream) {
+  int g = grid_for(c_elems, 4);
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == QX_C128)
+    hipLaunchKernelGGL(k_contract_reduce<double>, dim3(g), dim3(256), 0, s, (const double2 *)A, (const double2 *)B,
+                       (double2 *)C, c_elems, args);
+  else
+    hipLaunchKernelGGL(k_contract_reduce<float>, dim3(g), dim3(256), 0, s, (const float2 *)A, (const float2 *)B,
+                       (float2 *)C, c_elems, args);
+  check_launch("contract_reduce");
+}
 
 void launch_contract_bits(int prec, const void *A, const void *B, void *C, int64_t c_elems, const BitArgs &args,
                           void *stream) {

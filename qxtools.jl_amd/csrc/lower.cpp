@@ -10,6 +10,7 @@
 //     slice-independent), once per slice, and once per bitstring batch;
 //   * arena offsets come from a first-fit over liveness intervals.
 #include <algorithm>
+#include <climits>
 #include <numeric>
 
 #include "qx_internal.hpp"
@@ -84,6 +85,106 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
     for (auto &g : st.cl)
       if (g.sa < 0 || g.sb < 0 || g.sc < 0) throw Error(QX_ERR_STATE, "internal: negative stride");
   }
+}
+
+// Choose the kernel of a step and fill its launch arguments.
+//  * Rows   : >= 2^12 rows (H and M), K <= 8, N <= 8, C's fastest bit a row bit
+//  * Gemm   : K >= 8, N >= 8, M >= 32 (bits of each class counted without H)
+//  * Reduce : few C elements (<= 2^14) and K >= 256
+//  * Bits   : everything else with power-of-two dims
+//  * Radix  : any other dims
+void set_kernel_args(Step &st) {
+  if (st.kern == Kern::Radix) {
+    RadixArgs &a = st.radix;
+    a.ncl = (int)st.cl.size();
+    a.nkl = (int)st.kl.size();
+    for (int j = 0; j < a.ncl; ++j) a.cd[j] = st.cl[j].dim, a.ca[j] = st.cl[j].sa, a.cb[j] = st.cl[j].sb;
+    for (int j = 0; j < a.nkl; ++j) a.kd[j] = st.kl[j].dim, a.ka[j] = st.kl[j].sa, a.kb[j] = st.kl[j].sb;
+    return;
+  }
+  // expand legs into bits
+  struct CB { int64_t wa, wb, wc; };
+  std::vector<CB> cbits;
+  std::vector<std::pair<int64_t, int64_t>> kbits;
+  for (auto &g : st.cl)
+    for (int i = 0; i < ilog2(g.dim); ++i) cbits.push_back({g.sa << i, g.sb << i, g.sc << i});
+  for (auto &g : st.kl)
+    for (int i = 0; i < ilog2(g.dim); ++i) kbits.push_back({g.sa << i, g.sb << i});
+  BitArgs &b = st.bits;
+  b.nbc = (int)cbits.size();
+  b.nbk = (int)kbits.size();
+  for (int i = 0; i < b.nbc; ++i) b.wa[i] = cbits[i].wa, b.wb[i] = cbits[i].wb;
+  for (int i = 0; i < b.nbk; ++i) b.ka[i] = kbits[i].first, b.kb[i] = kbits[i].second;
+
+  std::vector<CB> hb, mb, nbits;
+  for (auto &c : cbits) {
+    if (c.wa && c.wb) hb.push_back(c);
+    else if (c.wa) mb.push_back(c);
+    else nbits.push_back(c);
+  }
+  const int nm = (int)mb.size(), nn = (int)nbits.size(), nk = (int)kbits.size();
+  // rows: many rows, K and N of at most 8 values each, C's fastest bit a row bit
+  const int nr = (int)(hb.size() + mb.size());
+  if (nr >= 12 && nk <= 3 && nn <= 3 && (nk + nn) >= 1 && !cbits.empty() && cbits[0].wa != 0) {
+    st.kern = Kern::Rows;
+    RowArgs &r = st.rows;
+    std::vector<CB> rb;
+    for (auto &c : cbits)
+      if (c.wa) rb.push_back(c);  // H and M bits, already in C order
+    r.nbr = (int)rb.size();
+    for (int i = 0; i < r.nbr; ++i) r.ra[i] = rb[i].wa, r.rb[i] = rb[i].wb, r.rc[i] = rb[i].wc;
+    r.nk = 1 << nk;
+    r.nn = 1 << nn;
+    for (int k = 0; k < r.nk; ++k) {
+      int64_t oa = 0, ob = 0;
+      for (int b = 0; b < nk; ++b)
+        if ((k >> b) & 1) oa += kbits[b].first, ob += kbits[b].second;
+      r.ka[k] = oa;
+      r.kb[k] = ob;
+    }
+    for (int n = 0; n < r.nn; ++n) {
+      int64_t ob = 0, oc = 0;
+      for (int b = 0; b < nn; ++b)
+        if ((n >> b) & 1) ob += nbits[b].wb, oc += nbits[b].wc;
+      r.nb[n] = ob;
+      r.nc[n] = oc;
+    }
+    return;
+  }
+  if (nk >= 3 && nn >= 3 && nm >= 5) {
+    st.kern = Kern::Gemm;
+    GemmArgs &g = st.gemm;
+    auto by_c = [](const CB &x, const CB &y) { return x.wc < y.wc; };
+    std::sort(mb.begin(), mb.end(), by_c);
+    std::sort(nbits.begin(), nbits.end(), by_c);
+    std::sort(kbits.begin(), kbits.end());
+    g.nbh = (int)hb.size();
+    g.nbm = nm;
+    g.nbn = nn;
+    g.nbk = nk;
+    for (int i = 0; i < g.nbh; ++i) g.ha[i] = hb[i].wa, g.hb[i] = hb[i].wb, g.hc[i] = hb[i].wc;
+    for (int i = 0; i < nm; ++i) g.ma[i] = mb[i].wa, g.mc[i] = mb[i].wc;
+    for (int i = 0; i < nn; ++i) g.nb[i] = nbits[i].wb, g.nc[i] = nbits[i].wc;
+    for (int i = 0; i < nk; ++i) g.ka[i] = kbits[i].first, g.kb[i] = kbits[i].second;
+    g.tile = nn <= 4 ? 0 : nn == 5 ? 1 : 2;
+    const int lm = g.tile == 2 ? 6 : 7, ln = g.tile == 0 ? 4 : g.tile == 1 ? 5 : 6, lk = 4;
+    auto min_w = [](const int64_t *w, int n) {
+      int64_t m = INT64_MAX;
+      for (int i = 0; i < n; ++i) m = std::min(m, w[i]);
+      return m;
+    };
+    int64_t kb_min_b = INT64_MAX, ka_min = min_w(g.ka, std::min(nk, lk));
+    for (int i = 0; i < std::min(nk, lk); ++i) kb_min_b = std::min(kb_min_b, g.kb[i]);
+    g.a_kfast = ka_min < min_w(g.ma, std::min(nm, lm));
+    g.b_kfast = kb_min_b < min_w(g.nb, std::min(nn, ln));
+    g.c_nfast = min_w(g.nc, std::min(nn, ln)) < min_w(g.mc, std::min(nm, lm));
+    return;
+  }
+  if (st.c_elems <= (int64_t(1) << 14) && nk >= 8) {
+    st.kern = Kern::Reduce;
+    return;
+  }
+
 }
 
 Lowered lower(const Program &p, int64_t batch) {
@@ -200,6 +301,7 @@ Lowered lower(const Program &p, int64_t batch) {
     if (st.kern == Kern::Radix && ((int)st.cl.size() > kMaxLegs || (int)st.kl.size() > kMaxLegs))
       throw Error(QX_ERR_UNSUPPORTED, "step with more than 48 legs of non power-of-two dims");
     st.section = C.dep ? 2 : (n.sdep ? 1 : 0);
+    set_kernel_args(st);
     L.steps.push_back(std::move(st));
   }
 

@@ -79,13 +79,54 @@ struct Value {
   int64_t elems = 0;               // elements of storage (arena values)
 };
 
+constexpr int kMaxBits = 48;   // bits of a C index or a K index
+constexpr int kMaxLegs = 48;
+
+struct BitArgs {
+  int64_t wa[kMaxBits], wb[kMaxBits];   // weight of each C-index bit in A and B
+  int64_t ka[kMaxBits], kb[kMaxBits];   // weight of each K-index bit in A and B
+  int nbc = 0, nbk = 0;
+  int conj_a = 0, conj_b = 0;
+};
+
+// Rows kernel: C[r, n] = sum_k A[r, k] B[h(r), k, n] with r over the H and M bits
+// (sorted by C weight), k over K bits, n over N bits; nk = 2^nbk, nn = 2^nbn small.
+struct RowArgs {
+  int64_t ra[kMaxBits], rb[kMaxBits], rc[kMaxBits];   // row bits: weights in A, B (H only), C
+  int64_t ka[8], kb[8];                               // per k value (not per bit)
+  int64_t nb[8], nc[8];                               // per n value
+  int nbr = 0, nk = 1, nn = 1;
+  int conj_a = 0, conj_b = 0;
+};
+
+struct RadixArgs {
+  int64_t cd[kMaxLegs], ca[kMaxLegs], cb[kMaxLegs];
+  int64_t kd[kMaxLegs], ka[kMaxLegs], kb[kMaxLegs];
+  int ncl = 0, nkl = 0;
+  int conj_a = 0, conj_b = 0;
+};
+
+// C[h, m, n] = sum_k A[h, m, k] B[h, k, n], every index mapped through bit
+// weights (all dims powers of two).  m bits are listed in increasing C weight,
+// likewise n bits; k bits in increasing A weight.
+struct GemmArgs {
+  int64_t ha[kMaxBits], hb[kMaxBits], hc[kMaxBits];
+  int64_t ma[kMaxBits], mc[kMaxBits];
+  int64_t nb[kMaxBits], nc[kMaxBits];
+  int64_t ka[kMaxBits], kb[kMaxBits];
+  int nbh = 0, nbm = 0, nbn = 0, nbk = 0;
+  int conj_a = 0, conj_b = 0;
+  int a_kfast = 0, b_kfast = 0, c_nfast = 0;  // coalescing direction of tile loads/stores
+  int tile = 0;                               // tile configuration index
+};
+
 // One leg of a step in canonical form: dims and element strides in A, B and C.
 struct Leg {
   int64_t dim;
   int64_t sa, sb, sc;
 };
 
-enum class Kern : int8_t { Bits, Radix, Gemm };
+enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows };
 
 struct Step {
   int node = -1;           // ncon node
@@ -96,6 +137,10 @@ struct Step {
   int64_t c_elems = 1;
   Kern kern = Kern::Bits;
   int section = 0;         // 0 shared slice-free, 1 shared per slice, 2 per bitstring batch
+  BitArgs bits;            // Kern::Bits / Kern::Reduce
+  GemmArgs gemm;           // Kern::Gemm
+  RowArgs rows;            // Kern::Rows
+  RadixArgs radix;         // Kern::Radix
 };
 
 struct Lowered {
@@ -116,36 +161,16 @@ int64_t first_fit(const std::vector<int64_t> &start, const std::vector<int64_t> 
 
 // ----------------------------------------------------------------------------
 // Kernel launch interface (kernels.hip).
-constexpr int kMaxBits = 48;   // bits of a C index or a K index
-constexpr int kMaxLegs = 48;
-
-struct BitArgs {
-  int64_t wa[kMaxBits], wb[kMaxBits];   // weight of each C-index bit in A and B
-  int64_t ka[kMaxBits], kb[kMaxBits];   // weight of each K-index bit in A and B
-  int nbc = 0, nbk = 0;
-  int conj_a = 0, conj_b = 0;
-};
-
-struct RadixArgs {
-  int64_t cd[kMaxLegs], ca[kMaxLegs], cb[kMaxLegs];
-  int64_t kd[kMaxLegs], ka[kMaxLegs], kb[kMaxLegs];
-  int ncl = 0, nkl = 0;
-  int conj_a = 0, conj_b = 0;
-};
-
-// C[m, n] (+)= sum_k A[m, k] B[k, n] per batch h, every index mapped through
-// bit weights (all dims powers of two).
-struct GemmArgs {
-  int64_t ha[kMaxBits], hb[kMaxBits], hc[kMaxBits];
-  int64_t ma[kMaxBits], mc[kMaxBits];
-  int64_t nb[kMaxBits], nc[kMaxBits];
-  int64_t ka[kMaxBits], kb[kMaxBits];
-  int nbh = 0, nbm = 0, nbn = 0, nbk = 0;
-  int conj_a = 0, conj_b = 0;
-};
-
+void set_kernel_args(Step &st);
 void launch_contract_bits(int prec, const void *A, const void *B, void *C, int64_t c_elems,
                           const BitArgs &args, void *stream);
+void launch_contract_gemm(int prec, const void *A, const void *B, void *C, const GemmArgs &args,
+                          void *stream);
+void launch_contract_rows(int prec, const void *A, const void *B, void *C, const RowArgs &args,
+                          void *stream);
+// few C elements, long K: one wave per C element, K split over lanes, shuffle reduce
+void launch_contract_reduce(int prec, const void *A, const void *B, void *C, int64_t c_elems,
+                            const BitArgs &args, void *stream);
 void launch_contract_radix(int prec, const void *A, const void *B, void *C, int64_t c_elems,
                            const RadixArgs &args, void *stream);
 void launch_fill_outputs(int prec, void *arena, const int64_t *d_out_off, int n_out,

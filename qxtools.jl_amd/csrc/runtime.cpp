@@ -70,12 +70,6 @@ struct DevBuf {
   }
 };
 
-int ilog2(int64_t d) {
-  int r = 0;
-  while ((int64_t(1) << r) < d) ++r;
-  return r;
-}
-
 }  // namespace
 
 struct qx_plan {
@@ -91,8 +85,6 @@ struct qx_plan {
 
   std::vector<DevBuf> data;                 // per key
   std::map<int64_t, qx::Lowered> lowered;   // per batch size
-  std::map<int64_t, std::vector<qx::BitArgs>> bitargs;
-  std::map<int64_t, std::vector<qx::RadixArgs>> radixargs;
   DevBuf arena, out_off, out_q, bits, acc, outbuf, relem_off;
   int64_t arena_cap_bytes = 0;
   std::vector<double> step_ms;
@@ -117,48 +109,6 @@ struct qx_plan {
 
 namespace {
 
-void build_args(qx_plan *P, int64_t batch) {
-  const qx::Lowered &L = P->lowered.at(batch);
-  auto &ba = P->bitargs[batch];
-  auto &ra = P->radixargs[batch];
-  ba.assign(L.steps.size(), qx::BitArgs{});
-  ra.assign(L.steps.size(), qx::RadixArgs{});
-  for (size_t s = 0; s < L.steps.size(); ++s) {
-    const qx::Step &st = L.steps[s];
-    if (st.kern == qx::Kern::Bits) {
-      qx::BitArgs &a = ba[s];
-      int b = 0;
-      for (auto &g : st.cl)
-        for (int i = 0; i < ilog2(g.dim); ++i, ++b) {
-          a.wa[b] = g.sa << i;
-          a.wb[b] = g.sb << i;
-        }
-      a.nbc = b;
-      b = 0;
-      for (auto &g : st.kl)
-        for (int i = 0; i < ilog2(g.dim); ++i, ++b) {
-          a.ka[b] = g.sa << i;
-          a.kb[b] = g.sb << i;
-        }
-      a.nbk = b;
-    } else {
-      qx::RadixArgs &a = ra[s];
-      a.ncl = (int)st.cl.size();
-      a.nkl = (int)st.kl.size();
-      for (int j = 0; j < a.ncl; ++j) {
-        a.cd[j] = st.cl[j].dim;
-        a.ca[j] = st.cl[j].sa;
-        a.cb[j] = st.cl[j].sb;
-      }
-      for (int j = 0; j < a.nkl; ++j) {
-        a.kd[j] = st.kl[j].dim;
-        a.ka[j] = st.kl[j].sa;
-        a.kb[j] = st.kl[j].sb;
-      }
-    }
-  }
-}
-
 const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch) {
   auto it = P->lowered.find(batch);
   if (it != P->lowered.end()) return it->second;
@@ -169,7 +119,6 @@ const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch) {
                                        std::to_string(batch) + " exceeds the cap of " +
                                        std::to_string(P->arena_cap_bytes >> 20) + " MiB");
   P->lowered.emplace(batch, std::move(L));
-  build_args(P, batch);
   return P->lowered.at(batch);
 }
 
@@ -201,10 +150,13 @@ void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int6
     P->event_step.push_back((int)s);
     hip_check(hipEventRecord(P->events[2 * ev], stream), "hipEventRecord");
   }
-  if (st.kern == qx::Kern::Radix)
-    qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, P->radixargs[L.batch][s], stream);
-  else
-    qx::launch_contract_bits(P->prec, A, B, C, st.c_elems, P->bitargs[L.batch][s], stream);
+  switch (st.kern) {
+    case qx::Kern::Radix: qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, st.radix, stream); break;
+    case qx::Kern::Gemm: qx::launch_contract_gemm(P->prec, A, B, C, st.gemm, stream); break;
+    case qx::Kern::Reduce: qx::launch_contract_reduce(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
+    case qx::Kern::Rows: qx::launch_contract_rows(P->prec, A, B, C, st.rows, stream); break;
+    default: qx::launch_contract_bits(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
+  }
   if (P->opt_profile) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
 }
 
@@ -311,6 +263,16 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   collect_profile(P, L);
 }
 
+const char *kern_name(qx::Kern k) {
+  switch (k) {
+    case qx::Kern::Radix: return "radix";
+    case qx::Kern::Gemm: return "gemm";
+    case qx::Kern::Reduce: return "reduce";
+    case qx::Kern::Rows: return "rows";
+    default: return "bits";
+  }
+}
+
 std::string stats_json(const qx_plan *P) {
   std::ostringstream os;
   os.precision(17);
@@ -330,7 +292,7 @@ std::string stats_json(const qx_plan *P) {
       os << "{\"name\":\"" << P->prog.nodes[st.node].name << "\",\"section\":" << st.section
          << ",\"B\":" << st.H << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K
          << ",\"flops\":" << flops << ",\"bytes\":" << bytes << ",\"kernel\":\""
-         << (st.kern == qx::Kern::Bits ? "bits" : st.kern == qx::Kern::Radix ? "radix" : "gemm") << "\"";
+         << kern_name(st.kern) << "\"";
       if (P->opt_profile && L.batch == P->lowered.rbegin()->first && s < P->step_ms.size())
         os << ",\"ms\":" << P->step_ms[s] << ",\"calls\":" << P->step_calls[s];
       os << "}";
@@ -402,6 +364,8 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
   return guard([&] {
     if (!plan || !name) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
     std::string n(name);
+    bool runtime_opt = (n == "profile");
+    if (plan->finalized && !runtime_opt) throw qx::Error(QX_ERR_STATE, "option '" + n + "' must be set before finalize");
     if (n == "batch") {
       if (value <= 0 || (value & (value - 1)) || value > (1 << 20))
         throw qx::Error(QX_ERR_ARGUMENT, "batch must be a power of two in [1, 2^20]");
@@ -412,7 +376,10 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
     } else if (n == "graph") {
       plan->opt_graph = value;
     } else if (n == "profile") {
+      // may be toggled at any time; every set clears the accumulated step times
       plan->opt_profile = value;
+      plan->step_ms.assign(plan->step_ms.size(), 0.0);
+      plan->step_calls.assign(plan->step_calls.size(), 0);
     } else {
       throw qx::Error(QX_ERR_ARGUMENT, "unknown option '" + n + "'");
     }

@@ -167,6 +167,9 @@ class Plan:
             o += n
         return out
 
+    def set_option(self, name: str, value: int) -> None:
+        _check(lib().qx_plan_set_option(self._h, name.encode(), int(value)))
+
     def stats(self) -> dict:
         cap = 1 << 16
         while True:
