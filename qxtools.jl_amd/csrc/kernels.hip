@@ -166,10 +166,15 @@ __global__ __launch_bounds__(256) void k_contract_gemm(const typename Vec<T>::ty
                                                        typename Vec<T>::type *__restrict__ C, GemmArgs a) {
   using V = typename Vec<T>::type;
   using G = GemmTile<TM, TN, TK>;
-  __shared__ V lds[G::LDS];
+  constexpr int PA = TM + 1, PB = TN + 1, PC = TN + 1;          // padded LDS rows
+  constexpr int LDS_AB = TK * (PA + PB), LDS_C = TM * PC;
+  constexpr int LDS = LDS_AB > LDS_C ? LDS_AB : LDS_C;
+  constexpr int EA = TM * TK / 256, EB = TN * TK / 256;         // tile elements per lane
+  static_assert(EA * 256 == TM * TK && EB * 256 == TN * TK, "tile/threads");
+  __shared__ V lds[LDS];
   __shared__ int64_t t_am[TM], t_cm[TM], t_bn[TN], t_cn[TN], t_ak[TK], t_bk[TK];
-  V *sA = lds;              // [TK][TM]
-  V *sB = lds + TK * TM;    // [TK][TN]
+  V *sA = lds;             // [TK][PA]
+  V *sB = lds + TK * PA;   // [TK][PB]
   const int tid = threadIdx.x;
   const int lm = ilog2_dev(TM), ln = ilog2_dev(TN), lk = ilog2_dev(TK);
   const int rows = a.nbm < lm ? (1 << a.nbm) : TM;
@@ -180,7 +185,6 @@ __global__ __launch_bounds__(256) void k_contract_gemm(const typename Vec<T>::ty
   const int64_t chunks = a.nbk > lk ? (int64_t(1) << (a.nbk - lk)) : 1;
   const int64_t ntiles = tiles_m * tiles_n * (int64_t(1) << a.nbh);
 
-  // low-bit offset tables of this tile shape
   for (int i = tid; i < TM; i += 256) {
     int64_t oa = 0, oc = 0;
     for (int b = 0; b < lm && b < a.nbm; ++b)
@@ -204,89 +208,118 @@ __global__ __launch_bounds__(256) void k_contract_gemm(const typename Vec<T>::ty
   }
   __syncthreads();
 
-  // XCD-aware order: consecutive tile ids land on the same XCD when the grid allows
+  // fixed per-lane gather offsets inside a tile, and LDS slots they fill
+  int64_t offa[EA], offb[EB];
+  int slota[EA], slotb[EB];
+  bool oka[EA], okb[EB];
+#pragma unroll
+  for (int j = 0; j < EA; ++j) {
+    int e = tid + 256 * j, i, l;
+    if (a.a_kfast) i = e / TK, l = e % TK;
+    else l = e / TM, i = e % TM;
+    oka[j] = i < rows && l < kk;
+    offa[j] = oka[j] ? t_am[i] + t_ak[l] : 0;
+    slota[j] = l * PA + i;
+  }
+#pragma unroll
+  for (int j = 0; j < EB; ++j) {
+    int e = tid + 256 * j, jj, l;
+    if (a.b_kfast) jj = e / TK, l = e % TK;
+    else l = e / TN, jj = e % TN;
+    okb[j] = jj < cols && l < kk;
+    offb[j] = okb[j] ? t_bn[jj] + t_bk[l] : 0;
+    slotb[j] = l * PB + jj;
+  }
+
   const int64_t nblk = gridDim.x;
   int64_t bid = blockIdx.x;
-  if (nblk % 8 == 0) bid = (bid % 8) * (nblk / 8) + bid / 8;
+  if (nblk % 8 == 0) bid = (bid % 8) * (nblk / 8) + bid / 8;   // XCD-contiguous tile ranges
+  const int64_t per = (ntiles + nblk - 1) / nblk;
+  const int64_t t_begin = bid * per;
+  const int64_t t_end = t_begin + per < ntiles ? t_begin + per : ntiles;
+  const int64_t items = t_begin < t_end ? (t_end - t_begin) * chunks : 0;
 
-  const int tx = tid % G::TX, ty = tid / G::TX;
-  for (int64_t tile = bid; tile < ntiles; tile += nblk) {
+  auto tile_offsets = [&](int64_t tile, int64_t &oa, int64_t &ob, int64_t &oc) {
     int64_t nt = tile % tiles_n;
     int64_t rest = tile / tiles_n;
     int64_t mt = rest % tiles_m;
     int64_t h = rest / tiles_m;
-    int64_t oa = 0, ob = 0, oc = 0;
+    oa = 0, ob = 0, oc = 0;
     for (int b = 0; b < a.nbh; ++b)
       if ((h >> b) & 1) oa += a.ha[b], ob += a.hb[b], oc += a.hc[b];
     for (int b = lm; b < a.nbm; ++b)
       if (((mt << lm) >> b) & 1) oa += a.ma[b], oc += a.mc[b];
     for (int b = ln; b < a.nbn; ++b)
       if (((nt << ln) >> b) & 1) ob += a.nb[b], oc += a.nc[b];
-
-    V acc[G::RM][G::RN];
+  };
+  V ra[EA], rb[EB];
+  auto fetch = [&](int64_t item) {
+    int64_t tile = t_begin + item / chunks, kc = item % chunks, oa, ob, oc;
+    tile_offsets(tile, oa, ob, oc);
+    for (int b = lk; b < a.nbk; ++b)
+      if (((kc << lk) >> b) & 1) oa += a.ka[b], ob += a.kb[b];
 #pragma unroll
-    for (int r = 0; r < G::RM; ++r)
+    for (int j = 0; j < EA; ++j) ra[j] = oka[j] ? A[oa + offa[j]] : V{0, 0};
 #pragma unroll
-      for (int c = 0; c < G::RN; ++c) acc[r][c] = V{0, 0};
+    for (int j = 0; j < EB; ++j) rb[j] = okb[j] ? B[ob + offb[j]] : V{0, 0};
+  };
 
-    for (int64_t kc = 0; kc < chunks; ++kc) {
-      int64_t hka = 0, hkb = 0;
-      for (int b = lk; b < a.nbk; ++b)
-        if (((kc << lk) >> b) & 1) hka += a.ka[b], hkb += a.kb[b];
-      const V *Ab = A + oa + hka;
-      const V *Bb = B + ob + hkb;
-      for (int e = tid; e < TM * TK; e += 256) {
-        int i, l;
-        if (a.a_kfast) i = e / TK, l = e % TK;
-        else l = e / TM, i = e % TM;
-        V v{0, 0};
-        if (i < rows && l < kk) {
-          v = Ab[t_am[i] + t_ak[l]];
-          if (a.conj_a) v.y = -v.y;
-        }
-        sA[l * TM + i] = v;
-      }
-      for (int e = tid; e < TN * TK; e += 256) {
-        int j, l;
-        if (a.b_kfast) j = e / TK, l = e % TK;
-        else l = e / TN, j = e % TN;
-        V v{0, 0};
-        if (j < cols && l < kk) {
-          v = Bb[t_bn[j] + t_bk[l]];
-          if (a.conj_b) v.y = -v.y;
-        }
-        sB[l * TN + j] = v;
-      }
-      __syncthreads();
+  const int tx = tid % G::TX, ty = tid / G::TX;
+  V acc[G::RM][G::RN];
+  if (items > 0) fetch(0);
+  for (int64_t it = 0; it < items; ++it) {
+    const int64_t kc = it % chunks;
+    if (kc == 0) {
+#pragma unroll
+      for (int r = 0; r < G::RM; ++r)
+#pragma unroll
+        for (int c = 0; c < G::RN; ++c) acc[r][c] = V{0, 0};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < EA; ++j) {
+      V v = ra[j];
+      if (a.conj_a) v.y = -v.y;
+      sA[slota[j]] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < EB; ++j) {
+      V v = rb[j];
+      if (a.conj_b) v.y = -v.y;
+      sB[slotb[j]] = v;
+    }
+    __syncthreads();
+    if (it + 1 < items) fetch(it + 1);   // next tile/chunk loads overlap this chunk's FMAs
 #pragma unroll 4
-      for (int l = 0; l < TK; ++l) {
-        V av[G::RM], bv[G::RN];
+    for (int l = 0; l < TK; ++l) {
+      V av[G::RM], bv[G::RN];
 #pragma unroll
-        for (int r = 0; r < G::RM; ++r) av[r] = sA[l * TM + tx + r * G::TX];
+      for (int r = 0; r < G::RM; ++r) av[r] = sA[l * PA + tx + r * G::TX];
 #pragma unroll
-        for (int c = 0; c < G::RN; ++c) bv[c] = sB[l * TN + ty + c * G::TY];
+      for (int c = 0; c < G::RN; ++c) bv[c] = sB[l * PB + ty + c * G::TY];
 #pragma unroll
-        for (int r = 0; r < G::RM; ++r)
+      for (int r = 0; r < G::RM; ++r)
 #pragma unroll
-          for (int c = 0; c < G::RN; ++c) cmac<T>(acc[r][c], av[r], bv[c], 0, 0);
-      }
+        for (int c = 0; c < G::RN; ++c) cmac<T>(acc[r][c], av[r], bv[c], 0, 0);
+    }
+    if (kc == chunks - 1) {
+      int64_t oa, ob, oc;
+      tile_offsets(t_begin + it / chunks, oa, ob, oc);
       __syncthreads();
-    }
-    // stage the C tile, then store along C's fastest tile bits
-    V *sC = lds;  // [TM][TN]
+      V *sC = lds;  // [TM][PC]
 #pragma unroll
-    for (int r = 0; r < G::RM; ++r)
+      for (int r = 0; r < G::RM; ++r)
 #pragma unroll
-      for (int c = 0; c < G::RN; ++c) sC[(tx + r * G::TX) * TN + ty + c * G::TY] = acc[r][c];
-    __syncthreads();
-    V *Cb = C + oc;
-    for (int e = tid; e < TM * TN; e += 256) {
-      int i, j;
-      if (a.c_nfast) i = e / TN, j = e % TN;
-      else j = e / TM, i = e % TM;
-      if (i < rows && j < cols) Cb[t_cm[i] + t_cn[j]] = sC[i * TN + j];
+        for (int c = 0; c < G::RN; ++c) sC[(tx + r * G::TX) * PC + ty + c * G::TY] = acc[r][c];
+      __syncthreads();
+      V *Cb = C + oc;
+      for (int e = tid; e < TM * TN; e += 256) {
+        int i, j;
+        if (a.c_nfast) i = e / TN, j = e % TN;
+        else j = e / TM, i = e % TM;
+        if (i < rows && j < cols) Cb[t_cm[i] + t_cn[j]] = sC[i * PC + j];
+      }
     }
-    __syncthreads();
   }
 }
 
@@ -325,33 +358,80 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const typename Vec<T>::
 }
 
 
-// Row kernel for small K and N on many rows: each lane owns one row r (H and M
-// bits, lanes along C's fastest bits), loads its NK values of A once, and writes
-// all NN outputs of the row; B (NK x NN per h) is read through the cache.
-template <typename T, int NK, int NN>
+// Row kernel for K <= 32, N <= 64 on many rows: each lane owns one row r (H and
+// M bits, lanes along C's fastest bits) and issues all NK loads of its A row at
+// once, then produces the row's outputs NC at a time.  B is staged in LDS once per
+// workgroup -- one K x N slab per value of the H bits that vary across the
+// workgroup's lanes -- so its reads are LDS broadcasts; the LDS size is set at
+// launch to what the slabs need.
+constexpr int kRowLdsElems = 4096;  // complex elements of staged B at most
+
+template <typename T, int NK, int NC>
 __global__ __launch_bounds__(256) void k_contract_rows(const typename Vec<T>::type *__restrict__ A,
                                                        const typename Vec<T>::type *__restrict__ B,
                                                        typename Vec<T>::type *__restrict__ C, RowArgs a) {
   using V = typename Vec<T>::type;
   constexpr int LB = 8;  // row bits resolved per lane
+  extern __shared__ __align__(16) unsigned char rows_smem[];
+  V *sB = reinterpret_cast<V *>(rows_smem);
+  const int nn = a.nn, kn = NK * nn;
   const int64_t nrows = int64_t(1) << a.nbr;
-  int64_t la = 0, lb = 0, lc = 0;
-  for (int b = 0; b < LB && b < a.nbr; ++b)
-    if ((threadIdx.x >> b) & 1) la += a.ra[b], lb += a.rb[b], lc += a.rc[b];
+  const int lowbits = a.nbr < LB ? a.nbr : LB;
+  int64_t la = 0, lc = 0;
+  int slab = 0, nslab_bits = 0;
+  for (int b = 0; b < lowbits; ++b) {
+    const bool on = (threadIdx.x >> b) & 1;
+    if (on) la += a.ra[b], lc += a.rc[b];
+    if ((a.hlo >> b) & 1) {
+      if (on) slab |= 1 << nslab_bits;
+      ++nslab_bits;
+    }
+  }
+  const int nslab = 1 << nslab_bits;
   for (int64_t base = (int64_t)blockIdx.x * 256; base < nrows; base += (int64_t)gridDim.x * 256) {
-    if (base + threadIdx.x >= nrows) break;
-    int64_t oa = la, ob = lb, oc = lc;
+    int64_t oa = la, obh = 0, oc = lc;
     for (int b = LB; b < a.nbr; ++b)
-      if ((base >> b) & 1) oa += a.ra[b], ob += a.rb[b], oc += a.rc[b];
+      if ((base >> b) & 1) oa += a.ra[b], obh += a.rb[b], oc += a.rc[b];
+    const bool live = base + threadIdx.x < nrows;
     V av[NK];
+    if (live) {
 #pragma unroll
-    for (int k = 0; k < NK; ++k) av[k] = A[oa + a.ka[k]];
+      for (int k = 0; k < NK; ++k) {
+        av[k] = A[oa + a.ka[k]];
+        if (a.conj_a) av[k].y = -av[k].y;
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nslab * kn; e += 256) {
+      const int sl = e / kn, r = e % kn, k = r / nn, n = r % nn;
+      int64_t ob = obh;
+      for (int b = 0, j = 0; b < lowbits; ++b)
+        if ((a.hlo >> b) & 1) {
+          if ((sl >> j) & 1) ob += a.rb[b];
+          ++j;
+        }
+      V v = B[ob + a.kb[k] + a.nb[n]];
+      if (a.conj_b) v.y = -v.y;
+      sB[e] = v;
+    }
+    __syncthreads();
+    if (live) {
+      const V *Bs = sB + slab * kn;
+      V *Cr = C + oc;
+      for (int n0 = 0; n0 < nn; n0 += NC) {
+        V acc[NC];
 #pragma unroll
-    for (int n = 0; n < NN; ++n) {
-      V acc{0, 0};
+        for (int c = 0; c < NC; ++c) acc[c] = V{0, 0};
 #pragma unroll
-      for (int k = 0; k < NK; ++k) cmac<T>(acc, av[k], B[ob + a.kb[k] + a.nb[n]], a.conj_a, a.conj_b);
-      C[oc + a.nc[n]] = acc;
+        for (int k = 0; k < NK; ++k) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            if (n0 + c < nn) cmac<T>(acc[c], av[k], Bs[k * nn + n0 + c], 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (n0 + c < nn) Cr[a.nc[n0 + c]] = acc[c];
+      }
     }
   }
 }
@@ -439,7 +519,9 @@ void gemm_launch(int prec, const void *A, const void *B, void *C, const GemmArgs
   while ((1 << ln) < TN) ++ln;
   int64_t tiles = (a.nbm > lm ? (int64_t(1) << (a.nbm - lm)) : 1) * (a.nbn > ln ? (int64_t(1) << (a.nbn - ln)) : 1) *
                   (int64_t(1) << a.nbh);
-  int g = (int)(tiles > (int64_t(1) << 20) ? (int64_t(1) << 20) : tiles);
+  // persistent grid: a few workgroups per CU, each walking a contiguous tile range
+  const int64_t cap = 256 * 4;
+  int g = (int)(tiles < cap ? tiles : cap);
   if (prec == QX_C128)
     hipLaunchKernelGGL((k_contract_gemm<double, TM, TN, TK>), dim3(g), dim3(256), 0, s, (const double2 *)A,
                        (const double2 *)B, (double2 *)C, a);
@@ -449,19 +531,22 @@ void gemm_launch(int prec, const void *A, const void *B, void *C, const GemmArgs
 }
 
 
-template <int NK, int NN>
+template <int NK, int NC>
 void rows_launch(int prec, const void *A, const void *B, void *C, const RowArgs &a, hipStream_t s) {
   int g = grid_for(int64_t(1) << a.nbr, 256);
+  int hl = 0;
+  for (int b = 0; b < 8; ++b) hl += (a.hlo >> b) & 1;
+  size_t smem = (size_t)(NK * a.nn) << hl;
   if (prec == QX_C128)
-    hipLaunchKernelGGL((k_contract_rows<double, NK, NN>), dim3(g), dim3(256), 0, s, (const double2 *)A,
-                       (const double2 *)B, (double2 *)C, a);
+    hipLaunchKernelGGL((k_contract_rows<double, NK, NC>), dim3(g), dim3(256), smem * sizeof(double2), s,
+                       (const double2 *)A, (const double2 *)B, (double2 *)C, a);
   else
-    hipLaunchKernelGGL((k_contract_rows<float, NK, NN>), dim3(g), dim3(256), 0, s, (const float2 *)A,
-                       (const float2 *)B, (float2 *)C, a);
+    hipLaunchKernelGGL((k_contract_rows<float, NK, NC>), dim3(g), dim3(256), smem * sizeof(float2), s,
+                       (const float2 *)A, (const float2 *)B, (float2 *)C, a);
 }
 
 template <int NK>
-void rows_launch_n(int prec, const void *A, const void *B, void *C, const RowArgs &a, hipStream_t s) {
+void rows_launch_k(int prec, const void *A, const void *B, void *C, const RowArgs &a, hipStream_t s) {
   switch (a.nn) {
     case 1: rows_launch<NK, 1>(prec, A, B, C, a, s); break;
     case 2: rows_launch<NK, 2>(prec, A, B, C, a, s); break;
@@ -475,10 +560,12 @@ void rows_launch_n(int prec, const void *A, const void *B, void *C, const RowArg
 void launch_contract_rows(int prec, const void *A, const void *B, void *C, const RowArgs &args, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   switch (args.nk) {
-    case 1: rows_launch_n<1>(prec, A, B, C, args, s); break;
-    case 2: rows_launch_n<2>(prec, A, B, C, args, s); break;
-    case 4: rows_launch_n<4>(prec, A, B, C, args, s); break;
-    default: rows_launch_n<8>(prec, A, B, C, args, s); break;
+    case 1: rows_launch_k<1>(prec, A, B, C, args, s); break;
+    case 2: rows_launch_k<2>(prec, A, B, C, args, s); break;
+    case 4: rows_launch_k<4>(prec, A, B, C, args, s); break;
+    case 8: rows_launch_k<8>(prec, A, B, C, args, s); break;
+    case 16: rows_launch_k<16>(prec, A, B, C, args, s); break;
+    default: rows_launch_k<32>(prec, A, B, C, args, s); break;
   }
   check_launch("contract_rows");
 }

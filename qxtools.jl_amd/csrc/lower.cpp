@@ -88,7 +88,7 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
 }
 
 // Choose the kernel of a step and fill its launch arguments.
-//  * Rows   : >= 2^12 rows (H and M), K <= 8, N <= 8, C's fastest bit a row bit
+//  * Rows   : >= 2^12 rows (H and M), K <= 16, N <= 64, C's fastest bit a row bit
 //  * Gemm   : K >= 8, N >= 8, M >= 32 (bits of each class counted without H)
 //  * Reduce : few C elements (<= 2^14) and K >= 256
 //  * Bits   : everything else with power-of-two dims
@@ -125,13 +125,22 @@ void set_kernel_args(Step &st) {
   const int nm = (int)mb.size(), nn = (int)nbits.size(), nk = (int)kbits.size();
   // rows: many rows, K and N of at most 8 values each, C's fastest bit a row bit
   const int nr = (int)(hb.size() + mb.size());
-  if (nr >= 12 && nk <= 3 && nn <= 3 && (nk + nn) >= 1 && !cbits.empty() && cbits[0].wa != 0) {
+  int hlow = 0;  // H bits among the 8 lowest row bits (they multiply the staged B)
+  {
+    int j = 0;
+    for (auto &c : cbits)
+      if (c.wa && j < 8) hlow += (c.wb != 0), ++j;
+  }
+  if (nr >= 12 && nk <= 4 && nn <= 6 && (nk + nn) >= 1 && !cbits.empty() && cbits[0].wa != 0 &&
+      ((int64_t(1) << (nk + nn + hlow)) <= 4096)) {
     st.kern = Kern::Rows;
     RowArgs &r = st.rows;
     std::vector<CB> rb;
     for (auto &c : cbits)
       if (c.wa) rb.push_back(c);  // H and M bits, already in C order
     r.nbr = (int)rb.size();
+    for (int i = 0; i < std::min(8, r.nbr); ++i)
+      if (rb[i].wb) r.hlo |= 1 << i;
     for (int i = 0; i < r.nbr; ++i) r.ra[i] = rb[i].wa, r.rb[i] = rb[i].wb, r.rc[i] = rb[i].wc;
     r.nk = 1 << nk;
     r.nn = 1 << nn;

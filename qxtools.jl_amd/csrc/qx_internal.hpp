@@ -93,10 +93,11 @@ struct BitArgs {
 // (sorted by C weight), k over K bits, n over N bits; nk = 2^nbk, nn = 2^nbn small.
 struct RowArgs {
   int64_t ra[kMaxBits], rb[kMaxBits], rc[kMaxBits];   // row bits: weights in A, B (H only), C
-  int64_t ka[8], kb[8];                               // per k value (not per bit)
-  int64_t nb[8], nc[8];                               // per n value
+  int64_t ka[64], kb[64];                             // per k value (not per bit)
+  int64_t nb[64], nc[64];                             // per n value
   int nbr = 0, nk = 1, nn = 1;
   int conj_a = 0, conj_b = 0;
+  int hlo = 0;                 // row bits among the low 8 that are H bits (bit mask)
 };
 
 struct RadixArgs {

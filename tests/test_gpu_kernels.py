@@ -58,8 +58,8 @@ def err(a, r):
     return np.linalg.norm(a - r) / np.linalg.norm(r)
 
 
-GEMM_CASES = [(0, 5, 3, 3), (0, 7, 4, 4), (0, 8, 5, 5), (1, 6, 4, 5), (2, 6, 5, 3), (0, 6, 6, 6),
-              (0, 9, 7, 4), (1, 5, 8, 6), (0, 12, 5, 5), (0, 7, 3, 8)]
+GEMM_CASES = [(0, 12, 5, 5), (0, 5, 3, 3), (0, 7, 4, 4), (0, 8, 5, 5), (1, 6, 4, 5), (2, 6, 5, 3), (0, 6, 6, 6),
+              (0, 9, 7, 4), (1, 5, 8, 6), (0, 12, 7, 5), (0, 7, 3, 8)]
 
 
 @pytest.mark.parametrize("case", range(len(GEMM_CASES)))
@@ -92,7 +92,8 @@ def test_bits_kernel(shape):
 
 
 @pytest.mark.parametrize("shape", [(0, 14, 1, 1), (0, 16, 2, 2), (2, 12, 3, 3), (1, 13, 0, 2), (0, 12, 3, 0),
-                                   (3, 10, 2, 1)])
+                                   (3, 10, 2, 1), (0, 14, 3, 3), (1, 12, 3, 2), (0, 14, 5, 5),
+                                   (1, 12, 4, 4), (0, 13, 6, 4), (2, 11, 5, 4), (3, 12, 2, 3)])
 def test_rows_kernel(shape):
     nh, nm, nn, nk = shape
     dsl, data, ref = single_step(13 + sum(shape), nh, nm, nn, nk, c_rows_first=True)
