@@ -38,7 +38,7 @@ def kernel_summary(stats: dict, prec: str) -> dict:
         calls = st.get("calls", 0)
         if not calls:
             continue
-        f = fam.setdefault(st["kernel"], {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
+        f = fam.setdefault(st["variant"], {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
         f["launches"] += calls
         f["ms"] += st["ms"]
         f["bytes"] += st["bytes"] * calls
@@ -130,7 +130,7 @@ def main() -> int:
         ach = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
                 "frac": ach / PEAK_TFLOPS[prec]}
-    roof.update({"traffic": None, "kernel": f"contract_{dom_name}", "launches": dom["launches"],
+    roof.update({"traffic": None, "kernel": dom_name, "launches": dom["launches"],
                  "avg_launch_us": 1e3 * dom["ms"] / dom["launches"],
                  "algorithmic_bytes_per_launch": dom["bytes"] / dom["launches"],
                  "algorithmic_flops_per_launch": dom["flops"] / dom["launches"]})

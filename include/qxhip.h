@@ -122,6 +122,13 @@ qx_status qx_plan_stats(const qx_plan *plan, char *json, size_t cap);
 
 void qx_plan_destroy(qx_plan *plan);
 
+/* Parse and lower a .qx program without a device and write the same JSON as
+ * qx_plan_stats (steps, kernel choice, arena size) plus each arena value's offset
+ * and live interval.  For planning tools and host-side checks of the reuse
+ * schedule; no reference counterpart. */
+qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t batch, char *json,
+                         size_t cap);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *qx_last_error(void);
 

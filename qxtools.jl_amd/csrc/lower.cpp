@@ -380,7 +380,10 @@ Lowered lower(const Program &p, int64_t batch) {
       Z.push_back(L.values[i].elems);
     }
   L.arena_elems = first_fit(S, E, Z, 8, off);
-  for (size_t j = 0; j < arena_nodes.size(); ++j) L.values[arena_nodes[j]].slot = off[j];
+  for (size_t j = 0; j < arena_nodes.size(); ++j) {
+    L.values[arena_nodes[j]].slot = off[j];
+    L.slots.push_back({arena_nodes[j], off[j], Z[j], S[j], E[j]});
+  }
   // propagate slots to views and save
   for (size_t i = 0; i < nn; ++i)
     if (root[i] >= 0 && root[i] != (int)i) L.values[i].slot = L.values[root[i]].slot;

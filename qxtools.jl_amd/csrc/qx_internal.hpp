@@ -144,8 +144,14 @@ struct Step {
   RadixArgs radix;         // Kern::Radix
 };
 
+struct ArenaSlot {
+  int node;
+  int64_t offset, elems, start, end;   // element offset and size, live [start, end)
+};
+
 struct Lowered {
   int64_t batch = 0;
+  std::vector<ArenaSlot> slots;     // arena reuse schedule
   std::vector<Value> values;        // per node
   std::vector<Step> steps;          // ncon nodes in order
   int64_t arena_elems = 0;          // forward arena size (elements)

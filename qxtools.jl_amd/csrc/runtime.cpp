@@ -273,6 +273,51 @@ const char *kern_name(qx::Kern k) {
   }
 }
 
+// Name of the kernel instantiation a step launches, as rocprofv3 reports it
+// (template arguments after the precision).
+std::string kern_variant(const qx::Step &st) {
+  switch (st.kern) {
+    case qx::Kern::Rows: {
+      int nc = st.rows.nn >= 8 ? 8 : st.rows.nn;
+      return "k_contract_rows<" + std::to_string(st.rows.nk) + "," + std::to_string(nc) + ">";
+    }
+    case qx::Kern::Gemm:
+      return st.gemm.tile == 0 ? "k_contract_gemm<128,16,16>" : st.gemm.tile == 1 ? "k_contract_gemm<128,32,16>"
+                                                                                   : "k_contract_gemm<64,64,16>";
+    case qx::Kern::Reduce: return "k_contract_reduce";
+    case qx::Kern::Radix: return "k_contract_radix";
+    default: return "k_contract_bits";
+  }
+}
+
+void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Lowered &L, size_t es,
+                  bool with_slots) {
+  os << "{\"batch\":" << L.batch << ",\"arena_bytes\":" << L.arena_elems * (int64_t)es
+     << ",\"result_elems\":" << L.result_elems;
+  if (with_slots) {
+    os << ",\"slots\":[";
+    for (size_t i = 0; i < L.slots.size(); ++i) {
+      const auto &s = L.slots[i];
+      if (i) os << ",";
+      os << "{\"name\":\"" << prog.nodes[s.node].name << "\",\"offset\":" << s.offset << ",\"elems\":" << s.elems
+         << ",\"start\":" << s.start << ",\"end\":" << s.end << "}";
+    }
+    os << "]";
+  }
+  os << ",\"steps\":[";
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    const qx::Step &st = L.steps[s];
+    if (s) os << ",";
+    double flops = 8.0 * st.H * st.M * st.N * st.K;
+    double bytes = (double)es * st.H * (st.M * st.K + st.K * st.N + st.M * st.N);
+    os << "{\"name\":\"" << prog.nodes[st.node].name << "\",\"section\":" << st.section << ",\"B\":" << st.H
+       << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K << ",\"flops\":" << flops
+       << ",\"bytes\":" << bytes << ",\"kernel\":\"" << kern_name(st.kern) << "\",\"variant\":\""
+       << kern_variant(st) << "\"}";
+  }
+  os << "]}";
+}
+
 std::string stats_json(const qx_plan *P) {
   std::ostringstream os;
   os.precision(17);
@@ -292,7 +337,7 @@ std::string stats_json(const qx_plan *P) {
       os << "{\"name\":\"" << P->prog.nodes[st.node].name << "\",\"section\":" << st.section
          << ",\"B\":" << st.H << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K
          << ",\"flops\":" << flops << ",\"bytes\":" << bytes << ",\"kernel\":\""
-         << kern_name(st.kern) << "\"";
+         << kern_name(st.kern) << "\",\"variant\":\"" << kern_variant(st) << "\"";
       if (P->opt_profile && L.batch == P->lowered.rbegin()->first && s < P->step_ms.size())
         os << ",\"ms\":" << P->step_ms[s] << ",\"calls\":" << P->step_calls[s];
       os << "}";
@@ -486,5 +531,20 @@ void qx_plan_destroy(qx_plan *plan) {
 }
 
 const char *qx_last_error(void) { return g_last_error.c_str(); }
+
+qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t batch, char *json, size_t cap) {
+  return guard([&] {
+    if (!json || cap == 0) throw qx::Error(QX_ERR_ARGUMENT, "null output buffer");
+    if (precision < QX_C128 || precision > QX_C64_BF16X3) throw qx::Error(QX_ERR_ARGUMENT, "unknown precision");
+    qx::Program prog = qx::parse_program(dsl, len);
+    qx::Lowered L = qx::lower(prog, batch);
+    std::ostringstream os;
+    os.precision(17);
+    lowered_json(os, prog, L, precision == QX_C128 ? 16 : 8, true);
+    std::string s = os.str();
+    if (s.size() + 1 > cap) throw qx::Error(QX_ERR_ARGUMENT, "inspect needs " + std::to_string(s.size() + 1) + " bytes");
+    std::memcpy(json, s.c_str(), s.size() + 1);
+  });
+}
 
 }  // extern "C"

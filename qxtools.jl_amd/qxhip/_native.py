@@ -19,6 +19,7 @@ SYMBOLS = [
     "qx_ctx_create", "qx_ctx_destroy", "qx_plan_from_dsl", "qx_plan_set_tensor",
     "qx_plan_set_option", "qx_plan_finalize", "qx_plan_info", "qx_plan_key",
     "qx_amplitudes", "qx_amplitudes_vjp", "qx_plan_stats", "qx_plan_destroy", "qx_last_error",
+    "qx_dsl_inspect",
 ]
 
 
@@ -53,6 +54,7 @@ def lib() -> C.CDLL:
         L.qx_plan_stats.argtypes = [p, C.c_char_p, sz]
         L.qx_plan_destroy.argtypes = [p]
         L.qx_plan_destroy.restype = None
+        L.qx_dsl_inspect.argtypes = [C.c_char_p, sz, C.c_int, i64, C.c_char_p, sz]
         L.qx_last_error.argtypes = []
         L.qx_last_error.restype = C.c_char_p
         for f in SYMBOLS:
@@ -69,6 +71,20 @@ def _check(status: int) -> None:
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def inspect(dsl: str, precision: str = "c128", batch: int = 1) -> dict:
+    """Lower a program on the host (no device): steps, kernels, arena schedule."""
+    b = dsl.encode()
+    cap = 1 << 16
+    while True:
+        buf = C.create_string_buffer(cap)
+        st = lib().qx_dsl_inspect(b, len(b), PRECISIONS[precision], batch, buf, cap)
+        if st == 0:
+            return json.loads(buf.value.decode())
+        if st != -1 or cap > (1 << 28):
+            _check(st)
+        cap *= 4
 
 
 class Context:

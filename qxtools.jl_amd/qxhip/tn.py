@@ -177,17 +177,7 @@ class TensorNetworkCircuit:
         # axes [o_1..o_k, i_1..i_k]; first qubit fastest (column-major reshape)
         t = np.asarray(mat, dtype=np.complex128).reshape((2,) * (2 * k), order="F")
         if decompose and k == 2:
-            # (o1,i1) x (o2,i2) matrix -> U S V^H, split sqrt(S) to each side
-            m = t.transpose(0, 2, 1, 3).reshape(4, 4, order="F")
-            u, s, vh = np.linalg.svd(m)
-            r = int(np.sum(s > 1e-12 * s[0]))
-            sq = np.sqrt(s[:r])
-            a = (u[:, :r] * sq).reshape(2, 2, r, order="F")            # [o1, i1, bond]
-            b = (sq[:, None] * vh[:r, :]).T.reshape(2, 2, r, order="F")  # [o2, i2, bond]
-            bond = self.tn.new_index(r)
-            names = [self._push_factor(qubits[0], a, bond), self._push_factor(qubits[1], b, bond)]
-            self.gate_tensors.append((names, list(qubits)))
-            return names
+            return self._push_decomposed(qubits, t)
         legs_out, legs_in, keep = [], [], []
         diag = [_diagonal_in(t, j, k + j) for j in range(k)]
         data = t
@@ -210,15 +200,52 @@ class TensorNetworkCircuit:
         self.gate_tensors.append(([name], list(qubits)))
         return [name]
 
-    def _push_factor(self, q: int, f: np.ndarray, bond: int) -> str:
-        """One SVD factor [o, i, bond] of a decomposed two-qubit gate."""
-        w = self.output_indices[q - 1]
-        if _diagonal_in(f, 0, 1):
-            return self.tn.push([w, bond], _take_diagonal(f, 0, 1))
-        o = self.tn.new_index(2)
-        name = self.tn.push([o, w, bond], f)
-        self.output_indices[q - 1] = o
-        return name
+    def _push_decomposed(self, qubits: list, t: np.ndarray) -> list:
+        """Split a two-qubit gate [o1,o2,i1,i2] by SVD into factors [o1,i1,bond] and
+        [o2,i2,bond] (sqrt of the singular values to each side).  Each factor that is
+        diagonal in (out, in) keeps its wire as one hyper leg; a factor that is then
+        also diagonal between that wire and the bond is a copy tensor, so the bond is
+        identified with the wire: the factor becomes a rank-1 tensor on the wire and
+        the other factor carries the wire in place of the bond.  The CX control side
+        is such a factor, which gives the hyperedge census of the reference's test
+        (test/test_contraction_planning.jl:39-44)."""
+        m = t.transpose(0, 2, 1, 3).reshape(4, 4, order="F")
+        u, s, vh = np.linalg.svd(m)
+        r = int(np.sum(s > 1e-12 * s[0]))
+        sq = np.sqrt(s[:r])
+        fa = (u[:, :r] * sq).reshape(2, 2, r, order="F")             # [o1, i1, bond]
+        fb = (sq[:, None] * vh[:r, :]).T.reshape(2, 2, r, order="F")   # [o2, i2, bond]
+        parts = []
+        for q, f in ((qubits[0], fa), (qubits[1], fb)):
+            w = self.output_indices[q - 1]
+            if _diagonal_in(f, 0, 1):
+                parts.append((q, _take_diagonal(f, 0, 1), [w], None))     # [w, bond]
+            else:
+                parts.append((q, f, [None, w], w))                        # [o, w, bond]
+        bond = None
+        for j, (q, f, legs, _) in enumerate(parts):
+            if legs == [self.output_indices[q - 1]] and f.shape[1] == f.shape[0] and _diagonal_in(f, 0, 1):
+                bond = legs[0]
+                parts[j] = (q, _take_diagonal(f, 0, 1), legs, "copy")    # rank-1 on the wire
+                break
+        if bond is None:
+            bond = self.new_index(r)
+        names = []
+        for q, f, legs, kind in parts:
+            if kind == "copy":
+                names.append(self.tn.push(legs, f))
+                continue
+            if legs[0] is None:
+                o = self.new_index(2)
+                names.append(self.tn.push([o, legs[1], bond], f))
+                self.output_indices[q - 1] = o
+            else:
+                names.append(self.tn.push(legs + [bond], f))
+        self.gate_tensors.append((names, list(qubits)))
+        return names
+
+    def new_index(self, dim: int) -> int:
+        return self.tn.new_index(dim)
 
     # -- boundary tensors ----------------------------------------------------------
     def add_input(self, bits: str | None = None) -> None:

@@ -92,7 +92,7 @@ def test_bits_kernel(shape):
 
 
 @pytest.mark.parametrize("shape", [(0, 14, 1, 1), (0, 16, 2, 2), (2, 12, 3, 3), (1, 13, 0, 2), (0, 12, 3, 0),
-                                   (3, 10, 2, 1), (0, 14, 3, 3), (1, 12, 3, 2), (0, 14, 5, 5),
+                                   (3, 10, 2, 1), (0, 14, 3, 3), (1, 12, 3, 2), (0, 14, 5, 4),
                                    (1, 12, 4, 4), (0, 13, 6, 4), (2, 11, 5, 4), (3, 12, 2, 3)])
 def test_rows_kernel(shape):
     nh, nm, nn, nk = shape

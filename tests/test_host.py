@@ -1,0 +1,187 @@
+"""Host-side parity with the reference's own unit tests (no GPU)."""
+
+import numpy as np
+import pytest
+
+import qxoracle as O
+import qxtest
+from qxhip import (TensorCache, build_compute_graph, circuits, compute_graph, convert_to_tnc, planning,
+                   simulation, workloads)
+
+
+# ------------------------------------------------------------------ test/test_tn_conversion.jl:6-28
+def test_tn_conversion_counts():
+    circ = circuits.create_test_circuit()
+    tnc = convert_to_tnc(circ, no_input=True, no_output=True, decompose=False)
+    assert tnc.qubits() == 3 and len(tnc) == 3
+    assert all(i != o for i, o in zip(tnc.input_indices, tnc.output_indices))
+    tnc = convert_to_tnc(circ, no_input=True, no_output=True, decompose=True)
+    assert len(tnc) == 5                        # each CX split in two by SVD
+    tnc = convert_to_tnc(circ, no_input=False, no_output=True, decompose=False)
+    assert len(tnc) == 6
+    first = tnc.tensor_data(tnc.keys()[0])
+    assert np.array_equal(first, circuits.gate_matrix(circ.ops[0]))
+
+
+def test_hyperedges_of_decomposed_ghz3():
+    """Decomposed GHZ-3 with inputs and outputs: the hyperedge census is
+    {4 tensors: 2 edges, 2 tensors: 4 edges} (test/test_contraction_planning.jl:39-44)."""
+    tnc = convert_to_tnc(circuits.create_test_circuit(), decompose=True)
+    holders = tnc.tn.index_holders()
+    census = {}
+    for i, h in holders.items():
+        census[len(h)] = census.get(len(h), 0) + 1
+    assert census == {4: 2, 2: 4}
+    assert len(planning.line_graph(tnc)) == 6
+
+
+# ------------------------------------------------------------------ test/test_compute_graph.jl:5-24
+def test_tensor_cache_dedupe_thresholds():
+    tc = TensorCache()
+    rng = np.random.default_rng(0)
+    a = rng.random((3, 4, 5))
+    sym = tc.push(a)
+    assert sym == tc.push(a)
+    assert sym != tc.push(rng.random((3, 4, 5)))
+    assert sym == tc.push(a + 0.1 * np.finfo(float).eps)
+    assert sym != tc.push(a + 1.1 * np.finfo(float).eps)
+    assert np.array_equal(tc[sym], a)
+    assert len(tc) == 3
+    assert sym == "data_1"
+    with pytest.raises(KeyError):
+        tc["data_99"]
+
+
+def test_compute_graph_node_count():
+    """nodes = tensors + plan + 1 save (test/test_compute_graph.jl:26-34), plus one
+    node per view when slicing."""
+    circ = circuits.create_test_circuit()
+    tnc = convert_to_tnc(circ)
+    plan = planning.contraction_plan(tnc, trials=2)
+    g = build_compute_graph(tnc, plan)
+    assert len(g) == len(tnc) + len(plan) + 1
+    groups = planning.expand_slice_bonds_to_hyperindices(tnc, planning.greedy_slicing(tnc, plan, 2))
+    g2 = build_compute_graph(tnc, plan, groups)
+    views = sum(1 for c in g2.commands if c.kind == "view")
+    assert views >= 2 and len(g2) == len(tnc) + len(plan) + 1 + views
+
+
+# ------------------------------------------------------------------ test/test_contraction_planning.jl
+@pytest.mark.parametrize("decompose", [False, True])
+def test_plan_lengths_and_intermediates(decompose):
+    tnc = convert_to_tnc(circuits.create_test_circuit(), decompose=decompose)
+    for plan in (planning.contraction_plan(tnc, trials=2),
+                 planning.order_to_contraction_plan(planning.elimination_order(planning.line_graph(tnc), 0), tnc),
+                 planning.sweep_contraction_plan(tnc)):
+        assert len(plan) == len(tnc) - 1
+        produced = [c for _, _, c in plan]
+        assert all(c.startswith("I") for c in produced) and len(set(produced)) == len(produced)
+        consumed = [x for a, b, _ in plan for x in (a, b)]
+        assert len(set(consumed)) == len(consumed)          # each tensor used once
+
+
+def test_forty_z_gates_hyperedge_plan():
+    """One qubit, 40 Z gates: one big hyperedge; the plan still contracts all 42
+    tensors (test/test_contraction_planning.jl:90-98)."""
+    c = circuits.Circuit(1)
+    for _ in range(40):
+        c.add("z", 1)
+    tnc = convert_to_tnc(c)
+    assert len(tnc) == 42
+    plan = planning.contraction_plan(tnc, trials=2)
+    assert len(plan) == 41
+    g = build_compute_graph(tnc, plan)
+    amps = O.amplitudes(g.to_dsl(), g.data, ["0", "1"])
+    assert np.allclose(amps, [1, 0])
+
+
+def test_slicing_scheme_groups_and_metadata():
+    tnc = convert_to_tnc(circuits.create_rqc_circuit(3, 3, 8, seed=1))
+    groups, plan, meta = planning.contraction_scheme(tnc, 3, trials=4)
+    assert len(groups) == 3 and all(len(g) == 1 for g in groups)
+    widths = meta["Slicing"]["log2 largest intermediate after slicing consecutive edges"]
+    assert widths == sorted(widths, reverse=True)
+    assert len(plan) == len(tnc) - 1
+
+
+# ------------------------------------------------------------------ DSL
+def test_dsl_round_trip_and_labels():
+    tnc = convert_to_tnc(circuits.create_rqc_circuit(2, 3, 6, seed=2))
+    groups, plan, meta = planning.contraction_scheme(tnc, 2, trials=2)
+    g = build_compute_graph(tnc, plan, groups, metadata=meta)
+    text = g.to_dsl()
+    assert text.startswith("# version: 0.4.0\n")
+    cmds = compute_graph.parse_dsl(text)
+    assert [(c.kind, c.name, tuple(map(str, c.args))) for c in cmds] == \
+        [(c.kind, c.name, tuple(map(str, c.args))) for c in g.commands]
+    for c in cmds:
+        if c.kind == "ncon":
+            cl, a, al, b, bl = c.args
+            # labels numbered by first appearance in A then B; C = survivors of A then new of B
+            seen = []
+            for x in al + bl:
+                if x not in seen:
+                    seen.append(x)
+            assert seen == list(range(1, len(seen) + 1))
+            assert cl == [x for x in al if x in cl] + [x for x in bl if x in cl and x not in al]
+
+
+def test_outputs_and_views_in_dsl():
+    tnc = convert_to_tnc(circuits.create_ghz_circuit(4))
+    groups, plan, meta = planning.contraction_scheme(tnc, 1, trials=2)
+    g = build_compute_graph(tnc, plan, groups)
+    outs = [c for c in g.commands if c.kind == "output"]
+    assert [c.args for c in outs] == [(q, 2) for q in range(1, 5)]
+    views = [c for c in g.commands if c.kind == "view"]
+    assert views and all(c.args[1] == "v1" and c.args[3] == 2 for c in views)
+
+
+# ------------------------------------------------------------------ test/test_simulation.jl:3-14
+def test_bitstring_generators():
+    allb = list(simulation.amplitudes_all(5))
+    assert len(allb) == 32 and all(len(b) == 5 for b in allb)
+    assert allb[1] == "00001" and allb[-1] == "11111"       # most significant first
+    u = simulation.amplitudes_uniform(5, None, 10)
+    assert len(u) == 10 and all(set(b) <= {"0", "1"} for b in u)
+    assert simulation.amplitudes_uniform(7, 3, 4) == simulation.amplitudes_uniform(7, 3, 4)
+
+
+def test_output_params_dict():
+    p = simulation.output_params_dict(5, 3, "List", bitstrings=["111", "101"])
+    assert p["method"] == "List" and p["params"]["num_samples"] == 2
+    p = simulation.output_params_dict(5, 20, "Rejection", seed=42)
+    assert p["params"]["num_samples"] == 20 and p["params"]["M"] == 0.0001
+    with pytest.raises(ValueError):
+        simulation.output_params_dict(5, 1, "Bogus")
+
+
+def test_generate_and_load_simulation_files(tmp_path):
+    circ = circuits.create_rqc_circuit(2, 2, 4, seed=3)
+    prefix = str(tmp_path / "rqc")
+    simulation.generate_simulation_files(circ, prefix, 1, output_args=simulation.output_params_dict(4, 5, seed=1))
+    dsl, data, params = simulation.load_simulation_files(prefix)
+    assert params["output"]["params"]["num_samples"] == 5
+    bits = params["output"]["params"]["bitstrings"]
+    amps = O.amplitudes(dsl, data, bits)
+    sv = qxtest.statevector_amps(circ, bits)
+    assert np.allclose(amps, sv, atol=1e-14)
+
+
+def test_committed_workloads_match_their_generator():
+    """The saved plans under workloads/ are what their generator writes."""
+    for name in ("rqc_3x3_d16", "rqc_5x5_d20"):
+        w = workloads.WORKLOADS[name]
+        dsl, data, bits = w.load()
+        assert bits == w.bitstrings()
+        tnc = convert_to_tnc(w.circuit())
+        plan = planning.sweep_contraction_plan(tnc)
+        body = lambda t: [ln for ln in t.splitlines() if ln and not ln.startswith("#")]
+        assert body(build_compute_graph(tnc, plan).to_dsl()) == body(dsl)
+
+
+def test_config1_workload_oracle_vs_statevector():
+    w = workloads.WORKLOADS["rqc_3x3_d16"]
+    dsl, data, bits = w.load()
+    amps = O.amplitudes(dsl, data, bits[:16])
+    sv = qxtest.statevector_amps(w.circuit(), bits[:16])
+    assert np.max(np.abs(amps - sv) / np.abs(sv)) < 1e-12

@@ -1,0 +1,136 @@
+"""Host-side checks of libqxhip (no GPU): symbols, DSL validation and error
+statuses, kernel choice, and the arena reuse schedule (no two live values overlap)."""
+
+import ctypes
+import re
+
+import pytest
+
+import test_gpu_kernels as K
+from qxhip import _native, workloads
+
+HEADER = __import__("os").path.join(__import__("os").path.dirname(__file__), "..", "include", "qxhip.h")
+
+
+def test_library_exports_every_declared_symbol():
+    text = open(HEADER).read()
+    declared = set(re.findall(r"\b(qx_[a-z_]+)\s*\(", text))
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(_native.SYMBOLS) <= declared
+
+
+def _status(dsl):
+    try:
+        _native.inspect(dsl)
+        return 0
+    except _native.QxError as e:
+        return e.status
+
+
+BAD = {
+    "unknown op": ("load A data_1 2\nfoo A\nsave output A\n", -2),
+    "no save": ("load A data_1 2\n", -2),
+    "bad dim": ("load A data_1 2,x\nsave output A\n", -2),
+    "undefined input": ("load A data_1 2\nncon C 1 A 1 B 1\nsave output C\n", -2),
+    "redefined": ("load A data_1 2\nload A data_2 2\nsave output A\n", -2),
+    "repeated label": ("load A data_1 2,2\nload B data_2 2\nncon C 1 A 1,1 B 1\nsave output C\n", -3),
+    "sum-out of A alone": ("load A data_1 2,2\nload B data_2 2\nncon C 1 A 1,2 B 1\nsave output C\n", -3),
+    "label in neither": ("load A data_1 2\nload B data_2 2\nncon C 1,3 A 1 B 1\nsave output C\n", -3),
+    "dim mismatch": ("load A data_1 2\nload B data_2 3\nncon C 0 A 1 B 1\nsave output C\n", -3),
+    "rank mismatch": ("load A data_1 2,2\nload B data_2 2\nncon C 1 A 1 B 1\nsave output C\n", -3),
+    "view dim": ("load A data_1 2\nview A_s A v1 1 3\nsave output A_s\n", -2),
+    "view leg": ("load A data_1 2\nview A_s A v1 2 2\nsave output A_s\n", -2),
+    "output dim": ("output o 1 3\nsave output o\n", -2),
+    "key sizes": ("load A data_1 2\nload B data_1 2,2\nsave output A\n", -2),
+}
+
+
+@pytest.mark.parametrize("case", sorted(BAD))
+def test_malformed_programs_return_status(case):
+    dsl, code = BAD[case]
+    assert _status(dsl) == code
+
+
+def test_dsl_example_from_users_guide_lowers():
+    """The two-qubit GHZ example of docs/src/users_guide.md:71-90 (views, hyper
+    labels, a scalar result) is accepted."""
+    dsl = """# version: 0.4.0
+load t5 data_1 2
+view t5_s t5 v2 1 2
+load t1 data_2 2,2
+view t1_s t1 v1 1 2
+load t4 data_1 2
+ncon I2 1 t1_s 1,2 t4 2
+ncon t8 1,2 t5_s 1 I2 2
+load t3 data_4 2
+view t3_s t3 v1 1 2
+ncon t9 1,2 t8 1,2 t3_s 2
+output t6 1 2
+view t6_s t6 v1 1 2
+ncon t10 1,2 t9 1,2 t6_s 2
+load t2 data_3 2,2,2
+view t2_s t2 v1 3 2
+view t2_s_s t2_s v2 2 2
+output t7 2 2
+ncon I1 2,3 t2_s_s 1,2,3 t7 1
+ncon t11 0 t10 1,2 I1 1,2
+save output t11
+"""
+    info = _native.inspect(dsl, batch=4)
+    names = [s["name"] for s in info["steps"]]
+    assert names == ["I2", "t8", "t9", "t10", "I1", "t11"]
+    assert info["result_elems"] == 1
+
+
+def _check_schedule(info):
+    slots = info["slots"]
+    for i, a in enumerate(slots):
+        assert a["offset"] % 16 == 0
+        assert a["offset"] + a["elems"] <= info["arena_bytes"] // (16 if info.get("c128", True) else 8)
+        for b in slots[i + 1:]:
+            live = a["start"] < b["end"] and b["start"] < a["end"]
+            disjoint = a["offset"] + a["elems"] <= b["offset"] or b["offset"] + b["elems"] <= a["offset"]
+            assert not live or disjoint, (a, b)
+
+
+@pytest.mark.parametrize("name", ["rqc_3x3_d16", "rqc_5x5_d20"])
+@pytest.mark.parametrize("batch", [1, 8, 256])
+def test_arena_schedule_has_no_overlap(name, batch):
+    dsl, data, bits = workloads.WORKLOADS[name].load()
+    info = _native.inspect(dsl, batch=batch)
+    _check_schedule(info)
+    peak_live = 0
+    ends = sorted({s["start"] for s in info["slots"]})
+    for t in ends:
+        peak_live = max(peak_live, sum(s["elems"] for s in info["slots"] if s["start"] <= t < s["end"]))
+    assert info["arena_bytes"] >= 16 * peak_live
+    # reuse must make the arena far smaller than the sum of all intermediates
+    assert info["arena_bytes"] < 16 * sum(s["elems"] for s in info["slots"])
+
+
+def test_sections_split_shared_and_bitstring_work():
+    dsl, data, bits = workloads.WORKLOADS["rqc_5x5_d20"].load()
+    info = _native.inspect(dsl, batch=256)
+    secs = [s["section"] for s in info["steps"]]
+    assert secs.count(2) < 40 < secs.count(0)   # outputs absorbed last by the sweep plan
+    big = [s for s in info["steps"] if s["section"] == 2 and s["K"] >= 4096]
+    assert big and big[0]["N"] == 256 and big[0]["kernel"] == "gemm"
+
+
+def test_kernel_choice_matches_gpu_test_shapes():
+    for case in K.GEMM_CASES:
+        dsl, _, _ = K.single_step(1, *case)
+        assert _native.inspect(dsl)["steps"][0]["kernel"] == "gemm", case
+    rows = [(0, 14, 1, 1), (0, 16, 2, 2), (2, 12, 3, 3), (1, 13, 0, 2), (0, 12, 3, 0), (3, 10, 2, 1),
+            (0, 14, 3, 3), (1, 12, 3, 2), (0, 14, 5, 4), (1, 12, 4, 4), (0, 13, 6, 4), (2, 11, 5, 4),
+            (3, 12, 2, 3)]
+    for shape in rows:
+        dsl, _, _ = K.single_step(13 + sum(shape), *shape, c_rows_first=True)
+        assert _native.inspect(dsl)["steps"][0]["kernel"] == "rows", shape
+    for shape in [(0, 0, 0, 10), (3, 0, 0, 9), (0, 2, 2, 8), (1, 3, 0, 12), (6, 0, 0, 8)]:
+        dsl, _, _ = K.single_step(7, *shape)
+        assert _native.inspect(dsl)["steps"][0]["kernel"] == "reduce", shape
+    dsl, _, _ = K.single_step(5, 1, 2, 2, 2, dims={1: 3, 2: 5, 5: 3, 6: 7})
+    assert _native.inspect(dsl)["steps"][0]["kernel"] == "radix"
