@@ -117,14 +117,15 @@ def _fetch(data: dict, key: str, dims: tuple) -> np.ndarray:
     return a
 
 
-def contract_slice(cmds: list, data: dict, bits: str, assign: dict) -> np.ndarray:
-    """Evaluate the DAG for one bitstring and one slice assignment."""
+def contract_slice(cmds: list, data: dict, bits: str, assign: dict, override: dict | None = None) -> np.ndarray:
+    """Evaluate the DAG for one bitstring and one slice assignment.  `override`
+    replaces the tensor of named load nodes (used for environment tensors)."""
     env: dict = {}
     result = None
     for op, f in cmds:
         if op == "load":
             dims = tuple(int(x) for x in f[2].split(",")) if len(f) > 2 else ()
-            env[f[0]] = _fetch(data, f[1], dims)
+            env[f[0]] = override[f[0]] if override and f[0] in override else _fetch(data, f[1], dims)
         elif op == "output":
             q, d = int(f[1]), int(f[2])
             if d != 2:
@@ -181,6 +182,34 @@ def amplitudes(dsl: str, data: dict, bitstrings: list, slices=None) -> np.ndarra
             acc.add(r)
         out.append(acc.value())
     return np.array(out)
+
+
+def env_gradient(dsl: str, data: dict, bitstrings: list, amp_cot: np.ndarray) -> dict:
+    """Reference gradient by environment tensors: the amplitude is linear in each
+    load node's tensor, so d amp / d T_node[i] is the amplitude with that node's
+    tensor replaced by the i-th basis tensor.  Returns, per data key,
+    sum_b amp_cot_b conj(d amp_b / d T_key) summed over the nodes loading the key
+    and over all slices.  Cost: one contraction per (node, element, bitstring,
+    slice) -- small networks only."""
+    cmds = parse(dsl)
+    assigns = list(slice_assignments(cmds))
+    cot = np.asarray(amp_cot, dtype=np.complex128).reshape(len(bitstrings), -1)
+    out: dict = {}
+    for op, f in cmds:
+        if op != "load":
+            continue
+        name, key = f[0], f[1]
+        dims = tuple(int(x) for x in f[2].split(",")) if len(f) > 2 else ()
+        n = int(np.prod(dims, dtype=np.int64)) if dims else 1
+        g = out.setdefault(key, np.zeros(n, dtype=np.complex128))
+        for i in range(n):
+            e = np.zeros(n, dtype=np.complex128)
+            e[i] = 1.0
+            ov = {name: e.reshape(dims, order="F") if dims else e.reshape(())}
+            for b, bits in enumerate(bitstrings):
+                d = sum(contract_slice(cmds, data, bits, a, ov) for a in assigns)
+                g[i] += np.sum(cot[b] * np.conj(np.asarray(d).reshape(-1, order="F")))
+    return out
 
 
 def per_slice(dsl: str, data: dict, bits: str) -> list:

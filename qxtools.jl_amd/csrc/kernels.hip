@@ -10,6 +10,8 @@
 // every bitstring-dependent tensor, guarantees it for batched steps).
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "qx_internal.hpp"
 
 namespace qx {
@@ -436,6 +438,64 @@ __global__ __launch_bounds__(256) void k_contract_rows(const typename Vec<T>::ty
   }
 }
 
+
+// dst[map(e)] += src[e]: src dense over dims (first fastest), map(e) = sum digit*stride.
+struct AddArgs {
+  int64_t dim[kMaxLegs], st[kMaxLegs];
+  int n = 0;
+};
+
+template <typename TS, typename TD>
+__global__ void k_add_mapped(typename Vec<TD>::type *__restrict__ dst, const typename Vec<TS>::type *__restrict__ src,
+                             int64_t n, AddArgs a) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = e, o = 0;
+    for (int j = 0; j < a.n; ++j) {
+      int64_t d = r % a.dim[j];
+      r /= a.dim[j];
+      o += d * a.st[j];
+    }
+    auto v = src[e];
+    auto w = dst[o];
+    w.x += (TD)v.x;
+    w.y += (TD)v.y;
+    dst[o] = w;
+  }
+}
+
+template <typename T>
+__global__ void k_root_cot(typename Vec<T>::type *__restrict__ dst, int dep, int64_t batch, int64_t nvalid,
+                           int64_t relems, const int64_t *__restrict__ relem_off, const double *__restrict__ amp,
+                           int64_t row0) {
+  using V = typename Vec<T>::type;
+  if (dep) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < batch * relems;
+         t += (int64_t)gridDim.x * blockDim.x) {
+      int64_t beta = t % batch, e = t / batch;
+      V v{0, 0};
+      if (beta < nvalid) {
+        const double *p = amp + 2 * ((row0 + beta) * relems + e);
+        v = V{(T)p[0], (T)p[1]};
+      }
+      dst[relem_off[e] + beta] = v;
+    }
+  } else {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < relems;
+         e += (int64_t)gridDim.x * blockDim.x) {
+      double re = 0, im = 0;
+      for (int64_t beta = 0; beta < nvalid; ++beta) {
+        const double *p = amp + 2 * ((row0 + beta) * relems + e);
+        re += p[0];
+        im += p[1];
+      }
+      V w = dst[relem_off[e]];
+      w.x += (T)re;
+      w.y += (T)im;
+      dst[relem_off[e]] = w;
+    }
+  }
+}
+
 // Output tensors <b_q| for a batch: O_i[beta + batch*x], beta fastest.
 template <typename T>
 __global__ void k_fill_outputs(typename Vec<T>::type *arena, const int64_t *__restrict__ out_off, int n_out,
@@ -648,6 +708,41 @@ void launch_accumulate(int prec, const void *root, int root_dep, int64_t batch, 
     hipLaunchKernelGGL(k_accumulate<float>, dim3(g), dim3(256), 0, s, (const float2 *)root, root_dep, batch,
                        nvalid, relems, d_relem_off, acc, row0);
   check_launch("accumulate");
+}
+
+
+void launch_add_mapped(int prec, int dst_double, void *dst, const void *src, int64_t n, const std::vector<int64_t> &dims,
+                       const std::vector<int64_t> &strides, void *stream) {
+  if (n <= 0) return;
+  if ((int)dims.size() > kMaxLegs) throw Error(QX_ERR_UNSUPPORTED, "add of a tensor with too many legs");
+  AddArgs a;
+  a.n = (int)dims.size();
+  for (int j = 0; j < a.n; ++j) a.dim[j] = dims[j], a.st[j] = strides[j];
+  int g = grid_for(n, 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == QX_C128)
+    hipLaunchKernelGGL((k_add_mapped<double, double>), dim3(g), dim3(256), 0, s, (double2 *)dst,
+                       (const double2 *)src, n, a);
+  else if (dst_double)
+    hipLaunchKernelGGL((k_add_mapped<float, double>), dim3(g), dim3(256), 0, s, (double2 *)dst,
+                       (const float2 *)src, n, a);
+  else
+    hipLaunchKernelGGL((k_add_mapped<float, float>), dim3(g), dim3(256), 0, s, (float2 *)dst, (const float2 *)src,
+                       n, a);
+  check_launch("add_mapped");
+}
+
+void launch_root_cot(int prec, void *dst, int dep, int64_t batch, int64_t nvalid, int64_t relems,
+                     const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream) {
+  int g = grid_for(dep ? batch * relems : relems, 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == QX_C128)
+    hipLaunchKernelGGL(k_root_cot<double>, dim3(g), dim3(256), 0, s, (double2 *)dst, dep, batch, nvalid, relems,
+                       d_relem_off, amp, row0);
+  else
+    hipLaunchKernelGGL(k_root_cot<float>, dim3(g), dim3(256), 0, s, (float2 *)dst, dep, batch, nvalid, relems,
+                       d_relem_off, amp, row0);
+  check_launch("root_cot");
 }
 
 void launch_finalize(const double *acc, double *out, int64_t n, void *stream) {

@@ -196,7 +196,72 @@ void set_kernel_args(Step &st) {
 
 }
 
-Lowered lower(const Program &p, int64_t batch) {
+Step make_step(const Value &A, const std::vector<int> &alab, bool conj_a, const Value &B,
+               const std::vector<int> &blab, bool conj_b, const Value &C, const std::vector<int> &clab,
+               int64_t batch) {
+  Step st;
+  std::map<int, std::pair<int64_t, int64_t>> la, lb;  // label -> (dim, stride)
+  for (size_t j = 0; j < alab.size(); ++j) la[alab[j]] = {A.dims[j], A.strides[j]};
+  for (size_t j = 0; j < blab.size(); ++j) lb[blab[j]] = {B.dims[j], B.strides[j]};
+  // the batch leg: free in C when C carries it, else contracted between A and B
+  if (C.dep) {
+    if (!A.dep && !B.dep) throw Error(QX_ERR_STATE, "internal: batch leg in C only");
+    st.cl.push_back({batch, A.dep ? A.beta_stride : 0, B.dep ? B.beta_stride : 0, C.beta_stride});
+    if (A.dep && B.dep) st.H *= batch;
+    else if (A.dep) st.M *= batch;
+    else st.N *= batch;
+  } else if (A.dep && B.dep) {
+    st.kl.push_back({batch, A.beta_stride, B.beta_stride, 0});
+    st.K *= batch;
+  } else if (A.dep || B.dep) {
+    throw Error(QX_ERR_STATE, "internal: batch leg summed out of one operand");
+  }
+  for (size_t j = 0; j < clab.size(); ++j) {
+    int l = clab[j];
+    int64_t d = C.dims[j];
+    bool ina = la.count(l), inb = lb.count(l);
+    if (ina && inb) st.H *= d;
+    else if (ina) st.M *= d;
+    else st.N *= d;
+    if (d == 1) continue;
+    st.cl.push_back({d, ina ? la[l].second : 0, inb ? lb[l].second : 0, C.strides[j]});
+  }
+  // C legs in memory order; C must be dense
+  std::stable_sort(st.cl.begin(), st.cl.end(), [](const Leg &x, const Leg &y) { return x.sc < y.sc; });
+  int64_t run = 1;
+  for (auto &g : st.cl) {
+    if (g.sc != run) throw Error(QX_ERR_STATE, "internal: step output is not dense");
+    run *= g.dim;
+  }
+  st.c_elems = run;
+  for (size_t j = 0; j < alab.size(); ++j) {
+    int l = alab[j];
+    if (!lb.count(l) || std::find(clab.begin(), clab.end(), l) != clab.end()) continue;
+    int64_t d = A.dims[j];
+    st.K *= d;
+    if (d == 1) continue;
+    st.kl.push_back({d, la[l].second, lb[l].second, 0});
+  }
+  bool all_pow2 = true;
+  int nbc = 0, nbk = 0;
+  for (auto &g : st.cl) {
+    all_pow2 &= pow2(g.dim);
+    nbc += ilog2(g.dim);
+  }
+  for (auto &g : st.kl) {
+    all_pow2 &= pow2(g.dim);
+    nbk += ilog2(g.dim);
+  }
+  st.kern = (all_pow2 && nbc <= kMaxBits && nbk <= kMaxBits) ? Kern::Bits : Kern::Radix;
+  if (st.kern == Kern::Radix && ((int)st.cl.size() > kMaxLegs || (int)st.kl.size() > kMaxLegs))
+    throw Error(QX_ERR_UNSUPPORTED, "step with more than 48 legs of non power-of-two dims");
+  set_kernel_args(st);
+  st.bits.conj_a = st.gemm.conj_a = st.rows.conj_a = st.radix.conj_a = conj_a;
+  st.bits.conj_b = st.gemm.conj_b = st.rows.conj_b = st.radix.conj_b = conj_b;
+  return st;
+}
+
+Lowered lower(const Program &p, int64_t batch, bool keep_all) {
   if (!pow2(batch)) throw Error(QX_ERR_ARGUMENT, "batch must be a power of two");
   Lowered L;
   L.batch = batch;
@@ -263,54 +328,11 @@ Lowered lower(const Program &p, int64_t batch) {
   for (size_t i = 0; i < nn; ++i) {
     const Node &n = p.nodes[i];
     if (n.op != Op::Ncon) continue;
-    const Value &A = L.values[n.ia], &B = L.values[n.ib], &C = L.values[i];
-    Step st;
+    Step st = make_step(L.values[n.ia], n.alab, false, L.values[n.ib], n.blab, false, L.values[i], n.clab, batch);
     st.node = (int)i;
     st.in_a = n.ia;
     st.in_b = n.ib;
-    std::map<int, std::pair<int64_t, int64_t>> la, lb;  // label -> (dim, stride)
-    for (size_t j = 0; j < n.alab.size(); ++j) la[n.alab[j]] = {A.dims[j], A.strides[j]};
-    for (size_t j = 0; j < n.blab.size(); ++j) lb[n.blab[j]] = {B.dims[j], B.strides[j]};
-    if (C.dep) {
-      st.cl.push_back({batch, A.dep ? A.beta_stride : 0, B.dep ? B.beta_stride : 0, 1});
-      if (A.dep && B.dep) st.H *= batch;
-      else if (A.dep) st.M *= batch;
-      else st.N *= batch;
-    }
-    for (size_t j = 0; j < n.clab.size(); ++j) {
-      int l = n.clab[j];
-      int64_t d = C.dims[j];
-      bool ina = la.count(l), inb = lb.count(l);
-      if (ina && inb) st.H *= d;
-      else if (ina) st.M *= d;
-      else st.N *= d;
-      if (d == 1) continue;
-      st.cl.push_back({d, ina ? la[l].second : 0, inb ? lb[l].second : 0, C.strides[j]});
-    }
-    for (size_t j = 0; j < n.alab.size(); ++j) {
-      int l = n.alab[j];
-      if (!lb.count(l) || std::find(n.clab.begin(), n.clab.end(), l) != n.clab.end()) continue;
-      int64_t d = A.dims[j];
-      st.K *= d;
-      if (d == 1) continue;
-      st.kl.push_back({d, la[l].second, lb[l].second, 0});
-    }
-    st.c_elems = C.elems;
-    bool all_pow2 = true;
-    int nbc = 0, nbk = 0;
-    for (auto &g : st.cl) {
-      all_pow2 &= pow2(g.dim);
-      nbc += ilog2(g.dim);
-    }
-    for (auto &g : st.kl) {
-      all_pow2 &= pow2(g.dim);
-      nbk += ilog2(g.dim);
-    }
-    st.kern = (all_pow2 && nbc <= kMaxBits && nbk <= kMaxBits) ? Kern::Bits : Kern::Radix;
-    if (st.kern == Kern::Radix && ((int)st.cl.size() > kMaxLegs || (int)st.kl.size() > kMaxLegs))
-      throw Error(QX_ERR_UNSUPPORTED, "step with more than 48 legs of non power-of-two dims");
-    st.section = C.dep ? 2 : (n.sdep ? 1 : 0);
-    set_kernel_args(st);
+    st.section = L.values[i].dep ? 2 : (n.sdep ? 1 : 0);
     L.steps.push_back(std::move(st));
   }
 
@@ -370,6 +392,9 @@ Lowered lower(const Program &p, int64_t batch) {
   }
   int rs = root[p.save];
   if (rs >= 0) en_pos[rs] = end_pos;
+  if (keep_all)
+    for (size_t i = 0; i < nn; ++i)
+      if (root[i] == (int)i) en_pos[i] = end_pos;
   std::vector<int> arena_nodes;
   std::vector<int64_t> S, E, Z, off;
   for (size_t i = 0; i < nn; ++i)
@@ -389,6 +414,142 @@ Lowered lower(const Program &p, int64_t batch) {
     if (root[i] >= 0 && root[i] != (int)i) L.values[i].slot = L.values[root[i]].slot;
   check_bounds(p, L, root);
   return L;
+}
+
+VjpLowered lower_vjp(const Program &p, int64_t batch) {
+  VjpLowered V;
+  V.fwd = lower(p, batch, true);
+  const Lowered &L = V.fwd;
+  const size_t nn = p.nodes.size();
+  // every tensor must feed exactly one consumer (a tree), so each cotangent has
+  // one producer per run of its consumer's section
+  std::vector<int> uses(nn, 0);
+  for (const Node &n : p.nodes) {
+    if (n.op == Op::Ncon) uses[n.ia]++, uses[n.ib]++;
+    if (n.op == Op::View || n.op == Op::Save) uses[n.ia]++;
+  }
+  for (size_t i = 0; i < nn; ++i)
+    if (uses[i] > 1)
+      throw Error(QX_ERR_UNSUPPORTED, "vjp: tensor '" + p.nodes[i].name + "' is used more than once");
+  std::vector<int> root(nn, -1), sec(nn, 0);
+  for (size_t i = 0; i < nn; ++i) {
+    const Node &n = p.nodes[i];
+    if (n.op == Op::View || n.op == Op::Save) root[i] = root[n.ia];
+    else root[i] = (int)i;
+  }
+  for (const Step &st : L.steps) sec[st.node] = st.section;
+  for (int o : p.outputs) sec[o] = 2;
+  V.root = root[p.save];
+  if (p.nodes[V.root].op != Op::Ncon) throw Error(QX_ERR_UNSUPPORTED, "vjp: the saved tensor is not a contraction");
+  // cotangent buffers: same dense layout as the forward values, no reuse
+  V.cot.resize(nn);
+  int64_t off = 0;
+  for (size_t i = 0; i < nn; ++i) {
+    if (p.nodes[i].op != Op::Ncon) continue;
+    Value c = L.values[i];
+    c.buf = Buf::Cot;
+    c.slot = off;
+    c.static_off = 0;
+    c.var_terms.clear();
+    V.cot[i] = c;
+    off += (c.elems + 31) / 32 * 32;
+  }
+  V.cot_elems = off;
+  V.bsteps.assign(3, {});
+  V.zero_at.assign(3, {});
+  auto mark_zero = [&](int node) {
+    auto &z = V.zero_at[sec[node]];
+    if (std::find(z.begin(), z.end(), node) == z.end()) z.push_back(node);
+  };
+  if (!L.values[V.root].dep) mark_zero(V.root);   // root cotangent summed over bitstring rows
+  std::map<std::string, int> key_index;
+  for (size_t i = 0; i < p.keys.size(); ++i) key_index[p.keys[i]] = (int)i;
+
+  for (auto it = L.steps.rbegin(); it != L.steps.rend(); ++it) {
+    const Step &fs = *it;
+    const Node &cn = p.nodes[fs.node];
+    for (int side = 0; side < 2; ++side) {
+      const int x = side == 0 ? cn.ia : cn.ib;
+      const int rx = root[x];
+      const Node &rn = p.nodes[rx];
+      if (rn.op == Op::Output) continue;        // bitstring tensors need no gradient
+      const Value &XV = L.values[x];
+      BwdStep b;
+      b.x = x;
+      b.direct = (p.nodes[x].op == Op::Ncon) && (sec[x] == fs.section);
+      Value out;
+      if (b.direct) {
+        out = V.cot[x];
+      } else {
+        out.buf = Buf::Temp;
+        out.slot = 0;
+        out.dims = XV.dims;
+        out.dep = XV.dep;
+        int64_t st = 1;
+        if (XV.dep) {
+          out.beta_stride = 1;
+          st = batch;
+          b.add_dims.push_back(batch);
+          b.add_strides.push_back(XV.beta_stride);
+        }
+        for (size_t j = 0; j < XV.dims.size(); ++j) {
+          out.strides.push_back(XV.dims[j] > 1 ? st : 0);
+          if (XV.dims[j] > 1) {
+            st *= XV.dims[j];
+            b.add_dims.push_back(XV.dims[j]);
+            b.add_strides.push_back(XV.strides[j]);
+          }
+        }
+        out.elems = st;
+        b.add_elems = st;
+        V.temp_elems = std::max(V.temp_elems, st);
+        if (rn.op == Op::Load) {
+          b.dest = Buf::Grad;
+          b.dest_key = key_index.at(rn.key);
+        } else {
+          b.dest = Buf::Cot;
+          b.dest_root = rx;
+          mark_zero(rx);
+        }
+      }
+      if (side == 0)
+        b.step = make_step(V.cot[fs.node], cn.clab, false, L.values[cn.ib], cn.blab, true, out, cn.alab, batch);
+      else
+        b.step = make_step(L.values[cn.ia], cn.alab, true, V.cot[fs.node], cn.clab, false, out, cn.blab, batch);
+      b.step.node = fs.node;
+      b.step.section = fs.section;
+      // host-side proof that the step and the add stay inside their buffers
+      auto ext = [&](const Value &v, int node) -> std::pair<int64_t, int64_t> {
+        int64_t mx = v.static_off;
+        for (auto &t : v.var_terms) mx += (p.var_dims[t.first] - 1) * t.second;
+        int64_t cap;
+        if (v.buf == Buf::Data) cap = p.key_elems.at(p.keys[v.data_key]);
+        else if (v.buf == Buf::Cot) cap = V.cot[node].elems;
+        else if (v.buf == Buf::Temp) cap = v.elems;
+        else cap = L.values[root[node]].elems;
+        return {mx, cap};
+      };
+      const Value &cv = V.cot[fs.node];
+      std::pair<int64_t, int64_t> ea = side == 0 ? ext(cv, fs.node) : ext(L.values[cn.ia], cn.ia);
+      std::pair<int64_t, int64_t> eb = side == 0 ? ext(L.values[cn.ib], cn.ib) : ext(cv, fs.node);
+      std::pair<int64_t, int64_t> ec = ext(out, x);
+      int64_t ma = 0, mb = 0, mc = 0;
+      for (auto &g : b.step.cl) ma += (g.dim - 1) * g.sa, mb += (g.dim - 1) * g.sb, mc += (g.dim - 1) * g.sc;
+      for (auto &g : b.step.kl) ma += (g.dim - 1) * g.sa, mb += (g.dim - 1) * g.sb;
+      if (ea.first + ma >= ea.second || eb.first + mb >= eb.second || ec.first + mc >= ec.second)
+        throw Error(QX_ERR_STATE, "internal: backward step of '" + cn.name + "' addresses outside its operands");
+      if (!b.direct) {
+        int64_t md = 0;
+        for (size_t j = 0; j < b.add_dims.size(); ++j) md += (b.add_dims[j] - 1) * b.add_strides[j];
+        std::pair<int64_t, int64_t> ex = ext(XV, x);
+        int64_t cap = b.dest == Buf::Grad ? p.key_elems.at(p.keys[b.dest_key]) : V.cot[b.dest_root].elems;
+        if (ex.first + md >= cap)
+          throw Error(QX_ERR_STATE, "internal: cotangent add of '" + p.nodes[x].name + "' outside its buffer");
+      }
+      V.bsteps[fs.section].push_back(std::move(b));
+    }
+  }
+  return V;
 }
 
 }  // namespace qx

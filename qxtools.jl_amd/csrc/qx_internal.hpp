@@ -63,7 +63,7 @@ Program parse_program(const char *text, size_t len);
 
 // ----------------------------------------------------------------------------
 // Lowered form for one batch size.
-enum class Buf : int8_t { None, Data, Arena };
+enum class Buf : int8_t { None, Data, Arena, Cot, Temp, Grad };
 
 // Where a tensor value lives and how its logical legs map to memory.
 struct Value {
@@ -159,7 +159,42 @@ struct Lowered {
   int64_t result_elems = 1;         // complex values per bitstring
 };
 
-Lowered lower(const Program &p, int64_t batch);
+Lowered lower(const Program &p, int64_t batch, bool keep_all = false);
+
+// C[clab] = sum A[alab] B[blab] over the given value layouts (C dense), kernel
+// chosen and arguments filled; conj flags conjugate an operand on load.
+Step make_step(const Value &A, const std::vector<int> &alab, bool conj_a, const Value &B,
+               const std::vector<int> &blab, bool conj_b, const Value &C, const std::vector<int> &clab,
+               int64_t batch);
+
+// Reverse-mode plan of one batch size (qx_amplitudes_vjp).  The forward keeps
+// every value of a slice; each forward step C = A.B contributes Abar = Cbar.conj(B)
+// and Bbar = conj(A).Cbar as ordinary steps.  A cotangent goes straight into its
+// value's cotangent buffer when its consumer runs in the same loop section,
+// otherwise through a temporary that is added (through the value's view offsets
+// and strides) into the cotangent of what it views, or into the FP64 gradient of
+// a data key.
+struct BwdStep {
+  Step step;                  // contraction; its C is the cotangent buffer or the temp
+  int x = -1;                 // forward node whose cotangent this computes
+  bool direct = false;
+  Buf dest = Buf::None;       // Cot (cotangent of an ncon value) or Grad (data key)
+  int dest_root = -1;         // node (Cot) whose buffer receives the add
+  int dest_key = -1;          // data key (Grad)
+  std::vector<int64_t> add_dims, add_strides;   // temp (dense) -> destination
+  int64_t add_elems = 0;
+};
+
+struct VjpLowered {
+  Lowered fwd;                          // forward lowering, every value kept
+  std::vector<Value> cot;               // per node: cotangent buffer of ncon values
+  std::vector<std::vector<BwdStep>> bsteps;   // per section, execution order
+  std::vector<std::vector<int>> zero_at;      // per section: accumulating cotangents
+  int64_t cot_elems = 0, temp_elems = 0;
+  int root = -1;
+};
+
+VjpLowered lower_vjp(const Program &p, int64_t batch);
 
 // First-fit offsets for [start, end) intervals of given sizes (elements), 256-byte
 // aligned for element size `esize`.  Returns the total size; offsets written out.
@@ -187,6 +222,13 @@ void launch_accumulate(int prec, const void *root, int root_dep, int64_t batch, 
                        int64_t relems, const int64_t *d_relem_off, double *acc, int64_t row0,
                        void *stream);
 void launch_finalize(const double *acc, double *out, int64_t n, void *stream);
+// dst[map(e)] += src[e] for src dense over dims (first fastest), map = sum digit*stride
+void launch_add_mapped(int prec, int dst_double, void *dst, const void *src, int64_t n,
+                       const std::vector<int64_t> &dims, const std::vector<int64_t> &strides,
+                       void *stream);
+// cotangent of the saved tensor from amp_cot rows [row0, row0 + nvalid)
+void launch_root_cot(int prec, void *dst, int dep, int64_t batch, int64_t nvalid, int64_t relems,
+                     const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream);
 void launch_zero(void *p, int64_t bytes, void *stream);
 
 }  // namespace qx

@@ -85,6 +85,8 @@ struct qx_plan {
 
   std::vector<DevBuf> data;                 // per key
   std::map<int64_t, qx::Lowered> lowered;   // per batch size
+  std::map<int64_t, qx::VjpLowered> vjp_lowered;
+  DevBuf varena, cot, temp, grad, ampcot;
   DevBuf arena, out_off, out_q, bits, acc, outbuf, relem_off;
   int64_t arena_cap_bytes = 0;
   std::vector<double> step_ms;
@@ -103,6 +105,11 @@ struct qx_plan {
     acc.release();
     outbuf.release();
     relem_off.release();
+    varena.release();
+    cot.release();
+    temp.release();
+    grad.release();
+    ampcot.release();
     for (auto e : events) (void)hipEventDestroy(e);
   }
 };
@@ -122,15 +129,45 @@ const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch) {
   return P->lowered.at(batch);
 }
 
-// device pointer of a value for the given slice-variable assignment
-void *value_ptr(qx_plan *P, const qx::Value &v, const std::vector<int64_t> &varval) {
+struct Bases {
+  char *arena = nullptr, *cot = nullptr, *temp = nullptr;
+};
+
+int64_t value_offset(const qx::Value &v, const std::vector<int64_t> &varval) {
   int64_t off = v.static_off;
   for (auto &t : v.var_terms) off += varval[t.first] * t.second;
+  return off;
+}
+
+// device pointer of a value for the given slice-variable assignment
+void *value_ptr(qx_plan *P, const qx::Value &v, const std::vector<int64_t> &varval, const Bases &b) {
+  const int64_t es = (int64_t)P->esize();
   char *base = nullptr;
-  if (v.buf == qx::Buf::Data) base = (char *)P->data[v.data_key].p;
-  else if (v.buf == qx::Buf::Arena) base = (char *)P->arena.p + v.slot * (int64_t)P->esize();
-  else throw qx::Error(QX_ERR_STATE, "value without storage");
-  return base + off * (int64_t)P->esize();
+  switch (v.buf) {
+    case qx::Buf::Data: base = (char *)P->data[v.data_key].p; break;
+    case qx::Buf::Arena: base = b.arena + v.slot * es; break;
+    case qx::Buf::Cot: base = b.cot + v.slot * es; break;
+    case qx::Buf::Temp: base = b.temp + v.slot * es; break;
+    default: throw qx::Error(QX_ERR_STATE, "value without storage");
+  }
+  return base + value_offset(v, varval) * es;
+}
+
+void *value_ptr(qx_plan *P, const qx::Value &v, const std::vector<int64_t> &varval) {
+  Bases b;
+  b.arena = (char *)P->arena.p;
+  return value_ptr(P, v, varval, b);
+}
+
+void launch_step(qx_plan *P, const qx::Step &st, void *A, void *B, void *C) {
+  hipStream_t stream = P->ctx->stream;
+  switch (st.kern) {
+    case qx::Kern::Radix: qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, st.radix, stream); break;
+    case qx::Kern::Gemm: qx::launch_contract_gemm(P->prec, A, B, C, st.gemm, stream); break;
+    case qx::Kern::Reduce: qx::launch_contract_reduce(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
+    case qx::Kern::Rows: qx::launch_contract_rows(P->prec, A, B, C, st.rows, stream); break;
+    default: qx::launch_contract_bits(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
+  }
 }
 
 void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int64_t> &varval) {
@@ -150,13 +187,7 @@ void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int6
     P->event_step.push_back((int)s);
     hip_check(hipEventRecord(P->events[2 * ev], stream), "hipEventRecord");
   }
-  switch (st.kern) {
-    case qx::Kern::Radix: qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, st.radix, stream); break;
-    case qx::Kern::Gemm: qx::launch_contract_gemm(P->prec, A, B, C, st.gemm, stream); break;
-    case qx::Kern::Reduce: qx::launch_contract_reduce(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
-    case qx::Kern::Rows: qx::launch_contract_rows(P->prec, A, B, C, st.rows, stream); break;
-    default: qx::launch_contract_bits(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
-  }
+  launch_step(P, st, A, B, C);
   if (P->opt_profile) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
 }
 
@@ -176,6 +207,15 @@ void collect_profile(qx_plan *P, const qx::Lowered &L) {
   P->event_step.clear();
 }
 
+void check_bits(qx_plan *P, const char *bits, int64_t nb) {
+  const int64_t nch = P->prog.num_chars;
+  for (int64_t i = 0; i < nb * nch; ++i) {
+    char c = bits[i];
+    if (c != '0' && c != '1' && c != '+' && c != '-')
+      throw qx::Error(QX_ERR_ARGUMENT, "bitstring characters must be 0, 1, + or -");
+  }
+}
+
 void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1, double *out) {
   if (!P->finalized) throw qx::Error(QX_ERR_STATE, "plan is not finalized");
   if (nb <= 0) throw qx::Error(QX_ERR_ARGUMENT, "need at least one bitstring");
@@ -184,11 +224,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   for (int64_t d : P->prog.var_dims) nslices *= d;
   if (s0 < 0 || s1 > nslices || s0 >= s1) throw qx::Error(QX_ERR_ARGUMENT, "slice range out of bounds");
   const int64_t nch = P->prog.num_chars;
-  for (int64_t i = 0; i < nb * nch; ++i) {
-    char c = bits[i];
-    if (c != '0' && c != '1' && c != '+' && c != '-')
-      throw qx::Error(QX_ERR_ARGUMENT, "bitstring characters must be 0, 1, + or -");
-  }
+  check_bits(P, bits, nb);
   int64_t batch = 1;
   while (batch < nb && batch < P->opt_batch) batch <<= 1;
   const qx::Lowered &L = ensure_lowered(P, batch);
@@ -316,6 +352,148 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
        << kern_variant(st) << "\"}";
   }
   os << "]}";
+}
+
+const qx::VjpLowered &ensure_vjp(qx_plan *P, int64_t batch) {
+  auto it = P->vjp_lowered.find(batch);
+  if (it != P->vjp_lowered.end()) return it->second;
+  qx::VjpLowered V = qx::lower_vjp(P->prog, batch);
+  const size_t es = P->esize();
+  size_t need = (size_t)(V.fwd.arena_elems + V.cot_elems + V.temp_elems) * es;
+  if (P->arena_cap_bytes > 0 && (int64_t)need > P->arena_cap_bytes)
+    throw qx::Error(QX_ERR_MEMORY, "vjp needs " + std::to_string(need >> 20) + " MiB of device memory for batch " +
+                                       std::to_string(batch) + ", over the cap of " +
+                                       std::to_string(P->arena_cap_bytes >> 20) + " MiB");
+  P->vjp_lowered.emplace(batch, std::move(V));
+  return P->vjp_lowered.at(batch);
+}
+
+void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double *grad) {
+  if (!P->finalized) throw qx::Error(QX_ERR_STATE, "plan is not finalized");
+  if (nb <= 0) throw qx::Error(QX_ERR_ARGUMENT, "need at least one bitstring");
+  if (!bits || !amp_cot || !grad) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
+  check_bits(P, bits, nb);
+  int64_t batch = 1;
+  while (batch < nb && batch < P->opt_batch) batch <<= 1;
+  const qx::VjpLowered &V = ensure_vjp(P, batch);
+  const qx::Lowered &L = V.fwd;
+  hipStream_t stream = P->ctx->stream;
+  const int64_t es = (int64_t)P->esize();
+  const int64_t nch = P->prog.num_chars;
+  const int64_t R = L.result_elems;
+  P->varena.reserve(std::max<size_t>((size_t)L.arena_elems * es, 256));
+  P->cot.reserve(std::max<size_t>((size_t)V.cot_elems * es, 256));
+  P->temp.reserve(std::max<size_t>((size_t)V.temp_elems * es, 256));
+  std::vector<int64_t> key_off;
+  int64_t ke = 0;
+  for (auto &k : P->prog.keys) {
+    key_off.push_back(ke);
+    ke += P->prog.key_elems.at(k);
+  }
+  P->grad.reserve(std::max<size_t>((size_t)ke * 16, 16));
+  qx::launch_zero(P->grad.p, ke * 16, stream);
+  P->ampcot.reserve((size_t)nb * R * 16);
+  hip_check(hipMemcpyAsync(P->ampcot.p, amp_cot, (size_t)nb * R * 16, hipMemcpyHostToDevice, stream), "copy");
+  size_t nbits = (size_t)nb * nch;
+  P->bits.reserve(std::max<size_t>(nbits, 16));
+  if (nbits) hip_check(hipMemcpyAsync(P->bits.p, bits, nbits, hipMemcpyHostToDevice, stream), "copy");
+  std::vector<int64_t> ooff;
+  std::vector<int> oq;
+  for (int o : P->prog.outputs) {
+    ooff.push_back(L.values[o].slot);
+    oq.push_back(P->prog.nodes[o].qubit);
+  }
+  const int nout = (int)ooff.size();
+  P->out_off.reserve(std::max<size_t>(ooff.size() * 8, 16));
+  P->out_q.reserve(std::max<size_t>(oq.size() * 4, 16));
+  if (nout) {
+    hip_check(hipMemcpyAsync(P->out_off.p, ooff.data(), ooff.size() * 8, hipMemcpyHostToDevice, stream), "copy");
+    hip_check(hipMemcpyAsync(P->out_q.p, oq.data(), oq.size() * 4, hipMemcpyHostToDevice, stream), "copy");
+  }
+  const qx::Value &rc = V.cot[V.root];
+  std::vector<int64_t> roff(R);
+  {
+    const qx::Value &rv = L.values[P->prog.save];
+    for (int64_t e = 0; e < R; ++e) {
+      int64_t r = e, o = 0;
+      for (size_t j = 0; j < rv.dims.size(); ++j) {
+        int64_t d = r % rv.dims[j];
+        r /= rv.dims[j];
+        o += d * rv.strides[j];
+      }
+      roff[e] = o;
+    }
+  }
+  P->relem_off.reserve(std::max<size_t>(R * 8, 16));
+  hip_check(hipMemcpyAsync(P->relem_off.p, roff.data(), R * 8, hipMemcpyHostToDevice, stream), "copy");
+
+  Bases bs;
+  bs.arena = (char *)P->varena.p;
+  bs.cot = (char *)P->cot.p;
+  bs.temp = (char *)P->temp.p;
+  const size_t nv = P->prog.vars.size();
+  std::vector<int64_t> varval(nv, 0);
+  auto zero_cots = [&](int sec) {
+    for (int node : V.zero_at[sec]) {
+      const qx::Value &c = V.cot[node];
+      int64_t elems = (c.elems + 31) / 32 * 32;
+      qx::launch_zero(bs.cot + c.slot * es, elems * es, stream);
+    }
+  };
+  auto run_fwd = [&](int sec) {
+    for (const qx::Step &st : L.steps)
+      if (st.section == sec)
+        launch_step(P, st, value_ptr(P, L.values[st.in_a], varval, bs), value_ptr(P, L.values[st.in_b], varval, bs),
+                    value_ptr(P, L.values[st.node], varval, bs));
+  };
+  auto run_bwd = [&](int sec) {
+    for (const qx::BwdStep &b : V.bsteps[sec]) {
+      const qx::Node &cn = P->prog.nodes[b.step.node];
+      const qx::Value &cv = V.cot[b.step.node];
+      const bool xa = b.x == cn.ia;
+      void *A = xa ? value_ptr(P, cv, varval, bs) : value_ptr(P, L.values[cn.ia], varval, bs);
+      void *B = xa ? value_ptr(P, L.values[cn.ib], varval, bs) : value_ptr(P, cv, varval, bs);
+      void *C = b.direct ? value_ptr(P, V.cot[b.x], varval, bs) : (void *)bs.temp;
+      launch_step(P, b.step, A, B, C);
+      if (b.direct) continue;
+      const int64_t xoff = value_offset(L.values[b.x], varval);
+      if (b.dest == qx::Buf::Grad) {
+        char *dst = (char *)P->grad.p + (key_off[b.dest_key] + xoff) * 16;
+        qx::launch_add_mapped(P->prec, 1, dst, bs.temp, b.add_elems, b.add_dims, b.add_strides, stream);
+      } else {
+        char *dst = bs.cot + (V.cot[b.dest_root].slot + xoff) * es;
+        qx::launch_add_mapped(P->prec, P->prec == QX_C128, dst, bs.temp, b.add_elems, b.add_dims, b.add_strides,
+                              stream);
+      }
+    }
+  };
+  int64_t nslices = 1;
+  for (int64_t d : P->prog.var_dims) nslices *= d;
+  run_fwd(0);
+  zero_cots(0);
+  for (int64_t sl = 0; sl < nslices; ++sl) {
+    int64_t r = sl;
+    for (size_t j = 0; j < nv; ++j) {
+      varval[j] = r % P->prog.var_dims[j];
+      r /= P->prog.var_dims[j];
+    }
+    run_fwd(1);
+    zero_cots(1);
+    for (int64_t row0 = 0; row0 < nb; row0 += batch) {
+      int64_t nvalid = std::min<int64_t>(batch, nb - row0);
+      zero_cots(2);
+      qx::launch_fill_outputs(P->prec, bs.arena, (const int64_t *)P->out_off.p, nout, (const int *)P->out_q.p,
+                              (const char *)P->bits.p + row0 * nch, nvalid, batch, nch, stream);
+      run_fwd(2);
+      qx::launch_root_cot(P->prec, value_ptr(P, rc, varval, bs), rc.dep ? 1 : 0, batch, nvalid, R,
+                          (const int64_t *)P->relem_off.p, (const double *)P->ampcot.p, row0, stream);
+      run_bwd(2);
+    }
+    run_bwd(1);
+  }
+  run_bwd(0);
+  hip_check(hipMemcpyAsync(grad, P->grad.p, (size_t)ke * 16, hipMemcpyDeviceToHost, stream), "copy out");
+  hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
 std::string stats_json(const qx_plan *P) {
@@ -505,12 +683,9 @@ qx_status qx_amplitudes(qx_plan *plan, const char *bits, int64_t nb, int64_t sli
 
 qx_status qx_amplitudes_vjp(qx_plan *plan, const char *bits, int64_t nb, const double *amp_cot, double *grad) {
   return guard([&] {
-    (void)bits;
-    (void)nb;
-    (void)amp_cot;
-    (void)grad;
     if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
-    throw qx::Error(QX_ERR_UNSUPPORTED, "vjp not available in this build");
+    hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+    vjp(plan, bits, nb, amp_cot, grad);
   });
 }
 
@@ -542,6 +717,19 @@ qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t bat
     os.precision(17);
     lowered_json(os, prog, L, precision == QX_C128 ? 16 : 8, true);
     std::string s = os.str();
+    // reverse-mode lowering (its own host-side bounds proof runs here too)
+    std::string v;
+    try {
+      qx::VjpLowered V = qx::lower_vjp(prog, batch);
+      size_t nb = V.bsteps[0].size() + V.bsteps[1].size() + V.bsteps[2].size();
+      v = ",\"vjp\":{\"backward_steps\":" + std::to_string(nb) + ",\"forward_elems\":" +
+          std::to_string(V.fwd.arena_elems) + ",\"cot_elems\":" + std::to_string(V.cot_elems) +
+          ",\"temp_elems\":" + std::to_string(V.temp_elems) + "}";
+    } catch (const qx::Error &e) {
+      if (e.code != QX_ERR_UNSUPPORTED) throw;
+      v = ",\"vjp_error\":\"" + std::string(e.what()) + "\"";
+    }
+    s.insert(s.size() - 1, v);
     if (s.size() + 1 > cap) throw qx::Error(QX_ERR_ARGUMENT, "inspect needs " + std::to_string(s.size() + 1) + " bytes");
     std::memcpy(json, s.c_str(), s.size() + 1);
   });

@@ -73,6 +73,18 @@ class TensorCache:
         return {k: self[k] for k in self.key_dim_map}
 
 
+class _NoDedupe(TensorCache):
+    """TensorCache that never merges equal tensors."""
+
+    def push(self, data: np.ndarray) -> str:
+        data = np.asarray(data)
+        dim = tuple(int(d) for d in data.shape)
+        sym = self.next_symbol()
+        self.tensors.setdefault(dim, []).append((sym, data.reshape(-1, order="F").astype(np.complex128)))
+        self.key_dim_map[sym] = dim
+        return sym
+
+
 @dataclass
 class Command:
     kind: str                 # load | output | view | ncon | save
@@ -133,9 +145,10 @@ def _metadata_lines(meta, indent: int = 0) -> list:
 
 
 def build_compute_graph(tnc: TensorNetworkCircuit, plan: list, bond_groups: list | None = None,
-                        metadata=None) -> ComputeGraph:
+                        metadata=None, dedupe: bool = True) -> ComputeGraph:
     """Lower (network, plan, bond groups) to a compute graph
-    (`src/compute_graph/compute_graph.jl:15-98`)."""
+    (`src/compute_graph/compute_graph.jl:15-98`).  With `dedupe=False` every tensor
+    gets its own data key (needed when gradients are taken per gate)."""
     tn = tnc.tn
     if hasattr(tnc, "open_indices") and callable(tnc.open_indices):
         open_set = set(tnc.open_indices())
@@ -146,7 +159,7 @@ def build_compute_graph(tnc: TensorNetworkCircuit, plan: list, bond_groups: list
     for n, l in legs.items():
         for i in l:
             count[i] = count.get(i, 0) + 1
-    tc = TensorCache()
+    tc = TensorCache() if dedupe else _NoDedupe()
     node: "OrderedDict[str, Command]" = OrderedDict()
     for n, t in tn.tensors.items():                                   # :25-29
         node[n] = Command("load", n, (tc.push(t.data), tuple(t.data.shape)))

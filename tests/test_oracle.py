@@ -183,3 +183,38 @@ def test_neumaier_recovers_cancellation():
     for x in itertools.repeat(np.array([0.1 + 0.1j, 1e-17]), 10):
         acc.add(x)
     assert acc.value()[0] == pytest.approx(1 + 1j, abs=1e-16)
+
+
+# --------------------------------------------------------------------- gradient oracle
+def test_env_gradient_matches_finite_differences():
+    """Environment-tensor gradients chained to gate angles equal central finite
+    differences of the statevector loss L = sum |amp|^2 (h = 1e-6, FP64)."""
+    from qxhip import compute_graph, planning, tn
+
+    c = circuits.create_param_rqc_circuit(2, 2, 2, seed=1)
+    tnc, names = tn.gate_tensor_map(c)
+    tnc.add_input()
+    tnc.add_output()
+    g = compute_graph.build_compute_graph(tnc, planning.sweep_contraction_plan(tnc), dedupe=False)
+    bits = ["0101", "1100", "+0-1"]
+    dsl = g.to_dsl()
+    amps = O.amplitudes(dsl, g.data, bits)
+    kg = O.env_gradient(dsl, g.data, bits, 2 * amps)
+    key_of = {cm.name: cm.args[0] for cm in g.commands if cm.kind == "load"}
+    th = c.params()
+    grad = np.zeros(len(th))
+    for gg, nm in zip(c.ops, names):
+        if gg.param is None:
+            continue
+        s = tn.TensorNetworkCircuit(4)
+        dn = s.push(circuits.gate_qubits(gg), circuits.gate_matrix_derivative(gg))
+        d = s.tn[dn[0]].data
+        grad[gg.param] += np.real(np.sum(np.conj(kg[key_of[nm[0]]].reshape(d.shape, order="F")) * d))
+
+    def loss(t):
+        return float(np.sum(np.abs(qxtest.statevector_amps(c.with_params(t), bits)) ** 2))
+
+    h = 1e-6
+    eye = np.eye(len(th))
+    fd = np.array([(loss(th + h * eye[p]) - loss(th - h * eye[p])) / (2 * h) for p in range(len(th))])
+    assert np.max(np.abs(grad - fd)) < 1e-8
