@@ -87,6 +87,7 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=1, help="replay each call from a captured hipGraph")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the contraction path on a single "
@@ -103,21 +104,30 @@ def main() -> int:
     prec = a.precision or w.precision
     dsl, data, bits = w.load()
     log(f"workload {w.name}: {len(bits)} bitstrings, precision {prec}")
-    ex = Executor(dsl, data, precision=prec, device=0, batch=a.batch or min(256, len(bits)))
+    ex = Executor(dsl, data, precision=prec, device=0, batch=a.batch or min(256, len(bits)), graph=a.graph)
     for i in range(a.warmup):
         ex.amplitudes(bits)
         log(f"warmup {i + 1}/{a.warmup} done")
-    ex.plan.set_option("profile", 1)   # per-launch HIP events on the library's stream
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
         amps = ex.amplitudes(bits)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    log(f"timed {a.steps} steps in {el:.3f} s")
-    stats = ex.stats()
+    log(f"timed {a.steps} steps in {el:.3f} s (graph replay {'on' if a.graph else 'off'})")
     ms_step = 1e3 * el / a.steps
     value = len(bits) * a.steps / el
+    # kernel durations: the same steps launched eagerly with a HIP event pair around
+    # every launch on the library's stream (graph replay records no events)
+    ex.plan.set_option("profile", 1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(a.steps):
+        ex.amplitudes(bits)
+    torch.cuda.synchronize()
+    el_ev = time.perf_counter() - t1
+    log(f"event-timed pass: {a.steps} steps in {el_ev:.3f} s")
+    stats = ex.stats()
 
     fam = kernel_summary(stats, prec)
     dom_name, dom = max(fam.items(), key=lambda kv: kv[1]["ms"])
@@ -145,6 +155,7 @@ def main() -> int:
                    "sliced_bonds": w.sliced, "precision": prec, "batch": ex.plan.stats()["batches"][-1]["batch"]},
         "roofline": roof,
         "plan_roofline_frac": plan_roofline(stats, prec, ms_step, a.steps),
+        "ms_per_step_eager_with_events": 1e3 * el_ev / a.steps,
         "kernels": kernels,
     }
     if not a.no_cpu:

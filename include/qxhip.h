@@ -73,11 +73,13 @@ qx_status qx_plan_from_dsl(qx_ctx *ctx, const char *dsl, size_t len, int precisi
  * `nelem` interleaved complex128 values in column-major order; it is copied. */
 qx_status qx_plan_set_tensor(qx_plan *plan, const char *key, const double *z, int64_t nelem);
 
-/* Plan options, set before qx_plan_finalize ("profile" may also be set later;
- * setting it clears the accumulated step times):
+/* Plan options, set before qx_plan_finalize ("profile" and "graph" may also be
+ * set later; setting "profile" clears the accumulated step times):
  *   "batch"       bitstrings contracted together (power of two, default 256)
  *   "arena_mb"    device arena cap in MiB (default: 90% of free memory)
- *   "graph"       1 = replay each slice's launches from a captured hipGraph
+ *   "graph"       1 = capture the device work of a qx_amplitudes call (all slices
+ *                 and batches) once per (batch, nb, slice range) and replay it;
+ *                 ignored while "profile" is on
  *   "profile"     1 = time every step with HIP events (qx_plan_stats) */
 qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value);
 

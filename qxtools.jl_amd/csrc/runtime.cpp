@@ -15,6 +15,7 @@
 #include <cstring>
 #include <memory>
 #include <sstream>
+#include <tuple>
 
 #include "qx_internal.hpp"
 
@@ -52,16 +53,18 @@ qx_status guard(F &&f) {
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
-  void reserve(size_t n) {
-    if (n <= bytes) return;
+  // returns true when the buffer moved (captured graphs that use it are stale)
+  bool reserve(size_t n) {
+    if (n <= bytes) return false;
     release();
-    if (n == 0) return;
+    if (n == 0) return true;
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
       p = nullptr;
       throw qx::Error(QX_ERR_MEMORY, "device allocation of " + std::to_string(n) + " bytes failed");
     }
     bytes = n;
+    return true;
   }
   void release() {
     if (p) (void)hipFree(p);
@@ -92,6 +95,12 @@ struct qx_plan {
   std::vector<double> step_ms;
   std::vector<int64_t> step_calls;
   std::vector<hipEvent_t> events;           // pool: 2 per profiled launch
+  // captured device work of qx_amplitudes per (batch, nb, slice_begin, slice_end)
+  std::map<std::tuple<int64_t, int64_t, int64_t, int64_t>, hipGraphExec_t> graphs;
+  void drop_graphs() {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
   std::vector<int> event_step;              // step of each recorded pair this call
 
   size_t esize() const { return prec == QX_C128 ? 16 : 8; }
@@ -111,6 +120,7 @@ struct qx_plan {
     grad.release();
     ampcot.release();
     for (auto e : events) (void)hipEventDestroy(e);
+    drop_graphs();
   }
 };
 
@@ -230,7 +240,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   const qx::Lowered &L = ensure_lowered(P, batch);
   hipStream_t stream = P->ctx->stream;
   const size_t es = P->esize();
-  P->arena.reserve(std::max<size_t>((size_t)L.arena_elems * es, 256));
+  bool moved = P->arena.reserve(std::max<size_t>((size_t)L.arena_elems * es, 256));
 
   // output tables
   std::vector<int64_t> ooff;
@@ -240,8 +250,8 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     oq.push_back(P->prog.nodes[o].qubit);
   }
   const int nout = (int)ooff.size();
-  P->out_off.reserve(std::max<size_t>(ooff.size() * 8, 16));
-  P->out_q.reserve(std::max<size_t>(oq.size() * 4, 16));
+  moved |= P->out_off.reserve(std::max<size_t>(ooff.size() * 8, 16));
+  moved |= P->out_q.reserve(std::max<size_t>(oq.size() * 4, 16));
   if (nout) {
     hip_check(hipMemcpyAsync(P->out_off.p, ooff.data(), ooff.size() * 8, hipMemcpyHostToDevice, stream), "copy");
     hip_check(hipMemcpyAsync(P->out_q.p, oq.data(), oq.size() * 4, hipMemcpyHostToDevice, stream), "copy");
@@ -259,41 +269,69 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     }
     roff[e] = o;
   }
-  P->relem_off.reserve(std::max<size_t>(R * 8, 16));
+  moved |= P->relem_off.reserve(std::max<size_t>(R * 8, 16));
   hip_check(hipMemcpyAsync(P->relem_off.p, roff.data(), R * 8, hipMemcpyHostToDevice, stream), "copy");
   size_t nbits = (size_t)nb * nch;
-  P->bits.reserve(std::max<size_t>(nbits, 16));
+  moved |= P->bits.reserve(std::max<size_t>(nbits, 16));
   if (nbits) hip_check(hipMemcpyAsync(P->bits.p, bits, nbits, hipMemcpyHostToDevice, stream), "copy");
-  P->acc.reserve((size_t)nb * R * 32);
-  qx::launch_zero(P->acc.p, nb * R * 32, stream);
-  P->outbuf.reserve((size_t)nb * R * 16);
+  moved |= P->acc.reserve((size_t)nb * R * 32);
+  moved |= P->outbuf.reserve((size_t)nb * R * 16);
+  if (moved) P->drop_graphs();
 
   P->event_step.clear();
 
-  const size_t nv = P->prog.vars.size();
-  std::vector<int64_t> varval(nv, 0);
-  auto run_section = [&](int sec) {
-    for (size_t s = 0; s < L.steps.size(); ++s)
-      if (L.steps[s].section == sec) run_step(P, L, s, varval);
+  // the device work of one call, from zeroing the accumulators to the final sum
+  auto enqueue = [&]() {
+    qx::launch_zero(P->acc.p, nb * R * 32, stream);
+    const size_t nv = P->prog.vars.size();
+    std::vector<int64_t> varval(nv, 0);
+    auto run_section = [&](int sec) {
+      for (size_t s = 0; s < L.steps.size(); ++s)
+        if (L.steps[s].section == sec) run_step(P, L, s, varval);
+    };
+    run_section(0);
+    for (int64_t sl = s0; sl < s1; ++sl) {
+      int64_t r = sl;
+      for (size_t j = 0; j < nv; ++j) {
+        varval[j] = r % P->prog.var_dims[j];
+        r /= P->prog.var_dims[j];
+      }
+      run_section(1);
+      for (int64_t row0 = 0; row0 < nb; row0 += batch) {
+        int64_t nvalid = std::min<int64_t>(batch, nb - row0);
+        qx::launch_fill_outputs(P->prec, P->arena.p, (const int64_t *)P->out_off.p, nout, (const int *)P->out_q.p,
+                                (const char *)P->bits.p + row0 * nch, nvalid, batch, nch, stream);
+        run_section(2);
+        qx::launch_accumulate(P->prec, value_ptr(P, rv, varval), rv.dep ? 1 : 0, batch, nvalid, R,
+                              (const int64_t *)P->relem_off.p, (double *)P->acc.p, row0, stream);
+      }
+    }
+    qx::launch_finalize((const double *)P->acc.p, (double *)P->outbuf.p, nb * R, stream);
   };
-  run_section(0);
-  for (int64_t sl = s0; sl < s1; ++sl) {
-    int64_t r = sl;
-    for (size_t j = 0; j < nv; ++j) {
-      varval[j] = r % P->prog.var_dims[j];
-      r /= P->prog.var_dims[j];
+  if (P->opt_graph && !P->opt_profile) {
+    auto key = std::make_tuple(batch, nb, s0, s1);
+    auto it = P->graphs.find(key);
+    if (it == P->graphs.end()) {
+      hipGraph_t g = nullptr;
+      hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        enqueue();
+      } catch (...) {
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+      hipGraphExec_t ex = nullptr;
+      hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      hip_check(e, "hipGraphInstantiate");
+      it = P->graphs.emplace(key, ex).first;
     }
-    run_section(1);
-    for (int64_t row0 = 0; row0 < nb; row0 += batch) {
-      int64_t nvalid = std::min<int64_t>(batch, nb - row0);
-      qx::launch_fill_outputs(P->prec, P->arena.p, (const int64_t *)P->out_off.p, nout, (const int *)P->out_q.p,
-                              (const char *)P->bits.p + row0 * nch, nvalid, batch, nch, stream);
-      run_section(2);
-      qx::launch_accumulate(P->prec, value_ptr(P, rv, varval), rv.dep ? 1 : 0, batch, nvalid, R,
-                            (const int64_t *)P->relem_off.p, (double *)P->acc.p, row0, stream);
-    }
+    hip_check(hipGraphLaunch(it->second, stream), "hipGraphLaunch");
+  } else {
+    enqueue();
   }
-  qx::launch_finalize((const double *)P->acc.p, (double *)P->outbuf.p, nb * R, stream);
   hip_check(hipMemcpyAsync(out, P->outbuf.p, (size_t)nb * R * 16, hipMemcpyDeviceToHost, stream), "copy out");
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   collect_profile(P, L);
@@ -587,7 +625,7 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
   return guard([&] {
     if (!plan || !name) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
     std::string n(name);
-    bool runtime_opt = (n == "profile");
+    bool runtime_opt = (n == "profile" || n == "graph");
     if (plan->finalized && !runtime_opt) throw qx::Error(QX_ERR_STATE, "option '" + n + "' must be set before finalize");
     if (n == "batch") {
       if (value <= 0 || (value & (value - 1)) || value > (1 << 20))

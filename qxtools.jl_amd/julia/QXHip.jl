@@ -1,0 +1,180 @@
+"""
+    QXHip
+
+Julia binding of libqxhip.so (C ABI in include/qxhip.h): the GPU executor for the
+compute graphs QXTools builds (`build_compute_graph`, src/compute_graph/compute_graph.jl:15-98)
+and for `single_amplitude` (src/simulation.jl:86-91).
+
+Source only in this repository (no Julia toolchain in the build image); every entry
+point below is a thin `ccall` over the C ABI that the Python harness exercises.
+"""
+module QXHip
+
+using ChainRulesCore
+
+const libqxhip = get(ENV, "QXHIP_LIB", joinpath(@__DIR__, "..", "qxhip", "libqxhip.so"))
+
+const C128 = Cint(0)
+const C64_4M = Cint(1)
+const C64_3M = Cint(2)
+const C64_BF16X3 = Cint(3)
+
+struct QxError <: Exception
+    status::Int32
+    msg::String
+end
+Base.showerror(io::IO, e::QxError) = print(io, "qxhip error ", e.status, ": ", e.msg)
+
+function check(status::Int32)
+    if status != 0
+        msg = unsafe_string(ccall((:qx_last_error, libqxhip), Cstring, ()))
+        throw(QxError(status, msg))
+    end
+    nothing
+end
+
+"""One device and one HIP stream; calls on a context must be serialised."""
+mutable struct Context
+    handle::Ptr{Cvoid}
+    lock::ReentrantLock
+    function Context(device::Integer=0)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:qx_ctx_create, libqxhip), Int32, (Cint, Ptr{Ptr{Cvoid}}), device, h))
+        ctx = new(h[], ReentrantLock())
+        finalizer(c -> ccall((:qx_ctx_destroy, libqxhip), Cvoid, (Ptr{Cvoid},), c.handle), ctx)
+        ctx
+    end
+end
+
+"""A .qx program with its tensors resident on the device."""
+mutable struct Plan
+    handle::Ptr{Cvoid}
+    ctx::Context
+    num_chars::Int
+    result_elems::Int
+    num_slices::Int
+end
+
+"""
+    Plan(ctx, dsl::String, data::Dict{Symbol,<:AbstractArray}; precision=C128, batch=256)
+
+`dsl` is the text of a .qx file (QXContexts' DSL, docs/src/users_guide.md:37-170);
+`data` maps each `load` key to its tensor (the dictionary `convert(Dict, tc)` of a
+TensorCache, src/compute_graph/tensor_cache.jl:109-111).
+"""
+function Plan(ctx::Context, dsl::String, data::AbstractDict; precision::Cint=C128, batch::Integer=256,
+              arena_mb::Integer=0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    lock(ctx.lock) do
+        check(ccall((:qx_plan_from_dsl, libqxhip), Int32, (Ptr{Cvoid}, Cstring, Csize_t, Cint, Ptr{Ptr{Cvoid}}),
+                    ctx.handle, dsl, sizeof(dsl), precision, h))
+        check(ccall((:qx_plan_set_option, libqxhip), Int32, (Ptr{Cvoid}, Cstring, Int64), h[], "batch", batch))
+        arena_mb > 0 && check(ccall((:qx_plan_set_option, libqxhip), Int32, (Ptr{Cvoid}, Cstring, Int64),
+                                    h[], "arena_mb", arena_mb))
+        for (k, v) in data
+            z = convert(Vector{ComplexF64}, vec(v))        # column-major, interleaved re/im
+            check(ccall((:qx_plan_set_tensor, libqxhip), Int32, (Ptr{Cvoid}, Cstring, Ptr{ComplexF64}, Int64),
+                        h[], String(k), z, length(z)))
+        end
+        check(ccall((:qx_plan_finalize, libqxhip), Int32, (Ptr{Cvoid},), h[]))
+    end
+    nc, re, ns = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:qx_plan_info, libqxhip), Int32,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}),
+                h[], nc, re, ns, C_NULL, C_NULL))
+    p = Plan(h[], ctx, nc[], re[], ns[])
+    finalizer(q -> ccall((:qx_plan_destroy, libqxhip), Cvoid, (Ptr{Cvoid},), q.handle), p)
+    p
+end
+
+function packbits(p::Plan, bitstrings::AbstractVector{<:AbstractString})
+    all(length(b) >= p.num_chars for b in bitstrings) || throw(ArgumentError("bitstring too short"))
+    join(first(b, p.num_chars) for b in bitstrings)
+end
+
+"""
+    amplitudes(p, bitstrings; slices=1:p.num_slices)
+
+Amplitudes of many bitstrings summed over a slice range: one call replaces the
+per-bitstring loop of `run_simulation` (src/simulation.jl:129-131) and the
+bitstring x slice loop of `QXContexts.execute` (bin/qxrun.jl:83-87).
+"""
+function amplitudes(p::Plan, bitstrings::AbstractVector{<:AbstractString}; slices::UnitRange=1:p.num_slices)
+    nb = length(bitstrings)
+    out = Vector{ComplexF64}(undef, nb * p.result_elems)
+    bits = packbits(p, bitstrings)
+    lock(p.ctx.lock) do
+        check(ccall((:qx_amplitudes, libqxhip), Int32, (Ptr{Cvoid}, Cstring, Int64, Int64, Int64, Ptr{ComplexF64}),
+                    p.handle, bits, nb, first(slices) - 1, last(slices), out))
+    end
+    p.result_elems == 1 ? out : reshape(out, p.result_elems, nb)
+end
+
+"""Keys of the plan's data in the order qx_amplitudes_vjp writes gradients."""
+function keys_and_sizes(p::Plan)
+    nk = Ref{Int64}(0)
+    check(ccall((:qx_plan_info, libqxhip), Int32,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}),
+                p.handle, C_NULL, C_NULL, C_NULL, nk, C_NULL))
+    buf = Vector{UInt8}(undef, 512)
+    n = Ref{Int64}(0)
+    out = Tuple{Symbol,Int}[]
+    for i in 0:nk[]-1
+        check(ccall((:qx_plan_key, libqxhip), Int32, (Ptr{Cvoid}, Int64, Ptr{UInt8}, Csize_t, Ptr{Int64}),
+                    p.handle, i, buf, length(buf), n))
+        push!(out, (Symbol(unsafe_string(pointer(buf))), n[]))
+    end
+    out
+end
+
+"""grad[key] = sum_b cot_b conj(d amp_b / d data[key]) for every data key."""
+function amplitudes_vjp(p::Plan, bitstrings::AbstractVector{<:AbstractString}, cot::AbstractVector{<:Complex})
+    ks = keys_and_sizes(p)
+    g = Vector{ComplexF64}(undef, sum(last, ks))
+    c = convert(Vector{ComplexF64}, cot)
+    bits = packbits(p, bitstrings)
+    lock(p.ctx.lock) do
+        check(ccall((:qx_amplitudes_vjp, libqxhip), Int32,
+                    (Ptr{Cvoid}, Cstring, Int64, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                    p.handle, bits, length(bitstrings), c, g))
+    end
+    out = Dict{Symbol,Vector{ComplexF64}}()
+    o = 0
+    for (k, n) in ks
+        out[k] = g[o+1:o+n]
+        o += n
+    end
+    out
+end
+
+"""
+    gpu_amplitudes(data, p, bitstrings)
+
+Amplitudes as a function of the tensor data, differentiable through ChainRulesCore:
+the pullback returns, per key, the gradient of the plan's contraction (the device
+re-uploads nothing; `data` must be the tensors the plan was finalised with).
+"""
+gpu_amplitudes(data::AbstractDict, p::Plan, bitstrings) = amplitudes(p, bitstrings)
+
+function ChainRulesCore.rrule(::typeof(gpu_amplitudes), data::AbstractDict, p::Plan, bitstrings)
+    y = amplitudes(p, bitstrings)
+    function pullback(ybar)
+        g = amplitudes_vjp(p, bitstrings, vec(unthunk(ybar)))
+        dd = Dict(k => reshape(g[Symbol(k)], size(v)) for (k, v) in data)
+        return NoTangent(), Tangent{typeof(data)}(dd), NoTangent(), NoTangent()
+    end
+    return y, pullback
+end
+
+"""
+    single_amplitude_gpu(dsl, data, bitstring)
+
+Drop-in for `single_amplitude(tnc, plan, bitstring)` (src/simulation.jl:86-91) once
+the network and plan are lowered with `build_compute_graph` and written as DSL text.
+"""
+function single_amplitude_gpu(ctx::Context, dsl::String, data::AbstractDict, bitstring::AbstractString)
+    p = Plan(ctx, dsl, data; batch=1)
+    amplitudes(p, [bitstring])[1]
+end
+
+end # module

@@ -1,0 +1,32 @@
+"""hipGraph replay of qx_amplitudes: every replay, with eager work and host
+synchronisation between replays and new bitstrings each time, must equal the
+eager result bit for bit (same kernels, same order)."""
+
+import numpy as np
+import pytest
+
+import qxoracle as O
+import qxtest
+from qxhip import Executor, circuits, simulation
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("planner,nslice,nb,batch", [("sweep", 0, 16, 16), ("greedy", 2, 10, 4)])
+def test_graph_replays_match_eager(planner, nslice, nb, batch):
+    circ = circuits.create_rqc_circuit(3, 3, 10, seed=21)
+    t, plan, g = qxtest.graph_for(circ, planner=planner, nslice=nslice)
+    dsl = g.to_dsl()
+    eager = Executor(dsl, g.data, batch=batch)
+    graphed = Executor(dsl, g.data, batch=batch, graph=1)
+    for rep in range(4):
+        bits = simulation.amplitudes_uniform(9, 100 + rep, nb)
+        want = eager.amplitudes(bits)
+        got = graphed.amplitudes(bits)
+        assert np.array_equal(got, want), f"replay {rep}"
+        if rep == 0:
+            ref = O.amplitudes(dsl, g.data, bits)
+            assert np.max(qxtest.rel_err(got, ref)) < 1e-12
+        _ = eager.amplitudes(bits[:1])          # eager work on the same device between replays
+    part = graphed.amplitudes(bits, 0, 1)      # another slice range: another graph
+    assert np.array_equal(part, eager.amplitudes(bits, 0, 1))
