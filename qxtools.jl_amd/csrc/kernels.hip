@@ -184,8 +184,9 @@ __global__ __launch_bounds__(256) void k_contract_gemm(const typename Vec<T>::ty
   const int kk = a.nbk < lk ? (1 << a.nbk) : TK;
   const int64_t tiles_m = a.nbm > lm ? (int64_t(1) << (a.nbm - lm)) : 1;
   const int64_t tiles_n = a.nbn > ln ? (int64_t(1) << (a.nbn - ln)) : 1;
-  const int64_t chunks = a.nbk > lk ? (int64_t(1) << (a.nbk - lk)) : 1;
-  const int64_t ntiles = tiles_m * tiles_n * (int64_t(1) << a.nbh);
+  const int64_t chunks_all = a.nbk > lk ? (int64_t(1) << (a.nbk - lk)) : 1;
+  const int64_t chunks = chunks_all / a.ksplit;                    // K chunks of one unit
+  const int64_t ntiles = tiles_m * tiles_n * (int64_t(1) << a.nbh) * a.ksplit;   // units
 
   for (int i = tid; i < TM; i += 256) {
     int64_t oa = 0, oc = 0;
@@ -241,7 +242,10 @@ __global__ __launch_bounds__(256) void k_contract_gemm(const typename Vec<T>::ty
   const int64_t t_end = t_begin + per < ntiles ? t_begin + per : ntiles;
   const int64_t items = t_begin < t_end ? (t_end - t_begin) * chunks : 0;
 
-  auto tile_offsets = [&](int64_t tile, int64_t &oa, int64_t &ob, int64_t &oc) {
+  // a unit is (tile, K split); the split's partial goes to C + split * c_elems
+  auto tile_offsets = [&](int64_t unit, int64_t &oa, int64_t &ob, int64_t &oc) {
+    const int64_t split = unit % a.ksplit;
+    const int64_t tile = unit / a.ksplit;
     int64_t nt = tile % tiles_n;
     int64_t rest = tile / tiles_n;
     int64_t mt = rest % tiles_m;
@@ -253,11 +257,13 @@ __global__ __launch_bounds__(256) void k_contract_gemm(const typename Vec<T>::ty
       if (((mt << lm) >> b) & 1) oa += a.ma[b], oc += a.mc[b];
     for (int b = ln; b < a.nbn; ++b)
       if (((nt << ln) >> b) & 1) ob += a.nb[b], oc += a.nc[b];
+    oc += split * a.c_elems;
   };
   V ra[EA], rb[EB];
   auto fetch = [&](int64_t item) {
-    int64_t tile = t_begin + item / chunks, kc = item % chunks, oa, ob, oc;
-    tile_offsets(tile, oa, ob, oc);
+    int64_t unit = t_begin + item / chunks, oa, ob, oc;
+    int64_t kc = (unit % a.ksplit) * chunks + item % chunks;
+    tile_offsets(unit, oa, ob, oc);
     for (int b = lk; b < a.nbk; ++b)
       if (((kc << lk) >> b) & 1) oa += a.ka[b], ob += a.kb[b];
 #pragma unroll
@@ -325,6 +331,214 @@ __global__ __launch_bounds__(256) void k_contract_gemm(const typename Vec<T>::ty
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Complex64 GEMM on FP32 matrix cores (v_mfma_f32_16x16x4f32), same bit-weight
+// tile gathers as k_contract_gemm.  A 64x64 complex C tile per workgroup, 4 waves
+// of 32x32 (2x2 MFMA blocks of 16x16).  4M: Re += Ar.Br - Ai.Bi and Im += Ar.Bi +
+// Ai.Br as four real products per block; 3M (Gauss): RR = Ar.Br, II = Ai.Bi,
+// SS = (Ar+Ai).(Br+Bi), Re = RR - II, Im = SS - RR - II with three.  FP32
+// accumulation.  MFMA operand layouts: A[i][k] from lane i + 16k, B[k][j] from lane
+// j + 16k, D[i][j] in lane j + 16(i/4), register i%4.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <bool GAUSS>
+__global__ __launch_bounds__(256) void k_contract_gemm_mfma(const float2 *__restrict__ A, const float2 *__restrict__ B,
+                                                            float2 *__restrict__ C, GemmArgs a) {
+  constexpr int TM = 64, TN = 64, TK = 16;
+  constexpr int PA = TM + 1, PB = TN + 1, PC = TN + 1;
+  constexpr int LDS_AB = TK * (PA + PB), LDS_C = TM * PC;
+  constexpr int LDS = LDS_AB > LDS_C ? LDS_AB : LDS_C;
+  constexpr int EA = TM * TK / 256, EB = TN * TK / 256;
+  __shared__ float2 lds[LDS];
+  __shared__ int64_t t_am[TM], t_cm[TM], t_bn[TN], t_cn[TN], t_ak[TK], t_bk[TK];
+  float2 *sA = lds;
+  float2 *sB = lds + TK * PA;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lm = 6, ln = 6, lk = 4;
+  const int rows = a.nbm < lm ? (1 << a.nbm) : TM;
+  const int cols = a.nbn < ln ? (1 << a.nbn) : TN;
+  const int kk = a.nbk < lk ? (1 << a.nbk) : TK;
+  const int64_t tiles_m = a.nbm > lm ? (int64_t(1) << (a.nbm - lm)) : 1;
+  const int64_t tiles_n = a.nbn > ln ? (int64_t(1) << (a.nbn - ln)) : 1;
+  const int64_t chunks_all = a.nbk > lk ? (int64_t(1) << (a.nbk - lk)) : 1;
+  const int64_t chunks = chunks_all / a.ksplit;
+  const int64_t ntiles = tiles_m * tiles_n * (int64_t(1) << a.nbh) * a.ksplit;
+  for (int i = tid; i < TM; i += 256) {
+    int64_t oa = 0, oc = 0;
+    for (int b = 0; b < lm && b < a.nbm; ++b)
+      if ((i >> b) & 1) oa += a.ma[b], oc += a.mc[b];
+    t_am[i] = oa;
+    t_cm[i] = oc;
+  }
+  for (int j = tid; j < TN; j += 256) {
+    int64_t ob = 0, oc = 0;
+    for (int b = 0; b < ln && b < a.nbn; ++b)
+      if ((j >> b) & 1) ob += a.nb[b], oc += a.nc[b];
+    t_bn[j] = ob;
+    t_cn[j] = oc;
+  }
+  for (int l = tid; l < TK; l += 256) {
+    int64_t oa = 0, ob = 0;
+    for (int b = 0; b < lk && b < a.nbk; ++b)
+      if ((l >> b) & 1) oa += a.ka[b], ob += a.kb[b];
+    t_ak[l] = oa;
+    t_bk[l] = ob;
+  }
+  __syncthreads();
+  int64_t offa[EA], offb[EB];
+  int slota[EA], slotb[EB];
+  bool oka[EA], okb[EB];
+#pragma unroll
+  for (int j = 0; j < EA; ++j) {
+    int e = tid + 256 * j, i, l;
+    if (a.a_kfast) i = e / TK, l = e % TK;
+    else l = e / TM, i = e % TM;
+    oka[j] = i < rows && l < kk;
+    offa[j] = oka[j] ? t_am[i] + t_ak[l] : 0;
+    slota[j] = l * PA + i;
+  }
+#pragma unroll
+  for (int j = 0; j < EB; ++j) {
+    int e = tid + 256 * j, jj, l;
+    if (a.b_kfast) jj = e / TK, l = e % TK;
+    else l = e / TN, jj = e % TN;
+    okb[j] = jj < cols && l < kk;
+    offb[j] = okb[j] ? t_bn[jj] + t_bk[l] : 0;
+    slotb[j] = l * PB + jj;
+  }
+  const int64_t nblk = gridDim.x;
+  int64_t bid = blockIdx.x;
+  if (nblk % 8 == 0) bid = (bid % 8) * (nblk / 8) + bid / 8;
+  const int64_t per = (ntiles + nblk - 1) / nblk;
+  const int64_t t_begin = bid * per;
+  const int64_t t_end = t_begin + per < ntiles ? t_begin + per : ntiles;
+  const int64_t items = t_begin < t_end ? (t_end - t_begin) * chunks : 0;
+  auto tile_offsets = [&](int64_t unit, int64_t &oa, int64_t &ob, int64_t &oc) {
+    const int64_t split = unit % a.ksplit;
+    const int64_t tile = unit / a.ksplit;
+    int64_t nt = tile % tiles_n;
+    int64_t rest = tile / tiles_n;
+    int64_t mt = rest % tiles_m;
+    int64_t h = rest / tiles_m;
+    oa = 0, ob = 0, oc = 0;
+    for (int b = 0; b < a.nbh; ++b)
+      if ((h >> b) & 1) oa += a.ha[b], ob += a.hb[b], oc += a.hc[b];
+    for (int b = lm; b < a.nbm; ++b)
+      if (((mt << lm) >> b) & 1) oa += a.ma[b], oc += a.mc[b];
+    for (int b = ln; b < a.nbn; ++b)
+      if (((nt << ln) >> b) & 1) ob += a.nb[b], oc += a.nc[b];
+    oc += split * a.c_elems;
+  };
+  float2 ra[EA], rb[EB];
+  auto fetch = [&](int64_t item) {
+    int64_t unit = t_begin + item / chunks, oa, ob, oc;
+    int64_t kc = (unit % a.ksplit) * chunks + item % chunks;
+    tile_offsets(unit, oa, ob, oc);
+    for (int b = lk; b < a.nbk; ++b)
+      if (((kc << lk) >> b) & 1) oa += a.ka[b], ob += a.kb[b];
+#pragma unroll
+    for (int j = 0; j < EA; ++j) ra[j] = oka[j] ? A[oa + offa[j]] : float2{0, 0};
+#pragma unroll
+    for (int j = 0; j < EB; ++j) rb[j] = okb[j] ? B[ob + offb[j]] : float2{0, 0};
+  };
+  // wave (wm, wn) owns rows 32*wm.., cols 32*wn..; block (bi, bj) of 16x16
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lq = lane >> 4;
+  f32x4 acc_rr[2][2], acc_ii[2][2], acc_x[2][2];
+  if (items > 0) fetch(0);
+  for (int64_t it = 0; it < items; ++it) {
+    const int64_t kc = it % chunks;
+    if (kc == 0) {
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          acc_rr[bi][bj] = f32x4{0, 0, 0, 0};
+          acc_ii[bi][bj] = f32x4{0, 0, 0, 0};
+          acc_x[bi][bj] = f32x4{0, 0, 0, 0};
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < EA; ++j) {
+      float2 v = ra[j];
+      if (a.conj_a) v.y = -v.y;
+      sA[slota[j]] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < EB; ++j) {
+      float2 v = rb[j];
+      if (a.conj_b) v.y = -v.y;
+      sB[slotb[j]] = v;
+    }
+    __syncthreads();
+    if (it + 1 < items) fetch(it + 1);
+#pragma unroll
+    for (int k0 = 0; k0 < TK; k0 += 4) {
+      float2 fa[2], fb[2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi) fa[bi] = sA[(k0 + lq) * PA + 32 * wm + 16 * bi + li];
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) fb[bj] = sB[(k0 + lq) * PB + 32 * wn + 16 * bj + li];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          acc_rr[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[bi].x, fb[bj].x, acc_rr[bi][bj], 0, 0, 0);
+          acc_ii[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[bi].y, fb[bj].y, acc_ii[bi][bj], 0, 0, 0);
+          if (GAUSS) {
+            acc_x[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[bi].x + fa[bi].y, fb[bj].x + fb[bj].y,
+                                                                  acc_x[bi][bj], 0, 0, 0);
+          } else {
+            acc_x[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[bi].x, fb[bj].y, acc_x[bi][bj], 0, 0, 0);
+            acc_x[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[bi].y, fb[bj].x, acc_x[bi][bj], 0, 0, 0);
+          }
+        }
+    }
+    if (kc == chunks - 1) {
+      int64_t oa, ob, oc;
+      tile_offsets(t_begin + it / chunks, oa, ob, oc);
+      __syncthreads();
+      float2 *sC = lds;
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = 32 * wm + 16 * bi + 4 * lq + r, j = 32 * wn + 16 * bj + li;
+            const float rr = acc_rr[bi][bj][r], ii = acc_ii[bi][bj][r], x = acc_x[bi][bj][r];
+            sC[i * PC + j] = GAUSS ? float2{rr - ii, x - rr - ii} : float2{rr - ii, x};
+          }
+      __syncthreads();
+      float2 *Cb = C + oc;
+      for (int e = tid; e < TM * TN; e += 256) {
+        int i, j;
+        if (a.c_nfast) i = e / TN, j = e % TN;
+        else j = e / TM, i = e % TM;
+        if (i < rows && j < cols) Cb[t_cm[i] + t_cn[j]] = sC[i * PC + j];
+      }
+    }
+  }
+}
+
+// C[i] = sum over K-split partials ws[s * n + i].
+template <typename T>
+__global__ void k_sum_splits(typename Vec<T>::type *__restrict__ C, const typename Vec<T>::type *__restrict__ ws,
+                             int64_t n, int ksplit) {
+  using V = typename Vec<T>::type;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    V acc = ws[i];
+    for (int s = 1; s < ksplit; ++s) {
+      V v = ws[(int64_t)s * n + i];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    C[i] = acc;
+  }
+}
+
 // One wave per C element, the K range split over lanes, shuffle reduction.
 template <typename T>
 __global__ __launch_bounds__(256) void k_contract_reduce(const typename Vec<T>::type *__restrict__ A,
@@ -368,7 +582,7 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const typename Vec<T>::
 // launch to what the slabs need.
 constexpr int kRowLdsElems = 4096;  // complex elements of staged B at most
 
-template <typename T, int NK, int NC>
+template <typename T, int NK, int NC, bool UB>
 __global__ __launch_bounds__(256) void k_contract_rows(const typename Vec<T>::type *__restrict__ A,
                                                        const typename Vec<T>::type *__restrict__ B,
                                                        typename Vec<T>::type *__restrict__ C, RowArgs a) {
@@ -402,6 +616,32 @@ __global__ __launch_bounds__(256) void k_contract_rows(const typename Vec<T>::ty
         av[k] = A[oa + a.ka[k]];
         if (a.conj_a) av[k].y = -av[k].y;
       }
+    }
+    if (UB) {
+      // no H bit varies inside the workgroup: B addresses are wave-uniform
+      const V *Bu = B + obh;
+      if (live) {
+        V *Cr = C + oc;
+        for (int n0 = 0; n0 < nn; n0 += NC) {
+          V acc[NC];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) acc[c] = V{0, 0};
+#pragma unroll
+          for (int k = 0; k < NK; ++k) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              if (n0 + c < nn) {
+                V bv = Bu[a.kb[k] + a.nb[n0 + c]];
+                if (a.conj_b) bv.y = -bv.y;
+                cmac<T>(acc[c], av[k], bv, 0, 0);
+              }
+          }
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            if (n0 + c < nn) Cr[a.nc[n0 + c]] = acc[c];
+        }
+      }
+      continue;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < nslab * kn; e += 256) {
@@ -437,7 +677,6 @@ __global__ __launch_bounds__(256) void k_contract_rows(const typename Vec<T>::ty
     }
   }
 }
-
 
 // dst[map(e)] += src[e]: src dense over dims (first fastest), map(e) = sum digit*stride.
 struct AddArgs {
@@ -578,7 +817,7 @@ void gemm_launch(int prec, const void *A, const void *B, void *C, const GemmArgs
   while ((1 << lm) < TM) ++lm;
   while ((1 << ln) < TN) ++ln;
   int64_t tiles = (a.nbm > lm ? (int64_t(1) << (a.nbm - lm)) : 1) * (a.nbn > ln ? (int64_t(1) << (a.nbn - ln)) : 1) *
-                  (int64_t(1) << a.nbh);
+                  (int64_t(1) << a.nbh) * a.ksplit;
   // persistent grid: a few workgroups per CU, each walking a contiguous tile range
   const int64_t cap = 256 * 4;
   int g = (int)(tiles < cap ? tiles : cap);
@@ -596,12 +835,21 @@ void rows_launch(int prec, const void *A, const void *B, void *C, const RowArgs 
   int g = grid_for(int64_t(1) << a.nbr, 256);
   int hl = 0;
   for (int b = 0; b < 8; ++b) hl += (a.hlo >> b) & 1;
+  if (a.hlo == 0) {
+    if (prec == QX_C128)
+      hipLaunchKernelGGL((k_contract_rows<double, NK, NC, true>), dim3(g), dim3(256), 0, s, (const double2 *)A,
+                         (const double2 *)B, (double2 *)C, a);
+    else
+      hipLaunchKernelGGL((k_contract_rows<float, NK, NC, true>), dim3(g), dim3(256), 0, s, (const float2 *)A,
+                         (const float2 *)B, (float2 *)C, a);
+    return;
+  }
   size_t smem = (size_t)(NK * a.nn) << hl;
   if (prec == QX_C128)
-    hipLaunchKernelGGL((k_contract_rows<double, NK, NC>), dim3(g), dim3(256), smem * sizeof(double2), s,
+    hipLaunchKernelGGL((k_contract_rows<double, NK, NC, false>), dim3(g), dim3(256), smem * sizeof(double2), s,
                        (const double2 *)A, (const double2 *)B, (double2 *)C, a);
   else
-    hipLaunchKernelGGL((k_contract_rows<float, NK, NC>), dim3(g), dim3(256), smem * sizeof(float2), s,
+    hipLaunchKernelGGL((k_contract_rows<float, NK, NC, false>), dim3(g), dim3(256), smem * sizeof(float2), s,
                        (const float2 *)A, (const float2 *)B, (float2 *)C, a);
 }
 
@@ -634,14 +882,36 @@ namespace {
 
 }  // namespace
 
-void launch_contract_gemm(int prec, const void *A, const void *B, void *C, const GemmArgs &args, void *stream) {
+void launch_contract_gemm(int prec, const void *A, const void *B, void *C, void *ws, const GemmArgs &args,
+                          void *stream) {
   hipStream_t s = (hipStream_t)stream;
-  switch (args.tile) {
-    case 0: gemm_launch<128, 16, 16>(prec, A, B, C, args, s); break;
-    case 1: gemm_launch<128, 32, 16>(prec, A, B, C, args, s); break;
-    default: gemm_launch<64, 64, 16>(prec, A, B, C, args, s); break;
+  void *dst = args.ksplit > 1 ? ws : C;
+  if (args.tile == 3) {   // complex64 on FP32 matrix cores, 64x64 tiles
+    int64_t tiles = (args.nbm > 6 ? (int64_t(1) << (args.nbm - 6)) : 1) *
+                    (args.nbn > 6 ? (int64_t(1) << (args.nbn - 6)) : 1) * (int64_t(1) << args.nbh) * args.ksplit;
+    int g = (int)(tiles < 1024 ? tiles : 1024);
+    if (prec == QX_C64_3M)
+      hipLaunchKernelGGL((k_contract_gemm_mfma<true>), dim3(g), dim3(256), 0, s, (const float2 *)A,
+                         (const float2 *)B, (float2 *)dst, args);
+    else
+      hipLaunchKernelGGL((k_contract_gemm_mfma<false>), dim3(g), dim3(256), 0, s, (const float2 *)A,
+                         (const float2 *)B, (float2 *)dst, args);
+  } else switch (args.tile) {
+    case 0: gemm_launch<128, 16, 16>(prec, A, B, dst, args, s); break;
+    case 1: gemm_launch<128, 32, 16>(prec, A, B, dst, args, s); break;
+    default: gemm_launch<64, 64, 16>(prec, A, B, dst, args, s); break;
   }
   check_launch("contract_gemm");
+  if (args.ksplit > 1) {
+    int g = grid_for(args.c_elems, 256);
+    if (prec == QX_C128)
+      hipLaunchKernelGGL(k_sum_splits<double>, dim3(g), dim3(256), 0, s, (double2 *)C, (const double2 *)ws,
+                         args.c_elems, args.ksplit);
+    else
+      hipLaunchKernelGGL(k_sum_splits<float>, dim3(g), dim3(256), 0, s, (float2 *)C, (const float2 *)ws,
+                         args.c_elems, args.ksplit);
+    check_launch("sum_splits");
+  }
 }
 
 void launch_contract_reduce(int prec, const void *A, const void *B, void *C, int64_t c_elems, const BitArgs &args,

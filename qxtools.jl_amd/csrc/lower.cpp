@@ -93,7 +93,7 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
 //  * Reduce : few C elements (<= 2^14) and K >= 256
 //  * Bits   : everything else with power-of-two dims
 //  * Radix  : any other dims
-void set_kernel_args(Step &st) {
+void set_kernel_args(Step &st, int precision) {
   if (st.kern == Kern::Radix) {
     RadixArgs &a = st.radix;
     a.ncl = (int)st.cl.size();
@@ -176,7 +176,9 @@ void set_kernel_args(Step &st) {
     for (int i = 0; i < nn; ++i) g.nb[i] = nbits[i].wb, g.nc[i] = nbits[i].wc;
     for (int i = 0; i < nk; ++i) g.ka[i] = kbits[i].first, g.kb[i] = kbits[i].second;
     g.tile = nn <= 4 ? 0 : nn == 5 ? 1 : 2;
-    const int lm = g.tile == 2 ? 6 : 7, ln = g.tile == 0 ? 4 : g.tile == 1 ? 5 : 6, lk = 4;
+    // complex64 with wide N and M: FP32 matrix cores (4M or 3M)
+    if (precision != QX_C128 && nn >= 6 && nm >= 6) g.tile = 3;
+    const int lm = g.tile >= 2 ? 6 : 7, ln = g.tile == 0 ? 4 : g.tile == 1 ? 5 : 6, lk = 4;
     auto min_w = [](const int64_t *w, int n) {
       int64_t m = INT64_MAX;
       for (int i = 0; i < n; ++i) m = std::min(m, w[i]);
@@ -187,6 +189,13 @@ void set_kernel_args(Step &st) {
     g.a_kfast = ka_min < min_w(g.ma, std::min(nm, lm));
     g.b_kfast = kb_min_b < min_w(g.nb, std::min(nn, ln));
     g.c_nfast = min_w(g.nc, std::min(nn, ln)) < min_w(g.mc, std::min(nm, lm));
+    // split K over workgroups when the tiles alone cannot fill the chip
+    const int64_t ntiles = (int64_t(1) << g.nbh) * (nm > lm ? int64_t(1) << (nm - lm) : 1) *
+                           (nn > ln ? int64_t(1) << (nn - ln) : 1);
+    const int64_t chunks = nk > lk ? int64_t(1) << (nk - lk) : 1;
+    g.c_elems = st.c_elems;
+    while (ntiles * g.ksplit < 1024 && chunks / g.ksplit >= 8) g.ksplit *= 2;
+    st.ws_elems = g.ksplit > 1 ? g.ksplit * st.c_elems : 0;
     return;
   }
   if (st.c_elems <= (int64_t(1) << 14) && nk >= 8) {
@@ -198,7 +207,7 @@ void set_kernel_args(Step &st) {
 
 Step make_step(const Value &A, const std::vector<int> &alab, bool conj_a, const Value &B,
                const std::vector<int> &blab, bool conj_b, const Value &C, const std::vector<int> &clab,
-               int64_t batch) {
+               int64_t batch, int precision) {
   Step st;
   std::map<int, std::pair<int64_t, int64_t>> la, lb;  // label -> (dim, stride)
   for (size_t j = 0; j < alab.size(); ++j) la[alab[j]] = {A.dims[j], A.strides[j]};
@@ -255,13 +264,13 @@ Step make_step(const Value &A, const std::vector<int> &alab, bool conj_a, const 
   st.kern = (all_pow2 && nbc <= kMaxBits && nbk <= kMaxBits) ? Kern::Bits : Kern::Radix;
   if (st.kern == Kern::Radix && ((int)st.cl.size() > kMaxLegs || (int)st.kl.size() > kMaxLegs))
     throw Error(QX_ERR_UNSUPPORTED, "step with more than 48 legs of non power-of-two dims");
-  set_kernel_args(st);
+  set_kernel_args(st, precision);
   st.bits.conj_a = st.gemm.conj_a = st.rows.conj_a = st.radix.conj_a = conj_a;
   st.bits.conj_b = st.gemm.conj_b = st.rows.conj_b = st.radix.conj_b = conj_b;
   return st;
 }
 
-Lowered lower(const Program &p, int64_t batch, bool keep_all) {
+Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision) {
   if (!pow2(batch)) throw Error(QX_ERR_ARGUMENT, "batch must be a power of two");
   Lowered L;
   L.batch = batch;
@@ -328,11 +337,13 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all) {
   for (size_t i = 0; i < nn; ++i) {
     const Node &n = p.nodes[i];
     if (n.op != Op::Ncon) continue;
-    Step st = make_step(L.values[n.ia], n.alab, false, L.values[n.ib], n.blab, false, L.values[i], n.clab, batch);
+    Step st = make_step(L.values[n.ia], n.alab, false, L.values[n.ib], n.blab, false, L.values[i], n.clab, batch,
+                        precision);
     st.node = (int)i;
     st.in_a = n.ia;
     st.in_b = n.ib;
     st.section = L.values[i].dep ? 2 : (n.sdep ? 1 : 0);
+    L.ws_elems = std::max(L.ws_elems, st.ws_elems);
     L.steps.push_back(std::move(st));
   }
 
@@ -416,9 +427,9 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all) {
   return L;
 }
 
-VjpLowered lower_vjp(const Program &p, int64_t batch) {
+VjpLowered lower_vjp(const Program &p, int64_t batch, int precision) {
   VjpLowered V;
-  V.fwd = lower(p, batch, true);
+  V.fwd = lower(p, batch, true, precision);
   const Lowered &L = V.fwd;
   const size_t nn = p.nodes.size();
   // every tensor must feed exactly one consumer (a tree), so each cotangent has
@@ -513,9 +524,11 @@ VjpLowered lower_vjp(const Program &p, int64_t batch) {
         }
       }
       if (side == 0)
-        b.step = make_step(V.cot[fs.node], cn.clab, false, L.values[cn.ib], cn.blab, true, out, cn.alab, batch);
+        b.step = make_step(V.cot[fs.node], cn.clab, false, L.values[cn.ib], cn.blab, true, out, cn.alab, batch,
+                           precision);
       else
-        b.step = make_step(L.values[cn.ia], cn.alab, true, V.cot[fs.node], cn.clab, false, out, cn.blab, batch);
+        b.step = make_step(L.values[cn.ia], cn.alab, true, V.cot[fs.node], cn.clab, false, out, cn.blab, batch,
+                           precision);
       b.step.node = fs.node;
       b.step.section = fs.section;
       // host-side proof that the step and the add stay inside their buffers
@@ -546,6 +559,7 @@ VjpLowered lower_vjp(const Program &p, int64_t batch) {
         if (ex.first + md >= cap)
           throw Error(QX_ERR_STATE, "internal: cotangent add of '" + p.nodes[x].name + "' outside its buffer");
       }
+      V.fwd.ws_elems = std::max(V.fwd.ws_elems, b.step.ws_elems);
       V.bsteps[fs.section].push_back(std::move(b));
     }
   }

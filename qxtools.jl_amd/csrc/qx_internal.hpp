@@ -119,6 +119,8 @@ struct GemmArgs {
   int conj_a = 0, conj_b = 0;
   int a_kfast = 0, b_kfast = 0, c_nfast = 0;  // coalescing direction of tile loads/stores
   int tile = 0;                               // tile configuration index
+  int ksplit = 1;                             // K split over workgroups (partials summed after)
+  int64_t c_elems = 0;                        // dense C size: stride between K-split partials
 };
 
 // One leg of a step in canonical form: dims and element strides in A, B and C.
@@ -136,6 +138,7 @@ struct Step {
   std::vector<Leg> kl;     // contracted legs
   int64_t H = 1, M = 1, N = 1, K = 1;  // class products (beta counted in its class)
   int64_t c_elems = 1;
+  int64_t ws_elems = 0;    // workspace for K-split partials
   Kern kern = Kern::Bits;
   int section = 0;         // 0 shared slice-free, 1 shared per slice, 2 per bitstring batch
   BitArgs bits;            // Kern::Bits / Kern::Reduce
@@ -155,17 +158,18 @@ struct Lowered {
   std::vector<Value> values;        // per node
   std::vector<Step> steps;          // ncon nodes in order
   int64_t arena_elems = 0;          // forward arena size (elements)
+  int64_t ws_elems = 0;             // largest K-split workspace of any step
   int64_t out_elems = 0;            // elements of one output tensor incl. batch
   int64_t result_elems = 1;         // complex values per bitstring
 };
 
-Lowered lower(const Program &p, int64_t batch, bool keep_all = false);
+Lowered lower(const Program &p, int64_t batch, bool keep_all = false, int precision = QX_C128);
 
 // C[clab] = sum A[alab] B[blab] over the given value layouts (C dense), kernel
 // chosen and arguments filled; conj flags conjugate an operand on load.
 Step make_step(const Value &A, const std::vector<int> &alab, bool conj_a, const Value &B,
                const std::vector<int> &blab, bool conj_b, const Value &C, const std::vector<int> &clab,
-               int64_t batch);
+               int64_t batch, int precision = QX_C128);
 
 // Reverse-mode plan of one batch size (qx_amplitudes_vjp).  The forward keeps
 // every value of a slice; each forward step C = A.B contributes Abar = Cbar.conj(B)
@@ -194,7 +198,7 @@ struct VjpLowered {
   int root = -1;
 };
 
-VjpLowered lower_vjp(const Program &p, int64_t batch);
+VjpLowered lower_vjp(const Program &p, int64_t batch, int precision = QX_C128);
 
 // First-fit offsets for [start, end) intervals of given sizes (elements), 256-byte
 // aligned for element size `esize`.  Returns the total size; offsets written out.
@@ -203,10 +207,12 @@ int64_t first_fit(const std::vector<int64_t> &start, const std::vector<int64_t> 
 
 // ----------------------------------------------------------------------------
 // Kernel launch interface (kernels.hip).
-void set_kernel_args(Step &st);
+void set_kernel_args(Step &st, int precision = QX_C128);
 void launch_contract_bits(int prec, const void *A, const void *B, void *C, int64_t c_elems,
                           const BitArgs &args, void *stream);
-void launch_contract_gemm(int prec, const void *A, const void *B, void *C, const GemmArgs &args,
+// With args.ksplit > 1, `ws` receives ksplit partials of C which are then summed
+// into C.
+void launch_contract_gemm(int prec, const void *A, const void *B, void *C, void *ws, const GemmArgs &args,
                           void *stream);
 void launch_contract_rows(int prec, const void *A, const void *B, void *C, const RowArgs &args,
                           void *stream);

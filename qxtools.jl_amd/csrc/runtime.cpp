@@ -89,7 +89,7 @@ struct qx_plan {
   std::vector<DevBuf> data;                 // per key
   std::map<int64_t, qx::Lowered> lowered;   // per batch size
   std::map<int64_t, qx::VjpLowered> vjp_lowered;
-  DevBuf varena, cot, temp, grad, ampcot;
+  DevBuf varena, cot, temp, grad, ampcot, ws;
   DevBuf arena, out_off, out_q, bits, acc, outbuf, relem_off;
   int64_t arena_cap_bytes = 0;
   std::vector<double> step_ms;
@@ -119,6 +119,7 @@ struct qx_plan {
     temp.release();
     grad.release();
     ampcot.release();
+    ws.release();
     for (auto e : events) (void)hipEventDestroy(e);
     drop_graphs();
   }
@@ -129,7 +130,7 @@ namespace {
 const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch) {
   auto it = P->lowered.find(batch);
   if (it != P->lowered.end()) return it->second;
-  qx::Lowered L = qx::lower(P->prog, batch);
+  qx::Lowered L = qx::lower(P->prog, batch, false, P->prec);
   size_t need = (size_t)L.arena_elems * P->esize();
   if (P->arena_cap_bytes > 0 && (int64_t)need > P->arena_cap_bytes)
     throw qx::Error(QX_ERR_MEMORY, "arena of " + std::to_string(need >> 20) + " MiB for batch " +
@@ -173,7 +174,7 @@ void launch_step(qx_plan *P, const qx::Step &st, void *A, void *B, void *C) {
   hipStream_t stream = P->ctx->stream;
   switch (st.kern) {
     case qx::Kern::Radix: qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, st.radix, stream); break;
-    case qx::Kern::Gemm: qx::launch_contract_gemm(P->prec, A, B, C, st.gemm, stream); break;
+    case qx::Kern::Gemm: qx::launch_contract_gemm(P->prec, A, B, C, P->ws.p, st.gemm, stream); break;
     case qx::Kern::Reduce: qx::launch_contract_reduce(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
     case qx::Kern::Rows: qx::launch_contract_rows(P->prec, A, B, C, st.rows, stream); break;
     default: qx::launch_contract_bits(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
@@ -241,6 +242,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   hipStream_t stream = P->ctx->stream;
   const size_t es = P->esize();
   bool moved = P->arena.reserve(std::max<size_t>((size_t)L.arena_elems * es, 256));
+  moved |= P->ws.reserve(std::max<size_t>((size_t)L.ws_elems * es, 256));
 
   // output tables
   std::vector<int64_t> ooff;
@@ -356,6 +358,7 @@ std::string kern_variant(const qx::Step &st) {
       return "k_contract_rows<" + std::to_string(st.rows.nk) + "," + std::to_string(nc) + ">";
     }
     case qx::Kern::Gemm:
+      if (st.gemm.tile == 3) return "k_contract_gemm_mfma";
       return st.gemm.tile == 0 ? "k_contract_gemm<128,16,16>" : st.gemm.tile == 1 ? "k_contract_gemm<128,32,16>"
                                                                                    : "k_contract_gemm<64,64,16>";
     case qx::Kern::Reduce: return "k_contract_reduce";
@@ -395,7 +398,7 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
 const qx::VjpLowered &ensure_vjp(qx_plan *P, int64_t batch) {
   auto it = P->vjp_lowered.find(batch);
   if (it != P->vjp_lowered.end()) return it->second;
-  qx::VjpLowered V = qx::lower_vjp(P->prog, batch);
+  qx::VjpLowered V = qx::lower_vjp(P->prog, batch, P->prec);
   const size_t es = P->esize();
   size_t need = (size_t)(V.fwd.arena_elems + V.cot_elems + V.temp_elems) * es;
   if (P->arena_cap_bytes > 0 && (int64_t)need > P->arena_cap_bytes)
@@ -420,6 +423,7 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
   const int64_t nch = P->prog.num_chars;
   const int64_t R = L.result_elems;
   P->varena.reserve(std::max<size_t>((size_t)L.arena_elems * es, 256));
+  if (P->ws.reserve(std::max<size_t>((size_t)L.ws_elems * es, 256))) P->drop_graphs();
   P->cot.reserve(std::max<size_t>((size_t)V.cot_elems * es, 256));
   P->temp.reserve(std::max<size_t>((size_t)V.temp_elems * es, 256));
   std::vector<int64_t> key_off;
@@ -750,7 +754,7 @@ qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t bat
     if (!json || cap == 0) throw qx::Error(QX_ERR_ARGUMENT, "null output buffer");
     if (precision < QX_C128 || precision > QX_C64_BF16X3) throw qx::Error(QX_ERR_ARGUMENT, "unknown precision");
     qx::Program prog = qx::parse_program(dsl, len);
-    qx::Lowered L = qx::lower(prog, batch);
+    qx::Lowered L = qx::lower(prog, batch, false, precision);
     std::ostringstream os;
     os.precision(17);
     lowered_json(os, prog, L, precision == QX_C128 ? 16 : 8, true);
@@ -758,7 +762,7 @@ qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t bat
     // reverse-mode lowering (its own host-side bounds proof runs here too)
     std::string v;
     try {
-      qx::VjpLowered V = qx::lower_vjp(prog, batch);
+      qx::VjpLowered V = qx::lower_vjp(prog, batch, precision);
       size_t nb = V.bsteps[0].size() + V.bsteps[1].size() + V.bsteps[2].size();
       v = ",\"vjp\":{\"backward_steps\":" + std::to_string(nb) + ",\"forward_elems\":" +
           std::to_string(V.fwd.arena_elems) + ",\"cot_elems\":" + std::to_string(V.cot_elems) +

@@ -59,7 +59,8 @@ def err(a, r):
 
 
 GEMM_CASES = [(0, 12, 5, 5), (0, 5, 3, 3), (0, 7, 4, 4), (0, 8, 5, 5), (1, 6, 4, 5), (2, 6, 5, 3), (0, 6, 6, 6),
-              (0, 9, 7, 4), (1, 5, 8, 6), (0, 12, 7, 5), (0, 7, 3, 8)]
+              (0, 9, 7, 4), (1, 5, 8, 6), (0, 12, 7, 5), (0, 7, 3, 8),
+              (0, 6, 6, 12), (1, 7, 5, 10)]
 
 
 @pytest.mark.parametrize("case", range(len(GEMM_CASES)))
@@ -115,3 +116,16 @@ def test_gemm_kernel_c64(case):
     dsl, data, ref = single_step(300 + case, nh, nm, nn, nk)
     out, kern = run(dsl, data, "c64_4m")
     assert err(out, ref) < 5e-6
+
+
+MFMA_CASES = [(0, 6, 6, 6), (0, 12, 7, 5), (0, 6, 6, 12), (1, 7, 6, 6), (0, 9, 8, 4)]
+
+
+@pytest.mark.parametrize("case", MFMA_CASES)
+@pytest.mark.parametrize("precision,tol", [("c64_4m", 2e-5), ("c64_3m", 5e-5)])
+def test_gemm_mfma_c64(case, precision, tol):
+    """complex64 GEMM steps on FP32 matrix cores, four or three real products."""
+    dsl, data, ref = single_step(500 + sum(case), *case)
+    out, kern = run(dsl, data, precision)
+    assert kern == "gemm"
+    assert err(out, ref) < tol

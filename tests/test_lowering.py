@@ -134,3 +134,11 @@ def test_kernel_choice_matches_gpu_test_shapes():
         assert _native.inspect(dsl)["steps"][0]["kernel"] == "reduce", shape
     dsl, _, _ = K.single_step(5, 1, 2, 2, 2, dims={1: 3, 2: 5, 5: 3, 6: 7})
     assert _native.inspect(dsl)["steps"][0]["kernel"] == "radix"
+
+
+def test_complex64_wide_gemm_uses_matrix_cores():
+    for case in K.MFMA_CASES:
+        dsl, _, _ = K.single_step(1, *case)
+        steps = _native.inspect(dsl, precision="c64_4m")["steps"]
+        assert steps[0]["variant"] == "k_contract_gemm_mfma", case
+        assert _native.inspect(dsl)["steps"][0]["variant"] != "k_contract_gemm_mfma"
