@@ -13,6 +13,8 @@ from __future__ import annotations
 
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -33,7 +35,7 @@ def kernel_summary(stats: dict, prec: str) -> dict:
     """Per kernel family: launches, time, algorithmic bytes and flops (per-step
     counts from the plan; bytes = e*B*(MK+KN+MN), flops = 8*B*M*N*K)."""
     fam: dict = {}
-    batch = max(stats["batches"], key=lambda b: b["batch"])
+    batch = next(b for b in stats["batches"] if b.get("active"))
     for st in batch["steps"]:
         calls = st.get("calls", 0)
         if not calls:
@@ -48,7 +50,7 @@ def kernel_summary(stats: dict, prec: str) -> dict:
 
 def plan_roofline(stats: dict, prec: str, ms_per_step: float, steps: int) -> float:
     """sum_s max(flops_s/P, bytes_s/BW) over the launches of one step / measured."""
-    batch = max(stats["batches"], key=lambda b: b["batch"])
+    batch = next(b for b in stats["batches"] if b.get("active"))
     t = 0.0
     for st in batch["steps"]:
         calls = st.get("calls", 0) / max(steps, 1)
@@ -77,6 +79,78 @@ def cpu_baseline(dsl: str, data: dict, bits: list, seconds: float) -> dict:
                       f"by the oracle ({el:.1f} s)"}
 
 
+def c64_comparison(dsl: str, data: dict, bits: list, ref: np.ndarray, a, precs=("c64_4m", "c64_3m")) -> dict:
+    """Other precisions on the same plan: throughput and per-amplitude relative error
+    against `ref` (the complex128 GPU result)."""
+    import torch
+
+    from qxhip.simulation import Executor
+
+    res = {}
+    for prec in precs:
+        ex = Executor(dsl, data, precision=prec, device=0, batch=min(256, len(bits)), graph=a.graph)
+        for _ in range(a.warmup):
+            ex.amplitudes(bits)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            amps = ex.amplitudes(bits)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        amps = np.asarray(amps).reshape(-1)
+        rel = np.abs(amps - ref.reshape(-1)) / np.abs(ref.reshape(-1))
+        res[prec] = {"value": amps.size * a.steps / el, "ms_per_step": 1e3 * el / a.steps,
+                     "rel_err_max": float(rel.max()), "rel_err_p99": float(np.quantile(rel, 0.99)),
+                     "rel_err_median": float(np.median(rel)),
+                     "normwise_err": float(np.linalg.norm(amps - ref.reshape(-1)) / np.linalg.norm(ref))}
+        ex.close()
+        log(f"{prec}: {res[prec]['value']:.0f} amplitudes/s, max rel err {res[prec]['rel_err_max']:.2e}")
+    return res
+
+
+def c128_reference(dsl, data, bits, w):
+    from qxhip.simulation import Executor
+
+    ex = Executor(dsl, data, precision="c128", device=0, batch=min(256, len(bits)))
+    r = np.asarray(ex.amplitudes(bits))
+    ex.close()
+    return r
+
+
+def grad_bench(a) -> dict:
+    """Config-5 style: loss = sum |amp|^2 over the bitstrings of a parameterised
+    circuit; forward amplitudes/s and gradient / forward time."""
+    import torch
+
+    from qxhip import gradients, workloads
+
+    w = workloads.PARAM_WORKLOAD
+    prec = a.precision or w.precision
+    pc = gradients.ParamCircuit(w.circuit(), planner="greedy", nslice=w.sliced, precision=prec,
+                                batch=min(256, w.num_bitstrings), **w.plan_kwargs)
+    bits = w.bitstrings()
+    for _ in range(a.warmup):
+        pc.loss_and_grad(bits)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        pc.amplitudes(bits)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / a.steps
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        loss, grad, _ = pc.loss_and_grad(bits)
+    torch.cuda.synchronize()
+    both = (time.perf_counter() - t0) / a.steps
+    return {"metric": "gradient time / forward time", "value": (both - fwd) / fwd, "unit": "ratio",
+            "higher_is_better": False, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "forward_ms": 1e3 * fwd, "forward_plus_gradient_ms": 1e3 * both,
+            "forward_amplitudes_per_s": len(bits) / fwd, "params": int(pc.num_params),
+            "dtype": "c128" if prec == "c128" else "c64", "data": "synthetic",
+            "config": {"workload": w.name, "qubits": w.qubits, "depth": w.depth, "bitstrings": len(bits),
+                       "sliced_bonds": w.sliced, "note": w.note}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,18 +162,23 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="replay each call from a captured hipGraph")
+    ap.add_argument("--no-c64", action="store_true", help="skip the complex64 comparison")
+    ap.add_argument("--grad", action="store_true", help="gradient / forward time of the parameterised workload")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the contraction path on a single "
               "MI355X (multi-GPU slice sharding is a design, not a measured path)", file=sys.stderr)
         return 2
 
-    import numpy as np
+    import numpy as np  # noqa: F811
     import torch
 
     from qxhip import workloads
     from qxhip.simulation import Executor
 
+    if a.grad:
+        print(json.dumps(grad_bench(a)), flush=True)
+        return 0
     w = workloads.WORKLOADS[a.workload]
     prec = a.precision or w.precision
     dsl, data, bits = w.load()
@@ -116,7 +195,8 @@ def main() -> int:
     el = time.perf_counter() - t0
     log(f"timed {a.steps} steps in {el:.3f} s (graph replay {'on' if a.graph else 'off'})")
     ms_step = 1e3 * el / a.steps
-    value = len(bits) * a.steps / el
+    per_call = len(bits) * ex.plan.result_elems      # open output legs: 2^m amplitudes per bitstring
+    value = per_call * a.steps / el
     # kernel durations: the same steps launched eagerly with a HIP event pair around
     # every launch on the library's stream (graph replay records no events)
     ex.plan.set_option("profile", 1)
@@ -152,12 +232,20 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "c128" if prec == "c128" else "c64",
         "data": "synthetic: seeded random grid circuit and seeded bitstrings",
         "config": {"workload": w.name, "qubits": w.qubits, "depth": w.depth, "bitstrings": len(bits),
-                   "sliced_bonds": w.sliced, "precision": prec, "batch": ex.plan.stats()["batches"][-1]["batch"]},
+                   "amplitudes_per_step": per_call, "note": w.note or None,
+                   "sliced_bonds": w.sliced, "precision": prec,
+                   "batch": next(b for b in stats["batches"] if b.get("active"))["batch"]},
         "roofline": roof,
         "plan_roofline_frac": plan_roofline(stats, prec, ms_step, a.steps),
         "ms_per_step_eager_with_events": 1e3 * el_ev / a.steps,
         "kernels": kernels,
     }
+    if not a.no_c64 and prec == "c128":
+        out["c64"] = c64_comparison(dsl, data, bits, np.asarray(amps), a)
+    elif not a.no_c64:
+        ref = c128_reference(dsl, data, bits, w)
+        out["error_vs_c128"] = c64_comparison(dsl, data, bits, ref, a, precs=(prec,))[prec]
+        out["error_vs_c128"].pop("value"), out["error_vs_c128"].pop("ms_per_step")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(dsl, data, bits, a.cpu_seconds)
     _ = np.asarray(amps)

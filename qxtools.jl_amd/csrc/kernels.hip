@@ -548,17 +548,20 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const typename Vec<T>::
   using V = typename Vec<T>::type;
   const int lane = threadIdx.x & 63;
   const int64_t nk = int64_t(1) << a.nbk;
+  const int64_t kper = nk / a.ksplit;         // K range of one wave
   // per-lane offsets of the low 6 k bits
   int64_t la = 0, lb = 0;
   for (int b = 0; b < 6 && b < a.nbk; ++b)
     if ((lane >> b) & 1) la += a.ka[b], lb += a.kb[b];
   const int64_t waves = (int64_t)gridDim.x * 4;
-  for (int64_t idx = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); idx < c_elems; idx += waves) {
+  const int64_t units = c_elems * a.ksplit;   // (C element, K range); partials land at C[split*c_elems + idx]
+  for (int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); u < units; u += waves) {
+    const int64_t idx = u % c_elems, split = u / c_elems;
     int64_t oa = 0, ob = 0;
     for (int b = 0; b < a.nbc; ++b)
       if ((idx >> b) & 1) oa += a.wa[b], ob += a.wb[b];
     V acc{0, 0};
-    for (int64_t k0 = 0; k0 < nk; k0 += 64) {
+    for (int64_t k0 = split * kper; k0 < (split + 1) * kper; k0 += 64) {
       if (k0 + lane >= nk) break;
       int64_t ha = 0, hb = 0;
       for (int b = 6; b < a.nbk; ++b)
@@ -569,7 +572,7 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const typename Vec<T>::
       acc.x += __shfl_xor(acc.x, off, 64);
       acc.y += __shfl_xor(acc.y, off, 64);
     }
-    if (lane == 0) C[idx] = acc;
+    if (lane == 0) C[split * c_elems + idx] = acc;
   }
 }
 
@@ -914,17 +917,28 @@ void launch_contract_gemm(int prec, const void *A, const void *B, void *C, void 
   }
 }
 
-void launch_contract_reduce(int prec, const void *A, const void *B, void *C, int64_t c_elems, const BitArgs &args,
-                            void *stream) {
-  int g = grid_for(c_elems, 4);
+void launch_contract_reduce(int prec, const void *A, const void *B, void *C, void *ws, int64_t c_elems,
+                            const BitArgs &args, void *stream) {
+  int g = grid_for(c_elems * args.ksplit, 4);
   hipStream_t s = (hipStream_t)stream;
+  void *dst = args.ksplit > 1 ? ws : C;
   if (prec == QX_C128)
     hipLaunchKernelGGL(k_contract_reduce<double>, dim3(g), dim3(256), 0, s, (const double2 *)A, (const double2 *)B,
-                       (double2 *)C, c_elems, args);
+                       (double2 *)dst, c_elems, args);
   else
     hipLaunchKernelGGL(k_contract_reduce<float>, dim3(g), dim3(256), 0, s, (const float2 *)A, (const float2 *)B,
-                       (float2 *)C, c_elems, args);
+                       (float2 *)dst, c_elems, args);
   check_launch("contract_reduce");
+  if (args.ksplit > 1) {
+    int gs = grid_for(c_elems, 256);
+    if (prec == QX_C128)
+      hipLaunchKernelGGL(k_sum_splits<double>, dim3(gs), dim3(256), 0, s, (double2 *)C, (const double2 *)ws, c_elems,
+                         args.ksplit);
+    else
+      hipLaunchKernelGGL(k_sum_splits<float>, dim3(gs), dim3(256), 0, s, (float2 *)C, (const float2 *)ws, c_elems,
+                         args.ksplit);
+    check_launch("sum_splits");
+  }
 }
 
 void launch_contract_bits(int prec, const void *A, const void *B, void *C, int64_t c_elems, const BitArgs &args,

@@ -200,6 +200,9 @@ void set_kernel_args(Step &st, int precision) {
   }
   if (st.c_elems <= (int64_t(1) << 14) && nk >= 8) {
     st.kern = Kern::Reduce;
+    // one wave per (C element, K range): enough waves to fill the chip, >= 64 k each
+    while (st.c_elems * b.ksplit < 8192 && (int64_t(1) << nk) / b.ksplit >= 128) b.ksplit *= 2;
+    st.ws_elems = b.ksplit > 1 ? b.ksplit * st.c_elems : 0;
     return;
   }
 
@@ -270,12 +273,55 @@ Step make_step(const Value &A, const std::vector<int> &alab, bool conj_a, const 
   return st;
 }
 
-Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision) {
+// Shapes and slice dependence of every node when the slice variables in `fused`
+// (bit j = variable j) are contracted inline instead of looped over: their views
+// pass the viewed tensor through unchanged, so the leg keeps its dimension and the
+// contractions sum over it (by linearity the same as summing the slices).
+static void fused_shapes(const Program &p, uint64_t fused, std::vector<std::vector<int64_t>> &shape,
+                         std::vector<char> &sdep) {
+  const size_t nn = p.nodes.size();
+  shape.assign(nn, {});
+  sdep.assign(nn, 0);
+  for (size_t i = 0; i < nn; ++i) {
+    const Node &n = p.nodes[i];
+    switch (n.op) {
+      case Op::Load:
+      case Op::Output: shape[i] = n.shape; break;
+      case Op::View:
+        shape[i] = shape[n.ia];
+        if ((fused >> n.var_id) & 1) {
+          sdep[i] = sdep[n.ia];
+        } else {
+          shape[i][n.pos] = 1;
+          sdep[i] = 1;
+        }
+        break;
+      case Op::Ncon: {
+        std::map<int, int64_t> dim;
+        for (size_t j = 0; j < n.alab.size(); ++j) dim[n.alab[j]] = shape[n.ia][j];
+        for (size_t j = 0; j < n.blab.size(); ++j) dim[n.blab[j]] = shape[n.ib][j];
+        for (int l : n.clab) shape[i].push_back(dim[l]);
+        sdep[i] = sdep[n.ia] || sdep[n.ib];
+        break;
+      }
+      case Op::Save:
+        shape[i] = shape[n.ia];
+        sdep[i] = sdep[n.ia];
+        break;
+    }
+  }
+}
+
+Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uint64_t fused) {
   if (!pow2(batch)) throw Error(QX_ERR_ARGUMENT, "batch must be a power of two");
   Lowered L;
   L.batch = batch;
+  L.fused = fused;
   const size_t nn = p.nodes.size();
   L.values.resize(nn);
+  std::vector<std::vector<int64_t>> shape;
+  std::vector<char> sdep;
+  fused_shapes(p, fused, shape, sdep);
   std::map<std::string, int> key_index;
   for (size_t i = 0; i < p.keys.size(); ++i) key_index[p.keys[i]] = (int)i;
 
@@ -286,9 +332,9 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision) {
       case Op::Load: {
         v.buf = Buf::Data;
         v.data_key = key_index.at(n.key);
-        v.dims = n.shape;
+        v.dims = shape[i];
         int64_t s = 1;
-        for (int64_t d : n.shape) {
+        for (int64_t d : shape[i]) {
           v.strides.push_back(s);
           s *= d;
         }
@@ -307,20 +353,22 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision) {
       case Op::View: {
         v = L.values[n.ia];
         v.slot = -1;
-        v.var_terms.push_back({n.var_id, v.strides[n.pos]});
-        v.dims[n.pos] = 1;
+        if (!((fused >> n.var_id) & 1)) {
+          v.var_terms.push_back({n.var_id, v.strides[n.pos]});
+          v.dims[n.pos] = 1;
+        }
         break;
       }
       case Op::Ncon: {
         v.buf = Buf::Arena;
-        v.dims = n.shape;
+        v.dims = shape[i];
         v.dep = n.dep;
         int64_t s = 1;
         if (n.dep) {
           v.beta_stride = 1;
           s = batch;
         }
-        for (int64_t d : n.shape) {
+        for (int64_t d : shape[i]) {
           v.strides.push_back(d > 1 ? s : 0);
           if (d > 1) s *= d;
         }
@@ -342,15 +390,14 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision) {
     st.node = (int)i;
     st.in_a = n.ia;
     st.in_b = n.ib;
-    st.section = L.values[i].dep ? 2 : (n.sdep ? 1 : 0);
+    st.section = L.values[i].dep ? 2 : (sdep[i] ? 1 : 0);
     L.ws_elems = std::max(L.ws_elems, st.ws_elems);
     L.steps.push_back(std::move(st));
   }
 
   // result elements per bitstring
-  const Node &sv = p.nodes[p.save];
   L.result_elems = 1;
-  for (int64_t d : sv.shape) L.result_elems *= d;
+  for (int64_t d : shape[p.save]) L.result_elems *= d;
 
   // liveness: schedule positions = sections 0, 1, then output fill, then section 2
   std::vector<int> order;
@@ -427,9 +474,9 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision) {
   return L;
 }
 
-VjpLowered lower_vjp(const Program &p, int64_t batch, int precision) {
+VjpLowered lower_vjp(const Program &p, int64_t batch, int precision, uint64_t fused) {
   VjpLowered V;
-  V.fwd = lower(p, batch, true, precision);
+  V.fwd = lower(p, batch, true, precision, fused);
   const Lowered &L = V.fwd;
   const size_t nn = p.nodes.size();
   // every tensor must feed exactly one consumer (a tree), so each cotangent has

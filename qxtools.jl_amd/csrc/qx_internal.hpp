@@ -87,6 +87,7 @@ struct BitArgs {
   int64_t ka[kMaxBits], kb[kMaxBits];   // weight of each K-index bit in A and B
   int nbc = 0, nbk = 0;
   int conj_a = 0, conj_b = 0;
+  int ksplit = 1;          // Kern::Reduce: K ranges per C element, partials summed after
 };
 
 // Rows kernel: C[r, n] = sum_k A[r, k] B[h(r), k, n] with r over the H and M bits
@@ -154,6 +155,7 @@ struct ArenaSlot {
 
 struct Lowered {
   int64_t batch = 0;
+  uint64_t fused = 0;               // slice variables contracted inline (bit j = variable j)
   std::vector<ArenaSlot> slots;     // arena reuse schedule
   std::vector<Value> values;        // per node
   std::vector<Step> steps;          // ncon nodes in order
@@ -163,7 +165,8 @@ struct Lowered {
   int64_t result_elems = 1;         // complex values per bitstring
 };
 
-Lowered lower(const Program &p, int64_t batch, bool keep_all = false, int precision = QX_C128);
+Lowered lower(const Program &p, int64_t batch, bool keep_all = false, int precision = QX_C128,
+              uint64_t fused = 0);
 
 // C[clab] = sum A[alab] B[blab] over the given value layouts (C dense), kernel
 // chosen and arguments filled; conj flags conjugate an operand on load.
@@ -198,7 +201,7 @@ struct VjpLowered {
   int root = -1;
 };
 
-VjpLowered lower_vjp(const Program &p, int64_t batch, int precision = QX_C128);
+VjpLowered lower_vjp(const Program &p, int64_t batch, int precision = QX_C128, uint64_t fused = 0);
 
 // First-fit offsets for [start, end) intervals of given sizes (elements), 256-byte
 // aligned for element size `esize`.  Returns the total size; offsets written out.
@@ -217,7 +220,7 @@ void launch_contract_gemm(int prec, const void *A, const void *B, void *C, void 
 void launch_contract_rows(int prec, const void *A, const void *B, void *C, const RowArgs &args,
                           void *stream);
 // few C elements, long K: one wave per C element, K split over lanes, shuffle reduce
-void launch_contract_reduce(int prec, const void *A, const void *B, void *C, int64_t c_elems,
+void launch_contract_reduce(int prec, const void *A, const void *B, void *C, void *ws, int64_t c_elems,
                             const BitArgs &args, void *stream);
 void launch_contract_radix(int prec, const void *A, const void *B, void *C, int64_t c_elems,
                            const RadixArgs &args, void *stream);

@@ -87,8 +87,11 @@ struct qx_plan {
   bool finalized = false;
 
   std::vector<DevBuf> data;                 // per key
-  std::map<int64_t, qx::Lowered> lowered;   // per batch size
-  std::map<int64_t, qx::VjpLowered> vjp_lowered;
+  std::map<std::pair<int64_t, uint64_t>, qx::Lowered> lowered;   // per (batch size, fused slice vars)
+  std::map<std::pair<int64_t, uint64_t>, qx::VjpLowered> vjp_lowered;
+  std::map<std::pair<int64_t, int>, uint64_t> fuse_choice;          // (batch, vjp) -> fused vars
+  int64_t opt_fuse = 1;
+  const qx::Lowered *last_lowered = nullptr;   // lowering used by the last qx_amplitudes
   DevBuf varena, cot, temp, grad, ampcot, ws;
   DevBuf arena, out_off, out_q, bits, acc, outbuf, relem_off;
   int64_t arena_cap_bytes = 0;
@@ -127,17 +130,18 @@ struct qx_plan {
 
 namespace {
 
-const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch) {
-  auto it = P->lowered.find(batch);
+const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch, uint64_t fused = 0) {
+  auto key = std::make_pair(batch, fused);
+  auto it = P->lowered.find(key);
   if (it != P->lowered.end()) return it->second;
-  qx::Lowered L = qx::lower(P->prog, batch, false, P->prec);
+  qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, fused);
   size_t need = (size_t)L.arena_elems * P->esize();
   if (P->arena_cap_bytes > 0 && (int64_t)need > P->arena_cap_bytes)
     throw qx::Error(QX_ERR_MEMORY, "arena of " + std::to_string(need >> 20) + " MiB for batch " +
                                        std::to_string(batch) + " exceeds the cap of " +
                                        std::to_string(P->arena_cap_bytes >> 20) + " MiB");
-  P->lowered.emplace(batch, std::move(L));
-  return P->lowered.at(batch);
+  P->lowered.emplace(key, std::move(L));
+  return P->lowered.at(key);
 }
 
 struct Bases {
@@ -175,7 +179,7 @@ void launch_step(qx_plan *P, const qx::Step &st, void *A, void *B, void *C) {
   switch (st.kern) {
     case qx::Kern::Radix: qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, st.radix, stream); break;
     case qx::Kern::Gemm: qx::launch_contract_gemm(P->prec, A, B, C, P->ws.p, st.gemm, stream); break;
-    case qx::Kern::Reduce: qx::launch_contract_reduce(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
+    case qx::Kern::Reduce: qx::launch_contract_reduce(P->prec, A, B, C, P->ws.p, st.c_elems, st.bits, stream); break;
     case qx::Kern::Rows: qx::launch_contract_rows(P->prec, A, B, C, st.rows, stream); break;
     default: qx::launch_contract_bits(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
   }
@@ -218,6 +222,62 @@ void collect_profile(qx_plan *P, const qx::Lowered &L) {
   P->event_step.clear();
 }
 
+// Slice variables to contract inline rather than loop over: the first j variables
+// for the largest j whose lowered program (arena + workspace, or for the vjp the
+// kept forward values + cotangents) fits half the memory cap.  Slicing bounds
+// memory; when memory does not bind, fewer and larger launches are faster and the
+// sum over the fused variables happens inside the contractions (by linearity).
+uint64_t choose_fused(qx_plan *P, int64_t batch, bool for_vjp) {
+  const int nv = (int)P->prog.vars.size();
+  if (!P->opt_fuse || nv == 0) return 0;
+  auto key = std::make_pair(batch, (int)for_vjp);
+  auto it = P->fuse_choice.find(key);
+  if (it != P->fuse_choice.end()) return it->second;
+  const int64_t budget = P->arena_cap_bytes / 2;
+  const int64_t es = (int64_t)P->esize();
+  uint64_t pick = 0;
+  for (int j = std::min(nv, 63); j > 0; --j) {
+    uint64_t mask = (uint64_t(1) << j) - 1;
+    try {
+      int64_t need;
+      if (for_vjp) {
+        qx::VjpLowered V = qx::lower_vjp(P->prog, batch, P->prec, mask);
+        need = (V.fwd.arena_elems + V.fwd.ws_elems + V.cot_elems + V.temp_elems) * es;
+      } else {
+        qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, mask);
+        need = (L.arena_elems + L.ws_elems) * es;
+      }
+      if (need <= budget) {
+        pick = mask;
+        break;
+      }
+    } catch (const qx::Error &e) {
+      if (e.code != QX_ERR_UNSUPPORTED) throw;
+    }
+  }
+  P->fuse_choice[key] = pick;
+  return pick;
+}
+
+// number of looped slices and the slice variables' values of looped slice `sl`
+int64_t looped_slices(const qx_plan *P, uint64_t fused) {
+  int64_t n = 1;
+  for (size_t j = 0; j < P->prog.vars.size(); ++j)
+    if (!((fused >> j) & 1)) n *= P->prog.var_dims[j];
+  return n;
+}
+
+void slice_values(const qx_plan *P, uint64_t fused, int64_t sl, std::vector<int64_t> &varval) {
+  for (size_t j = 0; j < P->prog.vars.size(); ++j) {
+    if ((fused >> j) & 1) {
+      varval[j] = 0;
+      continue;
+    }
+    varval[j] = sl % P->prog.var_dims[j];
+    sl /= P->prog.var_dims[j];
+  }
+}
+
 void check_bits(qx_plan *P, const char *bits, int64_t nb) {
   const int64_t nch = P->prog.num_chars;
   for (int64_t i = 0; i < nb * nch; ++i) {
@@ -238,7 +298,17 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   check_bits(P, bits, nb);
   int64_t batch = 1;
   while (batch < nb && batch < P->opt_batch) batch <<= 1;
-  const qx::Lowered &L = ensure_lowered(P, batch);
+  const uint64_t fused = (s0 == 0 && s1 == nslices) ? choose_fused(P, batch, false) : 0;
+  if (fused) {
+    s0 = 0;
+    s1 = looped_slices(P, fused);
+  }
+  const qx::Lowered &L = ensure_lowered(P, batch, fused);
+  if (P->last_lowered != &L) {
+    P->step_ms.clear();
+    P->step_calls.clear();
+  }
+  P->last_lowered = &L;
   hipStream_t stream = P->ctx->stream;
   const size_t es = P->esize();
   bool moved = P->arena.reserve(std::max<size_t>((size_t)L.arena_elems * es, 256));
@@ -293,11 +363,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     };
     run_section(0);
     for (int64_t sl = s0; sl < s1; ++sl) {
-      int64_t r = sl;
-      for (size_t j = 0; j < nv; ++j) {
-        varval[j] = r % P->prog.var_dims[j];
-        r /= P->prog.var_dims[j];
-      }
+      slice_values(P, fused, sl, varval);
       run_section(1);
       for (int64_t row0 = 0; row0 < nb; row0 += batch) {
         int64_t nvalid = std::min<int64_t>(batch, nb - row0);
@@ -311,7 +377,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     qx::launch_finalize((const double *)P->acc.p, (double *)P->outbuf.p, nb * R, stream);
   };
   if (P->opt_graph && !P->opt_profile) {
-    auto key = std::make_tuple(batch, nb, s0, s1);
+    auto key = std::make_tuple(batch, nb, s0, s1 + (fused ? (int64_t(1) << 40) : 0));
     auto it = P->graphs.find(key);
     if (it == P->graphs.end()) {
       hipGraph_t g = nullptr;
@@ -395,18 +461,19 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
   os << "]}";
 }
 
-const qx::VjpLowered &ensure_vjp(qx_plan *P, int64_t batch) {
-  auto it = P->vjp_lowered.find(batch);
+const qx::VjpLowered &ensure_vjp(qx_plan *P, int64_t batch, uint64_t fused) {
+  auto key = std::make_pair(batch, fused);
+  auto it = P->vjp_lowered.find(key);
   if (it != P->vjp_lowered.end()) return it->second;
-  qx::VjpLowered V = qx::lower_vjp(P->prog, batch, P->prec);
+  qx::VjpLowered V = qx::lower_vjp(P->prog, batch, P->prec, fused);
   const size_t es = P->esize();
   size_t need = (size_t)(V.fwd.arena_elems + V.cot_elems + V.temp_elems) * es;
   if (P->arena_cap_bytes > 0 && (int64_t)need > P->arena_cap_bytes)
     throw qx::Error(QX_ERR_MEMORY, "vjp needs " + std::to_string(need >> 20) + " MiB of device memory for batch " +
                                        std::to_string(batch) + ", over the cap of " +
                                        std::to_string(P->arena_cap_bytes >> 20) + " MiB");
-  P->vjp_lowered.emplace(batch, std::move(V));
-  return P->vjp_lowered.at(batch);
+  P->vjp_lowered.emplace(key, std::move(V));
+  return P->vjp_lowered.at(key);
 }
 
 void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double *grad) {
@@ -416,7 +483,8 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
   check_bits(P, bits, nb);
   int64_t batch = 1;
   while (batch < nb && batch < P->opt_batch) batch <<= 1;
-  const qx::VjpLowered &V = ensure_vjp(P, batch);
+  const uint64_t fused = choose_fused(P, batch, true);
+  const qx::VjpLowered &V = ensure_vjp(P, batch, fused);
   const qx::Lowered &L = V.fwd;
   hipStream_t stream = P->ctx->stream;
   const int64_t es = (int64_t)P->esize();
@@ -509,16 +577,11 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
       }
     }
   };
-  int64_t nslices = 1;
-  for (int64_t d : P->prog.var_dims) nslices *= d;
+  const int64_t nslices = looped_slices(P, fused);
   run_fwd(0);
   zero_cots(0);
   for (int64_t sl = 0; sl < nslices; ++sl) {
-    int64_t r = sl;
-    for (size_t j = 0; j < nv; ++j) {
-      varval[j] = r % P->prog.var_dims[j];
-      r /= P->prog.var_dims[j];
-    }
+    slice_values(P, fused, sl, varval);
     run_fwd(1);
     zero_cots(1);
     for (int64_t row0 = 0; row0 < nb; row0 += batch) {
@@ -547,8 +610,9 @@ std::string stats_json(const qx_plan *P) {
     const qx::Lowered &L = kv.second;
     if (!first) os << ",";
     first = false;
-    os << "{\"batch\":" << L.batch << ",\"arena_bytes\":" << L.arena_elems * (int64_t)P->esize()
-       << ",\"result_elems\":" << L.result_elems << ",\"steps\":[";
+    os << "{\"batch\":" << L.batch << ",\"fused\":" << L.fused << ",\"active\":" << (&L == P->last_lowered)
+       << ",\"arena_bytes\":" << L.arena_elems * (int64_t)P->esize() << ",\"result_elems\":" << L.result_elems
+       << ",\"steps\":[";
     for (size_t s = 0; s < L.steps.size(); ++s) {
       const qx::Step &st = L.steps[s];
       if (s) os << ",";
@@ -558,7 +622,7 @@ std::string stats_json(const qx_plan *P) {
          << ",\"B\":" << st.H << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K
          << ",\"flops\":" << flops << ",\"bytes\":" << bytes << ",\"kernel\":\""
          << kern_name(st.kern) << "\",\"variant\":\"" << kern_variant(st) << "\"";
-      if (P->opt_profile && L.batch == P->lowered.rbegin()->first && s < P->step_ms.size())
+      if (P->opt_profile && &L == P->last_lowered && s < P->step_ms.size())
         os << ",\"ms\":" << P->step_ms[s] << ",\"calls\":" << P->step_calls[s];
       os << "}";
     }
@@ -629,7 +693,7 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
   return guard([&] {
     if (!plan || !name) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
     std::string n(name);
-    bool runtime_opt = (n == "profile" || n == "graph");
+    bool runtime_opt = (n == "profile" || n == "graph" || n == "fuse_slices");
     if (plan->finalized && !runtime_opt) throw qx::Error(QX_ERR_STATE, "option '" + n + "' must be set before finalize");
     if (n == "batch") {
       if (value <= 0 || (value & (value - 1)) || value > (1 << 20))
@@ -640,6 +704,8 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
       plan->arena_cap_bytes = value > 0 ? (value << 20) : 0;
     } else if (n == "graph") {
       plan->opt_graph = value;
+    } else if (n == "fuse_slices") {
+      plan->opt_fuse = value;
     } else if (n == "profile") {
       // may be toggled at any time; every set clears the accumulated step times
       plan->opt_profile = value;

@@ -163,7 +163,8 @@ def build_compute_graph(tnc: TensorNetworkCircuit, plan: list, bond_groups: list
     node: "OrderedDict[str, Command]" = OrderedDict()
     for n, t in tn.tensors.items():                                   # :25-29
         node[n] = Command("load", n, (tc.push(t.data), tuple(t.data.shape)))
-    for q, o in enumerate(tnc.output_tensors, start=1):                  # :32-35
+    oq = getattr(tnc, "output_qubits", None) or list(range(1, len(tnc.output_tensors) + 1))
+    for q, o in zip(oq, tnc.output_tensors):                               # :32-35
         node[o] = Command("output", o, (q, int(tn[o].data.shape[0])))
     views = []
     renamed: dict = {}

@@ -403,19 +403,30 @@ class PlanCost:
 
 
 def contraction_plan(tn, trials: int = 32, seed: int = 42, max_log2_size: float | None = None,
-                     open_indices=()) -> list:
-    """Best-of-`trials` randomised greedy plan by total flops (rank cap optional)."""
+                     open_indices=(), min_fill_trials: int = 0) -> list:
+    """Best plan by total flops (rank cap optional) over `trials` randomised greedy
+    passes and `min_fill_trials` noisy min-fill line-graph elimination orders."""
     open_indices = _open_of(tn, open_indices)
     tn = _as_tn(tn)
     rng = np.random.default_rng(seed)
     best, best_key = None, None
+    cands = []
     for t in range(trials):
         if t == 0:
             alpha, temp = 1.0, 0.0
         else:
-            alpha, temp = float(rng.uniform(0.0, 1.5)), float(rng.uniform(0.0, 1.0))
-        plan = greedy_contraction_plan(tn, seed=int(rng.integers(1 << 31)), alpha=alpha,
-                                       temperature=temp, open_indices=open_indices)
+            alpha = float(np.exp(rng.uniform(np.log(0.1), np.log(4.0))))
+            temp = float(np.exp(rng.uniform(np.log(0.01), np.log(1.0))))
+        cands.append(("g", alpha, temp, int(rng.integers(1 << 31))))
+    for t in range(min_fill_trials):
+        cands.append(("m", float(rng.uniform(0.0, 0.3)), 0.0, int(rng.integers(1 << 31))))
+    adj = line_graph(tn) if min_fill_trials else None
+    for kind, a1, a2, sd in cands:
+        if kind == "g":
+            plan = greedy_contraction_plan(tn, seed=sd, alpha=a1, temperature=a2, open_indices=open_indices)
+        else:
+            order = elimination_order(adj, seed=sd, heuristic="min_fill", noise=a1)
+            plan = order_to_contraction_plan(order, tn, open_indices)
         flops, mx = PlanCost(tn, plan, open_indices).cost()
         over = 0.0 if max_log2_size is None else max(0.0, mx - max_log2_size)
         key = (over, flops, mx)
@@ -460,13 +471,13 @@ def expand_slice_bonds_to_hyperindices(tn, bonds: list) -> list:
 
 
 def contraction_scheme(tn, num: int, trials: int = 32, seed: int = 42,
-                       max_log2_size: float | None = None) -> tuple:
+                       max_log2_size: float | None = None, min_fill_trials: int = 0) -> tuple:
     """(bond_groups, plan, metadata) as `contraction_scheme` returns
     (`src/contraction_planning.jl:219-299`)."""
     open_indices = _open_of(tn, ())
     tn = _as_tn(tn)
     plan = contraction_plan(tn, trials=trials, seed=seed, max_log2_size=max_log2_size,
-                            open_indices=open_indices)
+                            open_indices=open_indices, min_fill_trials=min_fill_trials)
     pc = PlanCost(tn, plan, open_indices)
     flops0, mx0 = pc.cost()
     sliced = greedy_slicing(tn, plan, num, open_indices) if num > 0 else []

@@ -140,14 +140,14 @@ def output_params_dict(num_qubits: int, num_outputs: int = 10, output_method: st
 def generate_simulation_files(circ: Circuit, output_prefix: str = "simulation_input",
                               number_bonds_to_slice: int = 2, decompose: bool = False,
                               seed: int | None = None, output_args=None, planner: str = "auto",
-                              **kwargs) -> tuple:
+                              open_outputs=(), **kwargs) -> tuple:
     """Write `<prefix>.qx`, `<prefix>.npz`, `<prefix>.yml` (`src/simulation.jl:52-77`).
 
     planner: "sweep" (circuit-aware fused sweep, no slicing), "greedy" (randomised
     greedy + greedy slicing), or "auto" (sweep when nothing is sliced)."""
     import yaml
 
-    tnc = convert_to_tnc(circ, decompose=decompose)
+    tnc = convert_to_tnc(circ, decompose=decompose, open_outputs=open_outputs)
     if planner == "auto":
         planner = "sweep" if number_bonds_to_slice == 0 else "greedy"
     if planner == "sweep":

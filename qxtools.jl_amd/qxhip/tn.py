@@ -140,6 +140,7 @@ class TensorNetworkCircuit:
         self.output_indices = list(self.input_indices)
         self.input_tensors: list = []
         self.output_tensors: list = []
+        self.output_qubits: list = []  # qubit of each output tensor
         self.gate_tensors: list = []   # per pushed gate: (tensor names, qubits), time order
 
     def qubits(self) -> int:
@@ -151,8 +152,8 @@ class TensorNetworkCircuit:
         out = []
         if not self.input_tensors:
             out += self.input_indices
-        if not self.output_tensors:
-            out += [i for i in self.output_indices if i not in out]
+        capped = set(self.output_qubits)
+        out += [i for q, i in enumerate(self.output_indices, start=1) if q not in capped and i not in out]
         return out
 
     def __len__(self) -> int:
@@ -261,32 +262,39 @@ class TensorNetworkCircuit:
         for q, ch in enumerate(bits):
             self.input_tensors.append(self.tn.push([self.input_indices[q]], basis_vector(ch)))
 
-    def add_output(self, bits: str | None = None) -> None:
+    def add_output(self, bits: str | None = None, open_qubits=()) -> None:
         """Rank-1 output tensors <b_q| on each qubit's last wire index (QXTns
         `add_output!`, `src/tn_conversion.jl:38`, `src/simulation.jl:88`).  When the
-        outputs exist already their values are replaced."""
+        outputs exist already their values are replaced.  Qubits in `open_qubits` get
+        no output tensor: their last wire stays an open leg of the network."""
         bits = bits or "0" * self.num_qubits
         if len(bits) != self.num_qubits:
             raise ValueError("output bitstring length differs from qubit count")
         if self.output_tensors:
-            for name, ch in zip(self.output_tensors, bits):
-                self.tn[name].data = basis_vector(ch)
+            for name, q in zip(self.output_tensors, self.output_qubits):
+                self.tn[name].data = basis_vector(bits[q - 1])
             return
-        for q, ch in enumerate(bits):
-            self.output_tensors.append(self.tn.push([self.output_indices[q]], basis_vector(ch)))
+        skip = set(open_qubits)
+        for q, ch in enumerate(bits, start=1):
+            if q in skip:
+                continue
+            self.output_tensors.append(self.tn.push([self.output_indices[q - 1]], basis_vector(ch)))
+            self.output_qubits.append(q)
 
 
 def convert_to_tnc(circ: Circuit, input: str | None = None, output: str | None = None,
                    no_input: bool = False, no_output: bool = False,
-                   decompose: bool = False) -> TensorNetworkCircuit:
-    """Circuit -> TensorNetworkCircuit (`src/tn_conversion.jl:30-40`)."""
+                   decompose: bool = False, open_outputs=()) -> TensorNetworkCircuit:
+    """Circuit -> TensorNetworkCircuit (`src/tn_conversion.jl:30-40`).  `open_outputs`
+    leaves those qubits' output legs open (one contraction then yields 2^m
+    amplitudes)."""
     tnc = TensorNetworkCircuit(circ.num_qubits)
     for g in circ.ops:
         tnc.push(gate_qubits(g), gate_matrix(g), decompose=decompose)
     if not no_input:
         tnc.add_input(input)
     if not no_output:
-        tnc.add_output(output)
+        tnc.add_output(output, open_qubits=open_outputs)
     return tnc
 
 

@@ -35,6 +35,8 @@ class Workload:
     sliced: int = 0
     parameterised: bool = False
     plan_kwargs: dict = field(default_factory=dict)
+    open_outputs: tuple = ()          # qubits left open: 2^m amplitudes per bitstring
+    note: str = ""
 
     @property
     def qubits(self) -> int:
@@ -56,7 +58,8 @@ class Workload:
         out = simulation.output_params_dict(self.qubits, self.num_bitstrings, "List",
                                             bitstrings=self.bitstrings())
         return simulation.generate_simulation_files(self.circuit(), self.prefix(), self.sliced,
-                                                    output_args=out, **self.plan_kwargs)
+                                                    output_args=out, open_outputs=self.open_outputs,
+                                                    **self.plan_kwargs)
 
     def load(self) -> tuple:
         """(dsl, data, bitstrings) of the saved plan."""
@@ -70,7 +73,25 @@ WORKLOADS = {w.name: w for w in [
     Workload("rqc_3x3_d16", 3, 3, 16, 64, "c128"),
     # plain method at scale: 5x5 grid, depth 20, 256 bitstrings, complex128, no slicing
     Workload("rqc_5x5_d20", 5, 5, 20, 256, "c128"),
+    # precision: 6x6 grid, complex64 storage, 2^6 slices, 512 bitstrings.  Depth 12
+    # instead of 24: at depth 24 the planners here reach width 39 (2^58 flops per
+    # amplitude), beyond a single-GPU benchmark.
+    Workload("rqc_6x6_d12_s6", 6, 6, 12, 512, "c64_4m", sliced=6,
+             plan_kwargs={"trials": 16, "min_fill_trials": 4},
+             note="depth reduced from 24 to 12"),
+    # batching: 7x7 grid, 2^10 slices, 5 output legs left open so each of 32 bitstring
+    # prefixes yields 32 amplitudes (1024 in all).  Depth 12 instead of 24 (width).
+    Workload("rqc_7x7_d12_s10_open5", 7, 7, 12, 32, "c64_4m", sliced=10,
+             plan_kwargs={"trials": 16, "min_fill_trials": 4}, open_outputs=(45, 46, 47, 48, 49),
+             note="depth reduced from 24 to 12"),
 ]}
+
+# differentiation: parameterised 6x6 grid, Rz Ry Rz per qubit per cycle, 2^4 slices,
+# 64 bitstrings.  Depth 6 instead of 20 (648 angles); built at run time by
+# qxhip.gradients.ParamCircuit, which needs one data key per gate.
+PARAM_WORKLOAD = Workload("param_6x6_d6_s4", 6, 6, 6, 64, "c128", sliced=4, parameterised=True,
+                          plan_kwargs={"trials": 16, "min_fill_trials": 4},
+                          note="depth reduced from 20 to 6")
 
 
 def main(argv):

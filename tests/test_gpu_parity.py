@@ -139,3 +139,27 @@ def test_invalid_inputs_raise():
         ex.amplitudes(["0x1"])
     with pytest.raises(QxError):
         ex.amplitudes(["000"], 0, 2)
+
+
+def test_fused_slices_equal_looped_slices():
+    """Contracting slice variables inline (memory permitting) gives the same sum as
+    looping over the slices; a partial range always loops."""
+    circ = circuits.create_rqc_circuit(3, 3, 10, seed=13)
+    t, plan, g = qxtest.graph_for(circ, planner="greedy", nslice=4)
+    dsl = g.to_dsl()
+    bits = simulation.amplitudes_uniform(9, 8, 12)
+    fused = Executor(dsl, g.data, batch=8)
+    looped = Executor(dsl, g.data, batch=8)
+    looped.plan.set_option("fuse_slices", 0)
+    a = fused.amplitudes(bits)
+    b = looped.amplitudes(bits)
+    ref = O.amplitudes(dsl, g.data, bits)
+    assert np.max(qxtest.rel_err(a, ref)) <= TOL64
+    assert np.max(qxtest.rel_err(b, ref)) <= TOL64
+    act = [x for x in fused.stats()["batches"] if x["active"]][0]
+    assert act["fused"] != 0
+    part = fused.amplitudes(bits, 2, 9)
+    act = [x for x in fused.stats()["batches"] if x["active"]][0]
+    assert act["fused"] == 0
+    per = sum(O.amplitudes(dsl, g.data, bits, slices=[s]) for s in list(O.slice_assignments(O.parse(dsl)))[2:9])
+    assert np.max(qxtest.rel_err(part, per)) <= 1e-11
