@@ -164,6 +164,8 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=1, help="replay each call from a captured hipGraph")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64 comparison")
     ap.add_argument("--grad", action="store_true", help="gradient / forward time of the parameterised workload")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the steps, copy this many MiB device to device (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the contraction path on a single "
@@ -240,6 +242,14 @@ def main() -> int:
         "ms_per_step_eager_with_events": 1e3 * el_ev / a.steps,
         "kernels": kernels,
     }
+    if a.copy_mib:
+        # a plain copy of known size: HBM counters of this kernel calibrate the
+        # counters of the contraction kernels (bytes read = bytes written = size)
+        src = torch.ones(a.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = a.copy_mib << 20
     if not a.no_c64 and prec == "c128":
         out["c64"] = c64_comparison(dsl, data, bits, np.asarray(amps), a)
     elif not a.no_c64:

@@ -126,10 +126,12 @@ void qx_plan_destroy(qx_plan *plan);
 
 /* Parse and lower a .qx program without a device and write the same JSON as
  * qx_plan_stats (steps, kernel choice, arena size) plus each arena value's offset
- * and live interval.  For planning tools and host-side checks of the reuse
- * schedule; no reference counterpart. */
-qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t batch, char *json,
-                         size_t cap);
+ * and live interval.  `fused` selects slice variables contracted inline (bit j =
+ * j-th variable, as qx_amplitudes does over a full slice range when memory
+ * allows).  For planning tools and host-side checks of the reuse schedule; no
+ * reference counterpart. */
+qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t batch, uint64_t fused,
+                         char *json, size_t cap);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *qx_last_error(void);

@@ -57,8 +57,10 @@ __global__ __launch_bounds__(kThreads) void k_contract_bits(const typename Vec<T
     tkb[k] = ob;
   }
   __syncthreads();
-  const int64_t nk_hi = int64_t(1) << (a.nbk - nk_lo);
+  const int64_t nk_hi_all = int64_t(1) << (a.nbk - nk_lo);
+  const int64_t nk_hi = nk_hi_all / a.ksplit;   // high K iterations of one split
   const int64_t tile = int64_t(1) << kTileBits;
+  const int64_t ctiles = (c_elems + tile - 1) / tile;
   // per-lane low offsets for the kElems elements of this lane
   int64_t loa[kElems], lob[kElems];
 #pragma unroll
@@ -73,7 +75,10 @@ __global__ __launch_bounds__(kThreads) void k_contract_bits(const typename Vec<T
     loa[e] = oa;
     lob[e] = ob;
   }
-  for (int64_t base = (int64_t)blockIdx.x * tile; base < c_elems; base += (int64_t)gridDim.x * tile) {
+  // work unit = (C tile, K split); split s writes its partial at C + s * c_elems
+  for (int64_t unit = blockIdx.x; unit < ctiles * a.ksplit; unit += gridDim.x) {
+    const int64_t base = (unit % ctiles) * tile;
+    const int64_t split = unit / ctiles;
     int64_t hia = 0, hib = 0;
     for (int b = kTileBits; b < a.nbc; ++b)
       if ((base >> b) & 1) {
@@ -83,7 +88,7 @@ __global__ __launch_bounds__(kThreads) void k_contract_bits(const typename Vec<T
     V acc[kElems];
 #pragma unroll
     for (int e = 0; e < kElems; ++e) acc[e] = V{0, 0};
-    for (int64_t kh = 0; kh < nk_hi; ++kh) {
+    for (int64_t kh = split * nk_hi; kh < (split + 1) * nk_hi; ++kh) {
       int64_t oka = 0, okb = 0;
       for (int b = nk_lo; b < a.nbk; ++b)
         if ((kh >> (b - nk_lo)) & 1) {
@@ -102,7 +107,7 @@ __global__ __launch_bounds__(kThreads) void k_contract_bits(const typename Vec<T
 #pragma unroll
     for (int e = 0; e < kElems; ++e) {
       int64_t idx = base + e * kThreads + threadIdx.x;
-      if (idx < c_elems) C[idx] = acc[e];
+      if (idx < c_elems) C[split * c_elems + idx] = acc[e];
     }
   }
 }
@@ -941,17 +946,29 @@ void launch_contract_reduce(int prec, const void *A, const void *B, void *C, voi
   }
 }
 
-void launch_contract_bits(int prec, const void *A, const void *B, void *C, int64_t c_elems, const BitArgs &args,
-                          void *stream) {
-  int g = grid_for(c_elems, kThreads * kElems);
+void launch_contract_bits(int prec, const void *A, const void *B, void *C, void *ws, int64_t c_elems,
+                          const BitArgs &args, void *stream) {
+  int g = grid_for(c_elems * args.ksplit, kThreads * kElems);
+  if (args.ksplit > 1) g = grid_for(((c_elems + kThreads * kElems - 1) / (kThreads * kElems)) * args.ksplit, 1);
   hipStream_t s = (hipStream_t)stream;
+  void *dst = args.ksplit > 1 ? ws : C;
   if (prec == QX_C128)
     hipLaunchKernelGGL(k_contract_bits<double>, dim3(g), dim3(kThreads), 0, s, (const double2 *)A,
-                       (const double2 *)B, (double2 *)C, c_elems, args);
+                       (const double2 *)B, (double2 *)dst, c_elems, args);
   else
     hipLaunchKernelGGL(k_contract_bits<float>, dim3(g), dim3(kThreads), 0, s, (const float2 *)A, (const float2 *)B,
-                       (float2 *)C, c_elems, args);
+                       (float2 *)dst, c_elems, args);
   check_launch("contract_bits");
+  if (args.ksplit > 1) {
+    int gs = grid_for(c_elems, 256);
+    if (prec == QX_C128)
+      hipLaunchKernelGGL(k_sum_splits<double>, dim3(gs), dim3(256), 0, s, (double2 *)C, (const double2 *)ws, c_elems,
+                         args.ksplit);
+    else
+      hipLaunchKernelGGL(k_sum_splits<float>, dim3(gs), dim3(256), 0, s, (float2 *)C, (const float2 *)ws, c_elems,
+                         args.ksplit);
+    check_launch("sum_splits");
+  }
 }
 
 void launch_contract_radix(int prec, const void *A, const void *B, void *C, int64_t c_elems,

@@ -87,12 +87,180 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
   }
 }
 
-// Choose the kernel of a step and fill its launch arguments.
+// Kernel configurations a power-of-two step can run with.
+enum Choice { kBits = 0, kRows = 1, kGemm0 = 2, kGemm1 = 3, kGemm2 = 4, kMfma = 5, kReduce = 6, kNumChoices = 7 };
+
+namespace {
+struct CB {
+  int64_t wa, wb, wc;
+};
+
+struct StepBits {
+  std::vector<CB> cbits, hb, mb, nbits;
+  std::vector<std::pair<int64_t, int64_t>> kbits;
+  int nm = 0, nn = 0, nk = 0, nr = 0, hlow = 0;
+};
+
+StepBits step_bits(const Step &st) {
+  StepBits sb;
+  for (auto &g : st.cl)
+    for (int i = 0; i < ilog2(g.dim); ++i) sb.cbits.push_back({g.sa << i, g.sb << i, g.sc << i});
+  for (auto &g : st.kl)
+    for (int i = 0; i < ilog2(g.dim); ++i) sb.kbits.push_back({g.sa << i, g.sb << i});
+  for (auto &c : sb.cbits) {
+    if (c.wa && c.wb) sb.hb.push_back(c);
+    else if (c.wa) sb.mb.push_back(c);
+    else sb.nbits.push_back(c);
+  }
+  sb.nm = (int)sb.mb.size();
+  sb.nn = (int)sb.nbits.size();
+  sb.nk = (int)sb.kbits.size();
+  sb.nr = (int)(sb.hb.size() + sb.mb.size());
+  int j = 0;
+  for (auto &c : sb.cbits)
+    if (c.wa && j < 8) sb.hlow += (c.wb != 0), ++j;
+  return sb;
+}
+
+bool rows_ok(const StepBits &b) {
+  return b.nk <= 5 && b.nn <= 6 && (int64_t(1) << (b.nk + b.nn + b.hlow)) <= 4096;
+}
+}  // namespace
+
+// Configure `st` for kernel configuration `choice`; false if that kernel cannot
+// run this step.  Every configuration computes the same C.
+bool configure_kernel(Step &st, int precision, int choice) {
+  const StepBits sb = step_bits(st);
+  const int nm = sb.nm, nn = sb.nn, nk = sb.nk;
+  BitArgs &b = st.bits;
+  b = BitArgs{};
+  b.nbc = (int)sb.cbits.size();
+  b.nbk = nk;
+  for (int i = 0; i < b.nbc; ++i) b.wa[i] = sb.cbits[i].wa, b.wb[i] = sb.cbits[i].wb;
+  for (int i = 0; i < nk; ++i) b.ka[i] = sb.kbits[i].first, b.kb[i] = sb.kbits[i].second;
+  st.ws_elems = 0;
+  switch (choice) {
+    case kBits: {
+      st.kern = Kern::Bits;
+      // small C, long K: split K over workgroups (lanes walk C)
+      if (st.c_elems <= (int64_t(1) << 14) && nk >= 8) {
+        const int64_t tiles = std::max<int64_t>(1, st.c_elems >> 10);
+        const int nk_hi = nk > 9 ? nk - 9 : 0;
+        while (tiles * b.ksplit < 1024 && b.ksplit < (int64_t(1) << nk_hi)) b.ksplit *= 2;
+      }
+      break;
+    }
+    case kReduce: {
+      if (nk < 6) return false;
+      st.kern = Kern::Reduce;
+      while (st.c_elems * b.ksplit < 8192 && (int64_t(1) << nk) / b.ksplit >= 128) b.ksplit *= 2;
+      break;
+    }
+    case kRows: {
+      if (!rows_ok(sb)) return false;
+      st.kern = Kern::Rows;
+      RowArgs &r = st.rows;
+      r = RowArgs{};
+      std::vector<CB> rb;
+      for (auto &c : sb.cbits)
+        if (c.wa) rb.push_back(c);  // H and M bits, already in C order
+      r.nbr = (int)rb.size();
+      for (int i = 0; i < std::min(8, r.nbr); ++i)
+        if (rb[i].wb) r.hlo |= 1 << i;
+      for (int i = 0; i < r.nbr; ++i) r.ra[i] = rb[i].wa, r.rb[i] = rb[i].wb, r.rc[i] = rb[i].wc;
+      r.nk = 1 << nk;
+      r.nn = 1 << nn;
+      for (int k = 0; k < r.nk; ++k) {
+        int64_t oa = 0, ob = 0;
+        for (int q = 0; q < nk; ++q)
+          if ((k >> q) & 1) oa += sb.kbits[q].first, ob += sb.kbits[q].second;
+        r.ka[k] = oa;
+        r.kb[k] = ob;
+      }
+      for (int n = 0; n < r.nn; ++n) {
+        int64_t ob = 0, oc = 0;
+        for (int q = 0; q < nn; ++q)
+          if ((n >> q) & 1) ob += sb.nbits[q].wb, oc += sb.nbits[q].wc;
+        r.nb[n] = ob;
+        r.nc[n] = oc;
+      }
+      break;
+    }
+    case kGemm0:
+    case kGemm1:
+    case kGemm2:
+    case kMfma: {
+      if (choice == kMfma && precision == QX_C128) return false;
+      st.kern = Kern::Gemm;
+      GemmArgs &g = st.gemm;
+      g = GemmArgs{};
+      std::vector<CB> mb = sb.mb, nbits = sb.nbits;
+      std::vector<std::pair<int64_t, int64_t>> kbits = sb.kbits;
+      auto by_c = [](const CB &x, const CB &y) { return x.wc < y.wc; };
+      std::sort(mb.begin(), mb.end(), by_c);
+      std::sort(nbits.begin(), nbits.end(), by_c);
+      std::sort(kbits.begin(), kbits.end());
+      g.nbh = (int)sb.hb.size();
+      g.nbm = nm;
+      g.nbn = nn;
+      g.nbk = nk;
+      for (int i = 0; i < g.nbh; ++i) g.ha[i] = sb.hb[i].wa, g.hb[i] = sb.hb[i].wb, g.hc[i] = sb.hb[i].wc;
+      for (int i = 0; i < nm; ++i) g.ma[i] = mb[i].wa, g.mc[i] = mb[i].wc;
+      for (int i = 0; i < nn; ++i) g.nb[i] = nbits[i].wb, g.nc[i] = nbits[i].wc;
+      for (int i = 0; i < nk; ++i) g.ka[i] = kbits[i].first, g.kb[i] = kbits[i].second;
+      g.tile = choice - kGemm0;
+      const int lm = g.tile >= 2 ? 6 : 7, ln = g.tile == 0 ? 4 : g.tile == 1 ? 5 : 6, lk = 4;
+      auto min_w = [](const int64_t *w, int n) {
+        int64_t m = INT64_MAX;
+        for (int i = 0; i < n; ++i) m = std::min(m, w[i]);
+        return m;
+      };
+      int64_t kb_min_b = INT64_MAX, ka_min = min_w(g.ka, std::min(nk, lk));
+      for (int i = 0; i < std::min(nk, lk); ++i) kb_min_b = std::min(kb_min_b, g.kb[i]);
+      g.a_kfast = ka_min < min_w(g.ma, std::min(nm, lm));
+      g.b_kfast = kb_min_b < min_w(g.nb, std::min(nn, ln));
+      g.c_nfast = min_w(g.nc, std::min(nn, ln)) < min_w(g.mc, std::min(nm, lm));
+      // split K over workgroups when the tiles alone cannot fill the chip
+      const int64_t ntiles = (int64_t(1) << g.nbh) * (nm > lm ? int64_t(1) << (nm - lm) : 1) *
+                             (nn > ln ? int64_t(1) << (nn - ln) : 1);
+      const int64_t chunks = nk > lk ? int64_t(1) << (nk - lk) : 1;
+      g.c_elems = st.c_elems;
+      while (ntiles * g.ksplit < 1024 && chunks / g.ksplit >= 8) g.ksplit *= 2;
+      st.ws_elems = g.ksplit > 1 ? g.ksplit * st.c_elems : 0;
+      return true;
+    }
+    default: return false;
+  }
+  st.ws_elems = b.ksplit > 1 ? b.ksplit * st.c_elems : 0;
+  return true;
+}
+
+// The heuristic choice:
 //  * Rows   : >= 2^12 rows (H and M), K <= 16, N <= 64, C's fastest bit a row bit
-//  * Gemm   : K >= 8, N >= 8, M >= 32 (bits of each class counted without H)
-//  * Reduce : few C elements (<= 2^14) and K >= 256
+//  * Gemm   : K >= 8, N >= 8, M >= 32 (bits of each class counted without H); complex64
+//             with N, M >= 64 on FP32 matrix cores
+//  * small C (<= 2^14) with K >= 256: lanes along K (Reduce) when a K bit is A's
+//    unit stride, else the gather kernel with K split over workgroups
 //  * Bits   : everything else with power-of-two dims
 //  * Radix  : any other dims
+int default_choice(const Step &st, int precision) {
+  const StepBits sb = step_bits(st);
+  if (sb.nr >= 12 && sb.nk <= 4 && sb.nk + sb.nn >= 1 && !sb.cbits.empty() && sb.cbits[0].wa != 0 && rows_ok(sb))
+    return kRows;
+  if (sb.nk >= 3 && sb.nn >= 3 && sb.nm >= 5) {
+    if (precision != QX_C128 && sb.nn >= 6 && sb.nm >= 6) return kMfma;
+    return sb.nn <= 4 ? kGemm0 : sb.nn == 5 ? kGemm1 : kGemm2;
+  }
+  if (st.c_elems <= (int64_t(1) << 14) && sb.nk >= 8) {
+    int64_t kmin = INT64_MAX, cmin = INT64_MAX;
+    for (auto &kb : sb.kbits) kmin = std::min(kmin, kb.first);
+    for (auto &c : sb.cbits)
+      if (c.wa) cmin = std::min(cmin, c.wa);
+    return kmin < cmin ? kReduce : kBits;
+  }
+  return kBits;
+}
+
 void set_kernel_args(Step &st, int precision) {
   if (st.kern == Kern::Radix) {
     RadixArgs &a = st.radix;
@@ -102,110 +270,31 @@ void set_kernel_args(Step &st, int precision) {
     for (int j = 0; j < a.nkl; ++j) a.kd[j] = st.kl[j].dim, a.ka[j] = st.kl[j].sa, a.kb[j] = st.kl[j].sb;
     return;
   }
-  // expand legs into bits
-  struct CB { int64_t wa, wb, wc; };
-  std::vector<CB> cbits;
-  std::vector<std::pair<int64_t, int64_t>> kbits;
-  for (auto &g : st.cl)
-    for (int i = 0; i < ilog2(g.dim); ++i) cbits.push_back({g.sa << i, g.sb << i, g.sc << i});
-  for (auto &g : st.kl)
-    for (int i = 0; i < ilog2(g.dim); ++i) kbits.push_back({g.sa << i, g.sb << i});
-  BitArgs &b = st.bits;
-  b.nbc = (int)cbits.size();
-  b.nbk = (int)kbits.size();
-  for (int i = 0; i < b.nbc; ++i) b.wa[i] = cbits[i].wa, b.wb[i] = cbits[i].wb;
-  for (int i = 0; i < b.nbk; ++i) b.ka[i] = kbits[i].first, b.kb[i] = kbits[i].second;
+  if (!configure_kernel(st, precision, default_choice(st, precision)))
+    throw Error(QX_ERR_STATE, "internal: default kernel cannot run its step");
+}
 
-  std::vector<CB> hb, mb, nbits;
-  for (auto &c : cbits) {
-    if (c.wa && c.wb) hb.push_back(c);
-    else if (c.wa) mb.push_back(c);
-    else nbits.push_back(c);
-  }
-  const int nm = (int)mb.size(), nn = (int)nbits.size(), nk = (int)kbits.size();
-  // rows: many rows, K and N of at most 8 values each, C's fastest bit a row bit
-  const int nr = (int)(hb.size() + mb.size());
-  int hlow = 0;  // H bits among the 8 lowest row bits (they multiply the staged B)
-  {
-    int j = 0;
-    for (auto &c : cbits)
-      if (c.wa && j < 8) hlow += (c.wb != 0), ++j;
-  }
-  if (nr >= 12 && nk <= 4 && nn <= 6 && (nk + nn) >= 1 && !cbits.empty() && cbits[0].wa != 0 &&
-      ((int64_t(1) << (nk + nn + hlow)) <= 4096)) {
-    st.kern = Kern::Rows;
-    RowArgs &r = st.rows;
-    std::vector<CB> rb;
-    for (auto &c : cbits)
-      if (c.wa) rb.push_back(c);  // H and M bits, already in C order
-    r.nbr = (int)rb.size();
-    for (int i = 0; i < std::min(8, r.nbr); ++i)
-      if (rb[i].wb) r.hlo |= 1 << i;
-    for (int i = 0; i < r.nbr; ++i) r.ra[i] = rb[i].wa, r.rb[i] = rb[i].wb, r.rc[i] = rb[i].wc;
-    r.nk = 1 << nk;
-    r.nn = 1 << nn;
-    for (int k = 0; k < r.nk; ++k) {
-      int64_t oa = 0, ob = 0;
-      for (int b = 0; b < nk; ++b)
-        if ((k >> b) & 1) oa += kbits[b].first, ob += kbits[b].second;
-      r.ka[k] = oa;
-      r.kb[k] = ob;
+std::vector<Step> kernel_candidates(const Step &st, int precision) {
+  std::vector<Step> out;
+  out.push_back(st);
+  if (st.kern == Kern::Radix) return out;
+  const StepBits sb = step_bits(st);
+  std::vector<int> alts;
+  if (sb.nk >= 2 && sb.nn >= 2 && sb.nm >= 4) alts.insert(alts.end(), {kGemm0, kGemm1, kGemm2, kMfma});
+  if (sb.nr >= 8 && rows_ok(sb)) alts.push_back(kRows);
+  if (st.c_elems <= (int64_t(1) << 16) && sb.nk >= 6) alts.push_back(kReduce);
+  if (sb.nk <= 6 || st.c_elems <= (int64_t(1) << 16)) alts.push_back(kBits);
+  const int def = default_choice(st, precision);
+  for (int c : alts) {
+    if (c == def) continue;
+    Step v = st;
+    if (configure_kernel(v, precision, c)) {
+      v.bits.conj_a = v.gemm.conj_a = v.rows.conj_a = st.bits.conj_a;
+      v.bits.conj_b = v.gemm.conj_b = v.rows.conj_b = st.bits.conj_b;
+      out.push_back(std::move(v));
     }
-    for (int n = 0; n < r.nn; ++n) {
-      int64_t ob = 0, oc = 0;
-      for (int b = 0; b < nn; ++b)
-        if ((n >> b) & 1) ob += nbits[b].wb, oc += nbits[b].wc;
-      r.nb[n] = ob;
-      r.nc[n] = oc;
-    }
-    return;
   }
-  if (nk >= 3 && nn >= 3 && nm >= 5) {
-    st.kern = Kern::Gemm;
-    GemmArgs &g = st.gemm;
-    auto by_c = [](const CB &x, const CB &y) { return x.wc < y.wc; };
-    std::sort(mb.begin(), mb.end(), by_c);
-    std::sort(nbits.begin(), nbits.end(), by_c);
-    std::sort(kbits.begin(), kbits.end());
-    g.nbh = (int)hb.size();
-    g.nbm = nm;
-    g.nbn = nn;
-    g.nbk = nk;
-    for (int i = 0; i < g.nbh; ++i) g.ha[i] = hb[i].wa, g.hb[i] = hb[i].wb, g.hc[i] = hb[i].wc;
-    for (int i = 0; i < nm; ++i) g.ma[i] = mb[i].wa, g.mc[i] = mb[i].wc;
-    for (int i = 0; i < nn; ++i) g.nb[i] = nbits[i].wb, g.nc[i] = nbits[i].wc;
-    for (int i = 0; i < nk; ++i) g.ka[i] = kbits[i].first, g.kb[i] = kbits[i].second;
-    g.tile = nn <= 4 ? 0 : nn == 5 ? 1 : 2;
-    // complex64 with wide N and M: FP32 matrix cores (4M or 3M)
-    if (precision != QX_C128 && nn >= 6 && nm >= 6) g.tile = 3;
-    const int lm = g.tile >= 2 ? 6 : 7, ln = g.tile == 0 ? 4 : g.tile == 1 ? 5 : 6, lk = 4;
-    auto min_w = [](const int64_t *w, int n) {
-      int64_t m = INT64_MAX;
-      for (int i = 0; i < n; ++i) m = std::min(m, w[i]);
-      return m;
-    };
-    int64_t kb_min_b = INT64_MAX, ka_min = min_w(g.ka, std::min(nk, lk));
-    for (int i = 0; i < std::min(nk, lk); ++i) kb_min_b = std::min(kb_min_b, g.kb[i]);
-    g.a_kfast = ka_min < min_w(g.ma, std::min(nm, lm));
-    g.b_kfast = kb_min_b < min_w(g.nb, std::min(nn, ln));
-    g.c_nfast = min_w(g.nc, std::min(nn, ln)) < min_w(g.mc, std::min(nm, lm));
-    // split K over workgroups when the tiles alone cannot fill the chip
-    const int64_t ntiles = (int64_t(1) << g.nbh) * (nm > lm ? int64_t(1) << (nm - lm) : 1) *
-                           (nn > ln ? int64_t(1) << (nn - ln) : 1);
-    const int64_t chunks = nk > lk ? int64_t(1) << (nk - lk) : 1;
-    g.c_elems = st.c_elems;
-    while (ntiles * g.ksplit < 1024 && chunks / g.ksplit >= 8) g.ksplit *= 2;
-    st.ws_elems = g.ksplit > 1 ? g.ksplit * st.c_elems : 0;
-    return;
-  }
-  if (st.c_elems <= (int64_t(1) << 14) && nk >= 8) {
-    st.kern = Kern::Reduce;
-    // one wave per (C element, K range): enough waves to fill the chip, >= 64 k each
-    while (st.c_elems * b.ksplit < 8192 && (int64_t(1) << nk) / b.ksplit >= 128) b.ksplit *= 2;
-    st.ws_elems = b.ksplit > 1 ? b.ksplit * st.c_elems : 0;
-    return;
-  }
-
+  return out;
 }
 
 Step make_step(const Value &A, const std::vector<int> &alab, bool conj_a, const Value &B,

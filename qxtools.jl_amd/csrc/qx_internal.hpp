@@ -87,7 +87,7 @@ struct BitArgs {
   int64_t ka[kMaxBits], kb[kMaxBits];   // weight of each K-index bit in A and B
   int nbc = 0, nbk = 0;
   int conj_a = 0, conj_b = 0;
-  int ksplit = 1;          // Kern::Reduce: K ranges per C element, partials summed after
+  int ksplit = 1;          // K ranges whose partials are summed after (Reduce and Bits)
 };
 
 // Rows kernel: C[r, n] = sum_k A[r, k] B[h(r), k, n] with r over the H and M bits
@@ -156,6 +156,7 @@ struct ArenaSlot {
 struct Lowered {
   int64_t batch = 0;
   uint64_t fused = 0;               // slice variables contracted inline (bit j = variable j)
+  bool tuned = false;               // kernel choices measured by the autotuner
   std::vector<ArenaSlot> slots;     // arena reuse schedule
   std::vector<Value> values;        // per node
   std::vector<Step> steps;          // ncon nodes in order
@@ -211,7 +212,9 @@ int64_t first_fit(const std::vector<int64_t> &start, const std::vector<int64_t> 
 // ----------------------------------------------------------------------------
 // Kernel launch interface (kernels.hip).
 void set_kernel_args(Step &st, int precision = QX_C128);
-void launch_contract_bits(int prec, const void *A, const void *B, void *C, int64_t c_elems,
+// Alternative kernel configurations of a step (the current one first) for the autotuner.
+std::vector<Step> kernel_candidates(const Step &st, int precision);
+void launch_contract_bits(int prec, const void *A, const void *B, void *C, void *ws, int64_t c_elems,
                           const BitArgs &args, void *stream);
 // With args.ksplit > 1, `ws` receives ksplit partials of C which are then summed
 // into C.

@@ -181,7 +181,7 @@ void launch_step(qx_plan *P, const qx::Step &st, void *A, void *B, void *C) {
     case qx::Kern::Gemm: qx::launch_contract_gemm(P->prec, A, B, C, P->ws.p, st.gemm, stream); break;
     case qx::Kern::Reduce: qx::launch_contract_reduce(P->prec, A, B, C, P->ws.p, st.c_elems, st.bits, stream); break;
     case qx::Kern::Rows: qx::launch_contract_rows(P->prec, A, B, C, st.rows, stream); break;
-    default: qx::launch_contract_bits(P->prec, A, B, C, st.c_elems, st.bits, stream); break;
+    default: qx::launch_contract_bits(P->prec, A, B, C, P->ws.p, st.c_elems, st.bits, stream); break;
   }
 }
 
@@ -815,12 +815,13 @@ void qx_plan_destroy(qx_plan *plan) {
 
 const char *qx_last_error(void) { return g_last_error.c_str(); }
 
-qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t batch, char *json, size_t cap) {
+qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t batch, uint64_t fused, char *json,
+                         size_t cap) {
   return guard([&] {
     if (!json || cap == 0) throw qx::Error(QX_ERR_ARGUMENT, "null output buffer");
     if (precision < QX_C128 || precision > QX_C64_BF16X3) throw qx::Error(QX_ERR_ARGUMENT, "unknown precision");
     qx::Program prog = qx::parse_program(dsl, len);
-    qx::Lowered L = qx::lower(prog, batch, false, precision);
+    qx::Lowered L = qx::lower(prog, batch, false, precision, fused);
     std::ostringstream os;
     os.precision(17);
     lowered_json(os, prog, L, precision == QX_C128 ? 16 : 8, true);
@@ -828,7 +829,7 @@ qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t bat
     // reverse-mode lowering (its own host-side bounds proof runs here too)
     std::string v;
     try {
-      qx::VjpLowered V = qx::lower_vjp(prog, batch, precision);
+      qx::VjpLowered V = qx::lower_vjp(prog, batch, precision, fused);
       size_t nb = V.bsteps[0].size() + V.bsteps[1].size() + V.bsteps[2].size();
       v = ",\"vjp\":{\"backward_steps\":" + std::to_string(nb) + ",\"forward_elems\":" +
           std::to_string(V.fwd.arena_elems) + ",\"cot_elems\":" + std::to_string(V.cot_elems) +

@@ -54,7 +54,7 @@ def lib() -> C.CDLL:
         L.qx_plan_stats.argtypes = [p, C.c_char_p, sz]
         L.qx_plan_destroy.argtypes = [p]
         L.qx_plan_destroy.restype = None
-        L.qx_dsl_inspect.argtypes = [C.c_char_p, sz, C.c_int, i64, C.c_char_p, sz]
+        L.qx_dsl_inspect.argtypes = [C.c_char_p, sz, C.c_int, i64, C.c_uint64, C.c_char_p, sz]
         L.qx_last_error.argtypes = []
         L.qx_last_error.restype = C.c_char_p
         for f in SYMBOLS:
@@ -73,13 +73,14 @@ def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def inspect(dsl: str, precision: str = "c128", batch: int = 1) -> dict:
-    """Lower a program on the host (no device): steps, kernels, arena schedule."""
+def inspect(dsl: str, precision: str = "c128", batch: int = 1, fused: int = 0) -> dict:
+    """Lower a program on the host (no device): steps, kernels, arena schedule.
+    `fused`: bit mask of slice variables contracted inline."""
     b = dsl.encode()
     cap = 1 << 16
     while True:
         buf = C.create_string_buffer(cap)
-        st = lib().qx_dsl_inspect(b, len(b), PRECISIONS[precision], batch, buf, cap)
+        st = lib().qx_dsl_inspect(b, len(b), PRECISIONS[precision], batch, fused, buf, cap)
         if st == 0:
             return json.loads(buf.value.decode())
         if st != -1 or cap > (1 << 28):

@@ -78,7 +78,7 @@ def test_reduce_kernel(shape):
     nh, nm, nn, nk = shape
     dsl, data, ref = single_step(7, nh, nm, nn, nk)
     out, kern = run(dsl, data)
-    assert kern == "reduce"
+    assert kern in ("reduce", "bits")      # lanes along K, or along C with K split over workgroups
     assert err(out, ref) < 1e-12
 
 

@@ -129,9 +129,12 @@ def test_kernel_choice_matches_gpu_test_shapes():
     for shape in rows:
         dsl, _, _ = K.single_step(13 + sum(shape), *shape, c_rows_first=True)
         assert _native.inspect(dsl)["steps"][0]["kernel"] == "rows", shape
-    for shape in [(0, 0, 0, 10), (3, 0, 0, 9), (0, 2, 2, 8), (1, 3, 0, 12), (6, 0, 0, 8)]:
+    for shape in [(0, 0, 0, 10), (3, 0, 0, 9), (0, 2, 2, 8)]:
         dsl, _, _ = K.single_step(7, *shape)
         assert _native.inspect(dsl)["steps"][0]["kernel"] == "reduce", shape
+    for shape in [(1, 3, 0, 12), (6, 0, 0, 8)]:        # C's bits have the unit stride: lanes along C
+        dsl, _, _ = K.single_step(7, *shape)
+        assert _native.inspect(dsl)["steps"][0]["kernel"] == "bits", shape
     dsl, _, _ = K.single_step(5, 1, 2, 2, 2, dims={1: 3, 2: 5, 5: 3, 6: 7})
     assert _native.inspect(dsl)["steps"][0]["kernel"] == "radix"
 
