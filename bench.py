@@ -88,7 +88,10 @@ def c64_comparison(dsl: str, data: dict, bits: list, ref: np.ndarray, a, precs=(
 
     res = {}
     for prec in precs:
-        ex = Executor(dsl, data, precision=prec, device=0, batch=min(256, len(bits)), graph=a.graph)
+        ex = Executor(dsl, data, precision=prec, device=0, batch=min(256, len(bits)), graph=a.graph,
+                      autotune=a.autotune)
+        if a.autotune:
+            ex.amplitudes(bits)
         for _ in range(a.warmup):
             ex.amplitudes(bits)
         torch.cuda.synchronize()
@@ -164,6 +167,8 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=1, help="replay each call from a captured hipGraph")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64 comparison")
     ap.add_argument("--grad", action="store_true", help="gradient / forward time of the parameterised workload")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="time every candidate kernel per step in one untimed call first, keep the fastest")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the steps, copy this many MiB device to device (counter calibration)")
     a = ap.parse_args()
@@ -185,7 +190,12 @@ def main() -> int:
     prec = a.precision or w.precision
     dsl, data, bits = w.load()
     log(f"workload {w.name}: {len(bits)} bitstrings, precision {prec}")
-    ex = Executor(dsl, data, precision=prec, device=0, batch=a.batch or min(256, len(bits)), graph=a.graph)
+    ex = Executor(dsl, data, precision=prec, device=0, batch=a.batch or min(256, len(bits)), graph=a.graph,
+                  autotune=a.autotune)
+    if a.autotune:
+        t_tune = time.perf_counter()
+        ex.amplitudes(bits)
+        log(f"autotuning call done in {time.perf_counter() - t_tune:.2f} s")
     for i in range(a.warmup):
         ex.amplitudes(bits)
         log(f"warmup {i + 1}/{a.warmup} done")
@@ -235,7 +245,7 @@ def main() -> int:
         "data": "synthetic: seeded random grid circuit and seeded bitstrings",
         "config": {"workload": w.name, "qubits": w.qubits, "depth": w.depth, "bitstrings": len(bits),
                    "amplitudes_per_step": per_call, "note": w.note or None,
-                   "sliced_bonds": w.sliced, "precision": prec,
+                   "sliced_bonds": w.sliced, "precision": prec, "autotune": bool(a.autotune),
                    "batch": next(b for b in stats["batches"] if b.get("active"))["batch"]},
         "roofline": roof,
         "plan_roofline_frac": plan_roofline(stats, prec, ms_step, a.steps),

@@ -73,14 +73,24 @@ qx_status qx_plan_from_dsl(qx_ctx *ctx, const char *dsl, size_t len, int precisi
  * `nelem` interleaved complex128 values in column-major order; it is copied. */
 qx_status qx_plan_set_tensor(qx_plan *plan, const char *key, const double *z, int64_t nelem);
 
-/* Plan options, set before qx_plan_finalize ("profile" and "graph" may also be
- * set later; setting "profile" clears the accumulated step times):
+/* Plan options, set before qx_plan_finalize ("profile", "graph", "fuse_slices" and
+ * "autotune" may also be set later; setting "profile" clears the accumulated step
+ * times):
  *   "batch"       bitstrings contracted together (power of two, default 256)
  *   "arena_mb"    device arena cap in MiB (default: 90% of free memory)
  *   "graph"       1 = capture the device work of a qx_amplitudes call (all slices
  *                 and batches) once per (batch, nb, slice range) and replay it;
  *                 ignored while "profile" is on
- *   "profile"     1 = time every step with HIP events (qx_plan_stats) */
+ *   "profile"     1 = time every step with HIP events (qx_plan_stats)
+ *   "fuse_slices" 1 (default) = when a call covers every slice, contract as many
+ *                 slice variables inline as fit half the arena cap instead of
+ *                 looping over them; 0 = always loop
+ *   "autotune"    1 = the next call of each lowering runs every candidate kernel of
+ *                 every step (same results), times them with HIP events and keeps
+ *                 the fastest per step for later calls; default 0
+ *   "kernel"      -1 (default) = heuristic kernel choice; 0 gather, 1 rows, 2-4
+ *                 FP tile GEMMs, 5 matrix-core GEMM (complex64), 6 reduce = use
+ *                 that kernel on every step it can run (testing) */
 qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value);
 
 /* Check that every data key is present, lower the program (label classes,

@@ -274,6 +274,16 @@ void set_kernel_args(Step &st, int precision) {
     throw Error(QX_ERR_STATE, "internal: default kernel cannot run its step");
 }
 
+bool force_kernel(Step &st, int precision, int choice) {
+  if (st.kern == Kern::Radix) return false;
+  Step v = st;
+  if (!configure_kernel(v, precision, choice)) return false;
+  v.bits.conj_a = v.gemm.conj_a = v.rows.conj_a = st.bits.conj_a;
+  v.bits.conj_b = v.gemm.conj_b = v.rows.conj_b = st.bits.conj_b;
+  st = std::move(v);
+  return true;
+}
+
 std::vector<Step> kernel_candidates(const Step &st, int precision) {
   std::vector<Step> out;
   out.push_back(st);
@@ -288,11 +298,7 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
   for (int c : alts) {
     if (c == def) continue;
     Step v = st;
-    if (configure_kernel(v, precision, c)) {
-      v.bits.conj_a = v.gemm.conj_a = v.rows.conj_a = st.bits.conj_a;
-      v.bits.conj_b = v.gemm.conj_b = v.rows.conj_b = st.bits.conj_b;
-      out.push_back(std::move(v));
-    }
+    if (force_kernel(v, precision, c)) out.push_back(std::move(v));
   }
   return out;
 }

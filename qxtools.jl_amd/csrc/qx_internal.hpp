@@ -157,6 +157,7 @@ struct Lowered {
   int64_t batch = 0;
   uint64_t fused = 0;               // slice variables contracted inline (bit j = variable j)
   bool tuned = false;               // kernel choices measured by the autotuner
+  std::vector<std::string> tune_log;  // per step: mean ms of each candidate (JSON members)
   std::vector<ArenaSlot> slots;     // arena reuse schedule
   std::vector<Value> values;        // per node
   std::vector<Step> steps;          // ncon nodes in order
@@ -212,6 +213,9 @@ int64_t first_fit(const std::vector<int64_t> &start, const std::vector<int64_t> 
 // ----------------------------------------------------------------------------
 // Kernel launch interface (kernels.hip).
 void set_kernel_args(Step &st, int precision = QX_C128);
+// Run `st` with kernel configuration `choice` (0 gather, 1 rows, 2-4 FP tile GEMMs,
+// 5 matrix-core GEMM, 6 reduce); false (and `st` unchanged) if that kernel cannot.
+bool force_kernel(Step &st, int precision, int choice);
 // Alternative kernel configurations of a step (the current one first) for the autotuner.
 std::vector<Step> kernel_candidates(const Step &st, int precision);
 void launch_contract_bits(int prec, const void *A, const void *B, void *C, void *ws, int64_t c_elems,

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -84,6 +85,8 @@ struct qx_plan {
   int64_t opt_arena_mb = -1;
   int64_t opt_graph = 0;
   int64_t opt_profile = 0;
+  int64_t opt_autotune = 0;
+  int64_t opt_kernel = -1;                  // >= 0: force that kernel wherever it can run
   bool finalized = false;
 
   std::vector<DevBuf> data;                 // per key
@@ -98,6 +101,7 @@ struct qx_plan {
   std::vector<double> step_ms;
   std::vector<int64_t> step_calls;
   std::vector<hipEvent_t> events;           // pool: 2 per profiled launch
+  std::vector<hipEvent_t> tune_events;      // pool: 2 per timed candidate launch
   // captured device work of qx_amplitudes per (batch, nb, slice_begin, slice_end)
   std::map<std::tuple<int64_t, int64_t, int64_t, int64_t>, hipGraphExec_t> graphs;
   void drop_graphs() {
@@ -124,6 +128,7 @@ struct qx_plan {
     ampcot.release();
     ws.release();
     for (auto e : events) (void)hipEventDestroy(e);
+    for (auto e : tune_events) (void)hipEventDestroy(e);
     drop_graphs();
   }
 };
@@ -135,6 +140,9 @@ const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch, uint64_t fused = 0)
   auto it = P->lowered.find(key);
   if (it != P->lowered.end()) return it->second;
   qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, fused);
+  if (P->opt_kernel >= 0)
+    for (auto &st : L.steps)
+      if (qx::force_kernel(st, P->prec, (int)P->opt_kernel)) L.ws_elems = std::max(L.ws_elems, st.ws_elems);
   size_t need = (size_t)L.arena_elems * P->esize();
   if (P->arena_cap_bytes > 0 && (int64_t)need > P->arena_cap_bytes)
     throw qx::Error(QX_ERR_MEMORY, "arena of " + std::to_string(need >> 20) + " MiB for batch " +
@@ -204,6 +212,74 @@ void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int6
   }
   launch_step(P, st, A, B, C);
   if (P->opt_profile) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
+}
+
+std::string kern_variant(const qx::Step &st);
+
+// The autotuner's state for one call: every step's candidate configurations and
+// the launches timed so far.  Candidates of a step all write the same C, so
+// running them one after another leaves the results unchanged.
+struct Tuner {
+  static constexpr int kTimedLaunches = 4;   // timed launches per candidate per step
+  std::vector<std::vector<qx::Step>> cands;
+  std::vector<std::vector<double>> ms;
+  std::vector<int> seen;
+  std::vector<std::array<int, 3>> rec;        // (step, candidate, event pair)
+};
+
+void tune_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int64_t> &varval, Tuner &T) {
+  const qx::Step &st = L.steps[s];
+  void *A = value_ptr(P, L.values[st.in_a], varval);
+  void *B = value_ptr(P, L.values[st.in_b], varval);
+  void *C = value_ptr(P, L.values[st.node], varval);
+  const auto &cs = T.cands[s];
+  if (cs.size() < 2 || T.seen[s] >= Tuner::kTimedLaunches) {
+    launch_step(P, st, A, B, C);
+    return;
+  }
+  hipStream_t stream = P->ctx->stream;
+  if (T.seen[s] == 0)   // first launch of every candidate untimed: code loading, cold caches
+    for (auto &c : cs) launch_step(P, c, A, B, C);
+  ++T.seen[s];
+  for (size_t c = 0; c < cs.size(); ++c) {
+    size_t ev = T.rec.size();
+    while (P->tune_events.size() < 2 * (ev + 1)) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      P->tune_events.push_back(e);
+    }
+    T.rec.push_back({(int)s, (int)c, (int)ev});
+    hip_check(hipEventRecord(P->tune_events[2 * ev], stream), "hipEventRecord");
+    launch_step(P, cs[c], A, B, C);
+    hip_check(hipEventRecord(P->tune_events[2 * ev + 1], stream), "hipEventRecord");
+  }
+}
+
+// After the tuning call has drained: keep each step's fastest candidate (the
+// current choice on ties) and record what was measured.
+void finish_tuning(qx_plan *P, qx::Lowered &L, Tuner &T) {
+  for (auto &r : T.rec) {
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, P->tune_events[2 * r[2]], P->tune_events[2 * r[2] + 1]),
+              "hipEventElapsedTime");
+    T.ms[r[0]][r[1]] += ms;
+  }
+  L.tune_log.assign(L.steps.size(), std::string());
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    if (T.cands[s].size() < 2 || T.seen[s] == 0) continue;
+    size_t best = 0;
+    std::ostringstream os;
+    os.precision(4);
+    for (size_t c = 0; c < T.cands[s].size(); ++c) {
+      if (T.ms[s][c] < T.ms[s][best]) best = c;
+      os << (c ? "," : "") << "\"" << kern_variant(T.cands[s][c]) << "\":" << T.ms[s][c] / T.seen[s];
+    }
+    L.tune_log[s] = os.str();
+    L.steps[s] = T.cands[s][best];
+    L.ws_elems = std::max(L.ws_elems, L.steps[s].ws_elems);
+  }
+  L.tuned = true;
+  P->drop_graphs();
 }
 
 // After the stream has drained: add every recorded launch's time to its step.
@@ -303,7 +379,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     s0 = 0;
     s1 = looped_slices(P, fused);
   }
-  const qx::Lowered &L = ensure_lowered(P, batch, fused);
+  qx::Lowered &L = const_cast<qx::Lowered &>(ensure_lowered(P, batch, fused));
   if (P->last_lowered != &L) {
     P->step_ms.clear();
     P->step_calls.clear();
@@ -311,8 +387,19 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   P->last_lowered = &L;
   hipStream_t stream = P->ctx->stream;
   const size_t es = P->esize();
+  std::unique_ptr<Tuner> tuner;
+  int64_t ws_elems = L.ws_elems;
+  if (P->opt_autotune && !L.tuned) {
+    tuner = std::make_unique<Tuner>();
+    for (auto &st : L.steps) {
+      tuner->cands.push_back(qx::kernel_candidates(st, P->prec));
+      tuner->ms.emplace_back(tuner->cands.back().size(), 0.0);
+      for (auto &c : tuner->cands.back()) ws_elems = std::max(ws_elems, c.ws_elems);
+    }
+    tuner->seen.assign(L.steps.size(), 0);
+  }
   bool moved = P->arena.reserve(std::max<size_t>((size_t)L.arena_elems * es, 256));
-  moved |= P->ws.reserve(std::max<size_t>((size_t)L.ws_elems * es, 256));
+  moved |= P->ws.reserve(std::max<size_t>((size_t)ws_elems * es, 256));
 
   // output tables
   std::vector<int64_t> ooff;
@@ -358,8 +445,13 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     const size_t nv = P->prog.vars.size();
     std::vector<int64_t> varval(nv, 0);
     auto run_section = [&](int sec) {
-      for (size_t s = 0; s < L.steps.size(); ++s)
-        if (L.steps[s].section == sec) run_step(P, L, s, varval);
+      for (size_t s = 0; s < L.steps.size(); ++s) {
+        if (L.steps[s].section != sec) continue;
+        if (tuner)
+          tune_step(P, L, s, varval, *tuner);
+        else
+          run_step(P, L, s, varval);
+      }
     };
     run_section(0);
     for (int64_t sl = s0; sl < s1; ++sl) {
@@ -376,7 +468,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     }
     qx::launch_finalize((const double *)P->acc.p, (double *)P->outbuf.p, nb * R, stream);
   };
-  if (P->opt_graph && !P->opt_profile) {
+  if (P->opt_graph && !P->opt_profile && !tuner) {
     auto key = std::make_tuple(batch, nb, s0, s1 + (fused ? (int64_t(1) << 40) : 0));
     auto it = P->graphs.find(key);
     if (it == P->graphs.end()) {
@@ -402,6 +494,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   }
   hip_check(hipMemcpyAsync(out, P->outbuf.p, (size_t)nb * R * 16, hipMemcpyDeviceToHost, stream), "copy out");
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  if (tuner) finish_tuning(P, L, *tuner);
   collect_profile(P, L);
 }
 
@@ -456,7 +549,11 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
     os << "{\"name\":\"" << prog.nodes[st.node].name << "\",\"section\":" << st.section << ",\"B\":" << st.H
        << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K << ",\"flops\":" << flops
        << ",\"bytes\":" << bytes << ",\"kernel\":\"" << kern_name(st.kern) << "\",\"variant\":\""
-       << kern_variant(st) << "\"}";
+       << kern_variant(st) << "\",\"ws_elems\":" << st.ws_elems << ",\"candidates\":[";
+    auto cands = qx::kernel_candidates(st, es == 16 ? QX_C128 : QX_C64_4M);
+    for (size_t c = 0; c < cands.size(); ++c)
+      os << (c ? "," : "") << "\"" << kern_variant(cands[c]) << "\"";
+    os << "]}";
   }
   os << "]}";
 }
@@ -466,6 +563,14 @@ const qx::VjpLowered &ensure_vjp(qx_plan *P, int64_t batch, uint64_t fused) {
   auto it = P->vjp_lowered.find(key);
   if (it != P->vjp_lowered.end()) return it->second;
   qx::VjpLowered V = qx::lower_vjp(P->prog, batch, P->prec, fused);
+  if (P->opt_kernel >= 0) {
+    auto force = [&](qx::Step &st) {
+      if (qx::force_kernel(st, P->prec, (int)P->opt_kernel)) V.fwd.ws_elems = std::max(V.fwd.ws_elems, st.ws_elems);
+    };
+    for (auto &st : V.fwd.steps) force(st);
+    for (auto &sec : V.bsteps)
+      for (auto &b : sec) force(b.step);
+  }
   const size_t es = P->esize();
   size_t need = (size_t)(V.fwd.arena_elems + V.cot_elems + V.temp_elems) * es;
   if (P->arena_cap_bytes > 0 && (int64_t)need > P->arena_cap_bytes)
@@ -611,6 +716,7 @@ std::string stats_json(const qx_plan *P) {
     if (!first) os << ",";
     first = false;
     os << "{\"batch\":" << L.batch << ",\"fused\":" << L.fused << ",\"active\":" << (&L == P->last_lowered)
+       << ",\"tuned\":" << L.tuned
        << ",\"arena_bytes\":" << L.arena_elems * (int64_t)P->esize() << ",\"result_elems\":" << L.result_elems
        << ",\"steps\":[";
     for (size_t s = 0; s < L.steps.size(); ++s) {
@@ -624,6 +730,7 @@ std::string stats_json(const qx_plan *P) {
          << kern_name(st.kern) << "\",\"variant\":\"" << kern_variant(st) << "\"";
       if (P->opt_profile && &L == P->last_lowered && s < P->step_ms.size())
         os << ",\"ms\":" << P->step_ms[s] << ",\"calls\":" << P->step_calls[s];
+      if (s < L.tune_log.size() && !L.tune_log[s].empty()) os << ",\"tuning_ms\":{" << L.tune_log[s] << "}";
       os << "}";
     }
     os << "]}";
@@ -693,7 +800,7 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
   return guard([&] {
     if (!plan || !name) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
     std::string n(name);
-    bool runtime_opt = (n == "profile" || n == "graph" || n == "fuse_slices");
+    bool runtime_opt = (n == "profile" || n == "graph" || n == "fuse_slices" || n == "autotune");
     if (plan->finalized && !runtime_opt) throw qx::Error(QX_ERR_STATE, "option '" + n + "' must be set before finalize");
     if (n == "batch") {
       if (value <= 0 || (value & (value - 1)) || value > (1 << 20))
@@ -706,6 +813,11 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
       plan->opt_graph = value;
     } else if (n == "fuse_slices") {
       plan->opt_fuse = value;
+    } else if (n == "autotune") {
+      plan->opt_autotune = value;
+    } else if (n == "kernel") {
+      if (value < -1 || value > 6) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..6");
+      plan->opt_kernel = value;
     } else if (n == "profile") {
       // may be toggled at any time; every set clears the accumulated step times
       plan->opt_profile = value;

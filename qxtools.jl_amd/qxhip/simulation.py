@@ -52,11 +52,13 @@ class Executor:
 
     def __init__(self, dsl: str, data: dict, precision: str = "c128", device: int = 0,
                  batch: int | None = None, arena_mb: int | None = None, graph: int | None = None,
-                 profile: int | None = None):
+                 profile: int | None = None, **options):
+        """`options` are further `qx_plan_set_option` names (autotune, kernel,
+        fuse_slices), set before finalize."""
         self.dsl = dsl
         self.precision = precision
         self.plan = _native.Plan(context(device), dsl, data, precision, batch=batch,
-                                 arena_mb=arena_mb, graph=graph, profile=profile)
+                                 arena_mb=arena_mb, graph=graph, profile=profile, **options)
         self.num_chars = self.plan.num_chars
         self.num_slices = self.plan.num_slices
 

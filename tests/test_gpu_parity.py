@@ -163,3 +163,25 @@ def test_fused_slices_equal_looped_slices():
     assert act["fused"] == 0
     per = sum(O.amplitudes(dsl, g.data, bits, slices=[s]) for s in list(O.slice_assignments(O.parse(dsl)))[2:9])
     assert np.max(qxtest.rel_err(part, per)) <= 1e-11
+
+
+@pytest.mark.parametrize("nslice", [0, 3])
+def test_autotune_keeps_results(nslice):
+    """The tuning call runs every candidate kernel of every step and keeps the
+    fastest; the tuning call and the tuned (graph-replayed) calls after it match the
+    oracle, and the choices show in the stats."""
+    circ = circuits.create_rqc_circuit(3, 3, 12, seed=21)
+    t, plan, g = qxtest.graph_for(circ, planner="greedy", nslice=nslice)
+    dsl = g.to_dsl()
+    bits = simulation.amplitudes_uniform(9, 9, 32)
+    ref = O.amplitudes(dsl, g.data, bits)
+    ex = Executor(dsl, g.data, batch=16, graph=1, autotune=1)
+    for call in range(3):
+        out = ex.amplitudes(bits)
+        assert np.max(qxtest.rel_err(out, ref)) <= TOL64, call
+    act = [x for x in ex.stats()["batches"] if x["active"]][0]
+    assert act["tuned"]
+    timed = [s for s in act["steps"] if "tuning_ms" in s]
+    assert timed and all(s["variant"] in s["tuning_ms"] for s in timed)
+    for s in timed:   # the kept variant is the fastest measured
+        assert s["tuning_ms"][s["variant"]] <= min(s["tuning_ms"].values()) + 1e-12

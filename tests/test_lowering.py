@@ -145,3 +145,20 @@ def test_complex64_wide_gemm_uses_matrix_cores():
         steps = _native.inspect(dsl, precision="c64_4m")["steps"]
         assert steps[0]["variant"] == "k_contract_gemm_mfma", case
         assert _native.inspect(dsl)["steps"][0]["variant"] != "k_contract_gemm_mfma"
+
+
+@pytest.mark.parametrize("name", ["rqc_3x3_d16", "rqc_5x5_d20", "rqc_6x6_d12_s6"])
+def test_autotune_candidates(name):
+    """Every step lists its alternatives for the autotuner: the chosen kernel first,
+    no duplicates, every GEMM-shaped step all three FP64 tiles."""
+    w = workloads.WORKLOADS[name]
+    dsl, data, bits = w.load()
+    prec = "c128" if w.precision == "c128" else w.precision
+    info = _native.inspect(dsl, precision=prec, batch=64)
+    for st in info["steps"]:
+        c = st["candidates"]
+        assert c[0] == st["variant"] and len(set(c)) == len(c)
+        if st["kernel"] == "gemm":
+            assert {"k_contract_gemm<128,16,16>", "k_contract_gemm<128,32,16>", "k_contract_gemm<64,64,16>"} <= set(c)
+        if prec == "c128":
+            assert "k_contract_gemm_mfma" not in c
