@@ -119,6 +119,30 @@ qx_status qx_plan_key(const qx_plan *plan, int64_t i, char *name, size_t cap, in
 qx_status qx_amplitudes(qx_plan *plan, const char *bits, int64_t nb, int64_t slice_begin,
                         int64_t slice_end, double *out);
 
+/* Output sampling on the device (the output methods of output_params_dict,
+ * src/outputs.jl:47-78, executed by QXContexts and described in
+ * docs/src/features.md:64-110).  Candidate j (0-based) of stream `seed` is the
+ * bitstring whose character q is bit q of the Philox4x64-10 block at counter
+ * (j,0,0,0), key (seed,0); its acceptance uniform u_j is the top 53 bits of the
+ * first word of the block at key (seed,1), times 2^-53.  At most 256 qubits.
+ *
+ * :Uniform -- amplitudes of candidates 0..num-1 (all slices).  bits_out:
+ * num * num_chars characters '0'/'1'; out: num * result_elems complex128. */
+qx_status qx_amplitudes_uniform(qx_plan *plan, int64_t num, uint64_t seed, char *bits_out, double *out);
+
+/* :Rejection -- empirical-supremum rejection sampling (Caffo, Booth, Davison,
+ * Biometrika 89(4) 2002) with the uniform proposal: in candidate order,
+ * r_j = |a_j|^2 * 2^num_chars, M_j = M_{j-1} (fix_M) or max(M_{j-1}, r_j), accept
+ * when u_j * M_j < r_j (FP64, no fused multiply-add); stops at the num_samples-th
+ * acceptance.  Candidates are contracted one batch at a time and tested on the
+ * device.  Writes the accepted bitstrings (num_samples * num_chars) and their
+ * amplitudes (num_samples complex128), the final M and the number of candidates
+ * consumed.  QX_ERR_STATE when max_candidates run out first (outputs then hold
+ * what was accepted); QX_ERR_UNSUPPORTED when output legs are left open. */
+qx_status qx_sample_rejection(qx_plan *plan, int64_t num_samples, uint64_t seed, double M, int fix_M,
+                              int64_t max_candidates, char *bits_out, double *amps_out, double *M_out,
+                              int64_t *candidates_out);
+
 /* Vector-Jacobian product of qx_amplitudes over all slices: given amp_cot (nb *
  * result_elems complex), writes for every data key (qx_plan_key order, keys
  * concatenated) grad = sum_b amp_cot_b * conj(d amp_b / d T_key), summed over all

@@ -984,6 +984,117 @@ void launch_contract_radix(int prec, const void *A, const void *B, void *C, int6
   check_launch("contract_radix");
 }
 
+// ---------------------------------------------------------------------------
+// Output sampling on the device.  Counter-based Philox4x64-10 (Salmon, Moraes,
+// Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC 2011): candidate j
+// of a stream with seed s draws its bits from the block at counter (j, 0, 0, 0),
+// key (s, 0) -- character q is bit q of the 256-bit block -- and its acceptance
+// uniform from the first word of the block at key (s, 1), top 53 bits.
+__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
+  const uint64_t M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
+  const uint64_t W0 = 0x9E3779B97F4A7C15ull, W1 = 0xBB67AE8584CAA73Bull;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t hi0 = __umul64hi(M0, c[0]), lo0 = M0 * c[0];
+    const uint64_t hi1 = __umul64hi(M1, c[2]), lo1 = M1 * c[2];
+    const uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += W0;
+    k1 += W1;
+  }
+}
+
+__global__ void k_gen_bits(char *__restrict__ bits, int64_t nb, int nch, uint64_t seed, int64_t j0) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t c[4] = {(uint64_t)(j0 + i), 0, 0, 0};
+    philox4x64(c, seed, 0);
+    char *o = bits + i * nch;
+    for (int q = 0; q < nch; ++q) o[q] = '0' + (char)((c[q >> 6] >> (q & 63)) & 1);
+  }
+}
+
+// Empirical-supremum rejection over candidates j0 .. j0+nb-1 (amplitudes `amps`,
+// bitstrings `cand`), in candidate order: r_j = |a_j|^2 * scale, M_j = M_{j-1} or
+// max(M_{j-1}, r_j) when M is not fixed, accept when u_j * M_j < r_j; stop at the
+// num_samples-th acceptance.  One workgroup of 256 lanes walks the candidates 256
+// at a time with max and sum scans in LDS.  state[0] = M; cnt[0] = accepted so
+// far, cnt[1] = candidates consumed so far.
+__global__ __launch_bounds__(256) void k_reject(const double2 *__restrict__ amps, const char *__restrict__ cand,
+                                                int64_t nb, int nch, uint64_t seed, int64_t j0, double scale,
+                                                int fix_m, int64_t num_samples, double *state, int64_t *cnt,
+                                                char *__restrict__ out_bits, double2 *__restrict__ out_amps) {
+  __shared__ double smax[256];
+  __shared__ int64_t ssum[256];
+  __shared__ int64_t s_stop;
+  const int t = threadIdx.x;
+  double carry_m = state[0];
+  int64_t carry_n = cnt[0];
+  int64_t consumed = nb;
+  for (int64_t base = 0; base < nb; base += 256) {
+    const int64_t j = base + t;
+    const bool live = j < nb;
+    double r = 0.0;
+    double2 a = {0.0, 0.0};
+    if (live) {
+      a = amps[j];
+      r = __dmul_rn(__dadd_rn(__dmul_rn(a.x, a.x), __dmul_rn(a.y, a.y)), scale);
+    }
+    // inclusive max scan of r (M_j)
+    smax[t] = live ? r : 0.0;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      double v = t >= off ? smax[t - off] : 0.0;
+      __syncthreads();
+      smax[t] = fmax(smax[t], v);
+      __syncthreads();
+    }
+    const double mj = fix_m ? carry_m : fmax(carry_m, smax[t]);
+    bool acc = false;
+    if (live) {
+      uint64_t c[4] = {(uint64_t)(j0 + j), 0, 0, 0};
+      philox4x64(c, seed, 1);
+      const double u = (double)(c[0] >> 11) * 0x1.0p-53;
+      acc = __dmul_rn(u, mj) < r;
+    }
+    // inclusive sum scan of acceptances
+    ssum[t] = acc ? 1 : 0;
+    if (t == 0) s_stop = -1;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      int64_t v = t >= off ? ssum[t - off] : 0;
+      __syncthreads();
+      ssum[t] += v;
+      __syncthreads();
+    }
+    const int64_t rank = carry_n + ssum[t] - 1;   // output slot of an accepted candidate
+    if (acc && rank < num_samples) {
+      for (int q = 0; q < nch; ++q) out_bits[rank * nch + q] = cand[j * nch + q];
+      out_amps[rank] = a;
+      if (rank == num_samples - 1) s_stop = j;
+    }
+    __syncthreads();
+    const int64_t stop = s_stop;
+    const int last = stop >= 0 ? (int)(stop - base) : (int)((nb - base < 256 ? nb - base : 256) - 1);
+    const double m_last = fix_m ? carry_m : fmax(carry_m, smax[last]);
+    const int64_t n_last = carry_n + ssum[last];
+    __syncthreads();
+    carry_m = m_last;
+    carry_n = n_last;
+    if (stop >= 0) {
+      consumed = stop + 1;
+      break;
+    }
+  }
+  if (t == 0) {
+    state[0] = carry_m;
+    cnt[0] = carry_n;
+    cnt[1] += consumed;
+  }
+}
+
 void launch_fill_outputs(int prec, void *arena, const int64_t *d_out_off, int n_out, const int *d_out_q,
                          const char *d_bits, int64_t nvalid, int64_t batch, int64_t num_chars, void *stream) {
   if (n_out == 0) return;
@@ -1057,6 +1168,20 @@ void launch_zero(void *p, int64_t bytes, void *stream) {
   int64_t n16 = bytes / 16;
   hipLaunchKernelGGL(k_zero, dim3(grid_for(n16, 256)), dim3(256), 0, (hipStream_t)stream, (uint4 *)p, n16);
   check_launch("zero");
+}
+
+void launch_gen_bits(char *bits, int64_t nb, int nch, uint64_t seed, int64_t j0, void *stream) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_gen_bits, dim3(grid_for(nb, 256)), dim3(256), 0, (hipStream_t)stream, bits, nb, nch, seed, j0);
+  check_launch("gen_bits");
+}
+
+void launch_reject(const double *amps, const char *cand, int64_t nb, int nch, uint64_t seed, int64_t j0, double scale,
+                   int fix_m, int64_t num_samples, double *state, int64_t *cnt, char *out_bits, double *out_amps,
+                   void *stream) {
+  hipLaunchKernelGGL(k_reject, dim3(1), dim3(256), 0, (hipStream_t)stream, (const double2 *)amps, cand, nb, nch, seed,
+                     j0, scale, fix_m, num_samples, state, cnt, out_bits, (double2 *)out_amps);
+  check_launch("reject");
 }
 
 }  // namespace qx

@@ -245,6 +245,12 @@ void launch_add_mapped(int prec, int dst_double, void *dst, const void *src, int
 // cotangent of the saved tensor from amp_cot rows [row0, row0 + nvalid)
 void launch_root_cot(int prec, void *dst, int dep, int64_t batch, int64_t nvalid, int64_t relems,
                      const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream);
+// Output sampling: candidate bitstrings from Philox4x64-10 and the sequential
+// empirical-supremum rejection test over one round of candidates.
+void launch_gen_bits(char *bits, int64_t nb, int nch, uint64_t seed, int64_t j0, void *stream);
+void launch_reject(const double *amps, const char *cand, int64_t nb, int nch, uint64_t seed, int64_t j0, double scale,
+                   int fix_m, int64_t num_samples, double *state, int64_t *cnt, char *out_bits, double *out_amps,
+                   void *stream);
 void launch_zero(void *p, int64_t bytes, void *stream);
 
 }  // namespace qx

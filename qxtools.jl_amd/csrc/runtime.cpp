@@ -363,15 +363,20 @@ void check_bits(qx_plan *P, const char *bits, int64_t nb) {
   }
 }
 
-void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1, double *out) {
+// Amplitudes of nb bitstrings summed over slices [s0, s1).  `bits` null: the
+// bitstrings are already in P->bits on the device (sampling); `out` null: the
+// results stay in P->outbuf.
+void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1, double *out,
+                bool device_bits = false) {
   if (!P->finalized) throw qx::Error(QX_ERR_STATE, "plan is not finalized");
   if (nb <= 0) throw qx::Error(QX_ERR_ARGUMENT, "need at least one bitstring");
-  if (!bits || !out) throw qx::Error(QX_ERR_ARGUMENT, "null bitstring or output pointer");
+  if ((!bits && !device_bits) || (!out && !device_bits))
+    throw qx::Error(QX_ERR_ARGUMENT, "null bitstring or output pointer");
   int64_t nslices = 1;
   for (int64_t d : P->prog.var_dims) nslices *= d;
   if (s0 < 0 || s1 > nslices || s0 >= s1) throw qx::Error(QX_ERR_ARGUMENT, "slice range out of bounds");
   const int64_t nch = P->prog.num_chars;
-  check_bits(P, bits, nb);
+  if (bits) check_bits(P, bits, nb);
   int64_t batch = 1;
   while (batch < nb && batch < P->opt_batch) batch <<= 1;
   const uint64_t fused = (s0 == 0 && s1 == nslices) ? choose_fused(P, batch, false) : 0;
@@ -431,8 +436,12 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   moved |= P->relem_off.reserve(std::max<size_t>(R * 8, 16));
   hip_check(hipMemcpyAsync(P->relem_off.p, roff.data(), R * 8, hipMemcpyHostToDevice, stream), "copy");
   size_t nbits = (size_t)nb * nch;
-  moved |= P->bits.reserve(std::max<size_t>(nbits, 16));
-  if (nbits) hip_check(hipMemcpyAsync(P->bits.p, bits, nbits, hipMemcpyHostToDevice, stream), "copy");
+  if (bits) {
+    moved |= P->bits.reserve(std::max<size_t>(nbits, 16));
+    if (nbits) hip_check(hipMemcpyAsync(P->bits.p, bits, nbits, hipMemcpyHostToDevice, stream), "copy");
+  } else if (P->bits.bytes < nbits) {
+    throw qx::Error(QX_ERR_STATE, "internal: device bitstrings not staged");
+  }
   moved |= P->acc.reserve((size_t)nb * R * 32);
   moved |= P->outbuf.reserve((size_t)nb * R * 16);
   if (moved) P->drop_graphs();
@@ -492,10 +501,90 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   } else {
     enqueue();
   }
-  hip_check(hipMemcpyAsync(out, P->outbuf.p, (size_t)nb * R * 16, hipMemcpyDeviceToHost, stream), "copy out");
+  if (out) hip_check(hipMemcpyAsync(out, P->outbuf.p, (size_t)nb * R * 16, hipMemcpyDeviceToHost, stream), "copy out");
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   if (tuner) finish_tuning(P, L, *tuner);
   collect_profile(P, L);
+}
+
+// Stage nb Philox bitstrings (candidates j0 .. j0+nb-1 of stream `seed`) in P->bits.
+void stage_bits(qx_plan *P, int64_t nb, uint64_t seed, int64_t j0) {
+  const int nch = (int)P->prog.num_chars;
+  if (P->bits.reserve(std::max<size_t>((size_t)nb * nch, 16))) P->drop_graphs();
+  qx::launch_gen_bits((char *)P->bits.p, nb, nch, seed, j0, P->ctx->stream);
+}
+
+void check_sampling(const qx_plan *P) {
+  if (!P->finalized) throw qx::Error(QX_ERR_STATE, "plan is not finalized");
+  if (P->prog.num_chars > 256) throw qx::Error(QX_ERR_UNSUPPORTED, "sampling supports at most 256 qubits");
+}
+
+void amplitudes_uniform(qx_plan *P, int64_t num, uint64_t seed, char *bits_out, double *out) {
+  check_sampling(P);
+  if (num <= 0) throw qx::Error(QX_ERR_ARGUMENT, "need at least one bitstring");
+  if (!bits_out || !out) throw qx::Error(QX_ERR_ARGUMENT, "null output pointer");
+  int64_t nslices = 1;
+  for (int64_t d : P->prog.var_dims) nslices *= d;
+  stage_bits(P, num, seed, 0);
+  amplitudes(P, nullptr, num, 0, nslices, out, true);
+  const size_t nbits = (size_t)num * P->prog.num_chars;
+  if (nbits) hip_check(hipMemcpy(bits_out, P->bits.p, nbits, hipMemcpyDeviceToHost), "copy out");
+}
+
+// Rejection sampling in rounds of one batch of candidates: stage the round's
+// bitstrings, contract, run the sequential acceptance test on the device; only the
+// counters come back to the host between rounds.
+void sample_rejection(qx_plan *P, int64_t num_samples, uint64_t seed, double M, int fix_m, int64_t max_candidates,
+                      char *bits_out, double *amps_out, double *m_out, int64_t *cand_out) {
+  check_sampling(P);
+  if (num_samples < 0 || max_candidates < 0) throw qx::Error(QX_ERR_ARGUMENT, "negative count");
+  if (!(M > 0) || !std::isfinite(M)) throw qx::Error(QX_ERR_ARGUMENT, "M must be positive and finite");
+  if ((!bits_out || !amps_out) && num_samples > 0) throw qx::Error(QX_ERR_ARGUMENT, "null output pointer");
+  if (P->lowered.empty() || P->lowered.begin()->second.result_elems != 1)
+    throw qx::Error(QX_ERR_UNSUPPORTED, "rejection sampling needs every output leg fixed (one amplitude per bitstring)");
+  const int nch = (int)P->prog.num_chars;
+  hipStream_t stream = P->ctx->stream;
+  int64_t nslices = 1;
+  for (int64_t d : P->prog.var_dims) nslices *= d;
+  DevBuf obits, oamps, st;
+  obits.reserve(std::max<size_t>((size_t)num_samples * nch, 16));
+  oamps.reserve(std::max<size_t>((size_t)num_samples * 16, 16));
+  st.reserve(32);
+  int64_t cnt[2] = {0, 0};
+  double state[2] = {M, 0.0};
+  hip_check(hipMemcpyAsync(st.p, state, 8, hipMemcpyHostToDevice, stream), "copy");
+  hip_check(hipMemcpyAsync((char *)st.p + 16, cnt, 16, hipMemcpyHostToDevice, stream), "copy");
+  const double scale = std::ldexp(1.0, nch);
+  try {
+    while (cnt[0] < num_samples && cnt[1] < max_candidates) {
+      const int64_t nb = std::min<int64_t>(P->opt_batch, max_candidates - cnt[1]);
+      stage_bits(P, nb, seed, cnt[1]);
+      amplitudes(P, nullptr, nb, 0, nslices, nullptr, true);
+      qx::launch_reject((const double *)P->outbuf.p, (const char *)P->bits.p, nb, nch, seed, cnt[1], scale, fix_m,
+                        num_samples, (double *)st.p, (int64_t *)((char *)st.p + 16), (char *)obits.p,
+                        (double *)oamps.p, stream);
+      hip_check(hipMemcpyAsync(cnt, (char *)st.p + 16, 16, hipMemcpyDeviceToHost, stream), "copy");
+      hip_check(hipMemcpyAsync(state, st.p, 8, hipMemcpyDeviceToHost, stream), "copy");
+      hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+    if (cnt[0] > 0) {
+      hip_check(hipMemcpy(bits_out, obits.p, (size_t)cnt[0] * nch, hipMemcpyDeviceToHost), "copy out");
+      hip_check(hipMemcpy(amps_out, oamps.p, (size_t)cnt[0] * 16, hipMemcpyDeviceToHost), "copy out");
+    }
+  } catch (...) {
+    obits.release();
+    oamps.release();
+    st.release();
+    throw;
+  }
+  obits.release();
+  oamps.release();
+  st.release();
+  if (m_out) *m_out = state[0];
+  if (cand_out) *cand_out = cnt[1];
+  if (cnt[0] < num_samples)
+    throw qx::Error(QX_ERR_STATE, "rejection sampling stopped at max_candidates with " + std::to_string(cnt[0]) +
+                                      " of " + std::to_string(num_samples) + " samples");
 }
 
 const char *kern_name(qx::Kern k) {
@@ -898,6 +987,24 @@ qx_status qx_amplitudes(qx_plan *plan, const char *bits, int64_t nb, int64_t sli
     if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
     hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
     amplitudes(plan, bits, nb, slice_begin, slice_end, out);
+  });
+}
+
+qx_status qx_amplitudes_uniform(qx_plan *plan, int64_t num, uint64_t seed, char *bits_out, double *out) {
+  return guard([&] {
+    if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
+    hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+    amplitudes_uniform(plan, num, seed, bits_out, out);
+  });
+}
+
+qx_status qx_sample_rejection(qx_plan *plan, int64_t num_samples, uint64_t seed, double M, int fix_M,
+                              int64_t max_candidates, char *bits_out, double *amps_out, double *M_out,
+                              int64_t *candidates_out) {
+  return guard([&] {
+    if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
+    hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+    sample_rejection(plan, num_samples, seed, M, fix_M, max_candidates, bits_out, amps_out, M_out, candidates_out);
   });
 }
 

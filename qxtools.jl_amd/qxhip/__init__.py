@@ -14,5 +14,5 @@ from .planning import (contraction_plan, contraction_scheme, greedy_contraction_
                        order_to_contraction_plan, sweep_contraction_plan)
 from .simulation import (Executor, amplitudes_all, amplitudes_uniform,  # noqa: F401
                          generate_simulation_files, load_simulation_files, output_params_dict,
-                         run_simulation, single_amplitude)
+                         run_outputs, run_simulation, single_amplitude)
 from .tn import TensorNetwork, TensorNetworkCircuit, convert_to_tnc  # noqa: F401

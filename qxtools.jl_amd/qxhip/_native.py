@@ -19,7 +19,7 @@ SYMBOLS = [
     "qx_ctx_create", "qx_ctx_destroy", "qx_plan_from_dsl", "qx_plan_set_tensor",
     "qx_plan_set_option", "qx_plan_finalize", "qx_plan_info", "qx_plan_key",
     "qx_amplitudes", "qx_amplitudes_vjp", "qx_plan_stats", "qx_plan_destroy", "qx_last_error",
-    "qx_dsl_inspect",
+    "qx_dsl_inspect", "qx_amplitudes_uniform", "qx_sample_rejection",
 ]
 
 
@@ -55,6 +55,9 @@ def lib() -> C.CDLL:
         L.qx_plan_destroy.argtypes = [p]
         L.qx_plan_destroy.restype = None
         L.qx_dsl_inspect.argtypes = [C.c_char_p, sz, C.c_int, i64, C.c_uint64, C.c_char_p, sz]
+        L.qx_amplitudes_uniform.argtypes = [p, i64, C.c_uint64, C.c_char_p, dp]
+        L.qx_sample_rejection.argtypes = [p, i64, C.c_uint64, C.c_double, C.c_int, i64, C.c_char_p, dp, dp,
+                                          C.POINTER(i64)]
         L.qx_last_error.argtypes = []
         L.qx_last_error.restype = C.c_char_p
         for f in SYMBOLS:
@@ -167,6 +170,33 @@ class Plan:
         _check(lib().qx_amplitudes(self._h, self._bits(bitstrings), nb, slice_begin, se,
                                    _dptr(out.view(np.float64))))
         return out if self.result_elems == 1 else out.reshape(nb, self.result_elems)
+
+    def _split(self, raw: bytes, n: int) -> list:
+        nc = self.num_chars
+        return [raw[i * nc:(i + 1) * nc].decode() for i in range(n)]
+
+    def amplitudes_uniform(self, num: int, seed: int) -> tuple:
+        """(bitstrings, amplitudes) of `num` device-generated uniform bitstrings."""
+        bits = C.create_string_buffer(max(1, num * self.num_chars))
+        out = np.zeros(num * self.result_elems, dtype=np.complex128)
+        _check(lib().qx_amplitudes_uniform(self._h, num, seed & (2 ** 64 - 1), bits, _dptr(out.view(np.float64))))
+        amps = out if self.result_elems == 1 else out.reshape(num, self.result_elems)
+        return self._split(bits.raw, num), amps
+
+    def sample_rejection(self, num_samples: int, seed: int, M: float = 1e-4, fix_M: bool = False,
+                         max_candidates: int | None = None) -> tuple:
+        """(bitstrings, amplitudes, final M, candidates consumed) of device-side
+        empirical-supremum rejection sampling; QxError if max_candidates run out."""
+        if max_candidates is None:
+            max_candidates = 1000 * max(1, num_samples) + 100000
+        bits = C.create_string_buffer(max(1, num_samples * self.num_chars))
+        amps = np.zeros(num_samples, dtype=np.complex128)
+        m_out = C.c_double(0.0)
+        used = C.c_int64(0)
+        _check(lib().qx_sample_rejection(self._h, num_samples, seed & (2 ** 64 - 1), float(M), int(bool(fix_M)),
+                                         max_candidates, bits, _dptr(amps.view(np.float64)), C.byref(m_out),
+                                         C.byref(used)))
+        return self._split(bits.raw, num_samples), amps, m_out.value, used.value
 
     def vjp(self, bitstrings, amp_cot: np.ndarray) -> dict:
         bitstrings = list(bitstrings)

@@ -68,6 +68,13 @@ class Executor:
     def vjp(self, bitstrings, amp_cot) -> dict:
         return self.plan.vjp(bitstrings, amp_cot)
 
+    def amplitudes_uniform(self, num: int, seed: int) -> tuple:
+        return self.plan.amplitudes_uniform(num, seed)
+
+    def sample_rejection(self, num_samples: int, seed: int, M: float = 1e-4, fix_M: bool = False,
+                         max_candidates: int | None = None) -> tuple:
+        return self.plan.sample_rejection(num_samples, seed, M, fix_M, max_candidates)
+
     def stats(self) -> dict:
         return self.plan.stats()
 
@@ -115,6 +122,31 @@ def run_simulation(circ: Circuit, num_amplitudes: int | None = None, seed: int |
     finally:
         ex.close()
     return OrderedDict((b, complex(a)) for b, a in zip(uniq, amps))
+
+
+def run_outputs(ex: Executor, output_args: dict):
+    """Execute the output section of a saved plan's parameters (the `output` entry
+    that `generate_parameter_file` writes, `src/simulation.jl:149-155`, built by
+    `output_params_dict`, `src/outputs.jl:47-78`) on the GPU:
+
+    * List      -> OrderedDict bitstring -> amplitude of the given bitstrings;
+    * Uniform   -> OrderedDict of `num_samples` device-generated uniform bitstrings;
+    * Rejection -> (list of sampled bitstrings, their amplitudes, final M, candidates).
+    A missing seed is 0."""
+    method = str(output_args["method"]).lstrip(":")
+    p = output_args.get("params", {})
+    seed = p.get("seed")
+    seed = 0 if seed is None else int(seed)
+    if method == "List":
+        bits = list(p["bitstrings"])
+        amps = ex.amplitudes(bits)
+        return OrderedDict((b, complex(a)) for b, a in zip(bits, np.asarray(amps).reshape(len(bits), -1)[:, 0]))
+    if method == "Uniform":
+        bits, amps = ex.amplitudes_uniform(int(p["num_samples"]), seed)
+        return OrderedDict((b, complex(a)) for b, a in zip(bits, np.asarray(amps).reshape(len(bits), -1)[:, 0]))
+    if method == "Rejection":
+        return ex.sample_rejection(int(p["num_samples"]), seed, float(p.get("M", 1e-4)), bool(p.get("fix_M", False)))
+    raise ValueError(f'Output method "{method}" not supported')
 
 
 def output_params_dict(num_qubits: int, num_outputs: int = 10, output_method: str = "List",
