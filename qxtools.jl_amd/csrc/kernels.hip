@@ -985,6 +985,152 @@ void launch_contract_radix(int prec, const void *A, const void *B, void *C, int6
 }
 
 // ---------------------------------------------------------------------------
+// Chain kernel: X_m = S_m ... S_1 X_0 in one pass.  A workgroup owns tiles of
+// 2^kChainTileBits elements (the tile bits hold the legs the chain touches and the
+// fastest untouched ones; the other bits index tiles).  Per tile: load from X_0
+// (lanes along the fastest X_0 legs), apply every S_j in LDS, store to X_m (lanes
+// along the fastest X_m legs).  Step j works on groups of 2^k elements that differ
+// only in its contracted tile bits q: out[nu] = sum_kappa S[h, kappa, nu] in[kappa],
+// the new legs taking the contracted legs' tile bits.  S_j is staged in LDS as
+// [h][nu][kappa].
+constexpr int kChainTile = 1 << kChainTileBits;
+constexpr int kChainThreads = 1024;
+constexpr int kChainPer = kChainTile / kChainThreads;   // tile elements per lane (8)
+
+template <typename T, int K>
+__device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const typename Vec<T>::type *sS,
+                                            const ChainStepArgs &s) {
+  using V = typename Vec<T>::type;
+  constexpr int LK = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : K == 16 ? 4 : 5;
+  const int t = threadIdx.x;
+  uint32_t kmask = 0;
+  for (int i = 0; i < LK; ++i) kmask |= 1u << s.q[i];
+  auto koff = [&](int kap) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < LK; ++i)
+      if ((kap >> i) & 1) o |= 1u << s.q[i];
+    return o;
+  };
+  auto deposit = [&](uint32_t g) {
+    uint32_t base = 0;
+    int j = 0;
+    for (int b = 0; b < kChainTileBits; ++b)
+      if (!((kmask >> b) & 1)) {
+        if ((g >> j) & 1) base |= 1u << b;
+        ++j;
+      }
+    return base;
+  };
+  auto hrow = [&](uint32_t base) {
+    int h = 0;
+    for (int i = 0; i < s.h; ++i) h |= (int)((base >> s.hq[i]) & 1) << i;
+    return sS + h * K * K;
+  };
+  if (K <= kChainPer) {
+    // whole groups per lane, updated in place
+    for (int i = 0; i < kChainPer / K; ++i) {
+      const uint32_t base = deposit((uint32_t)(t + kChainThreads * i));
+      const V *Sr = hrow(base);
+      V in[K];
+#pragma unroll
+      for (int kap = 0; kap < K; ++kap) in[kap] = tile[base | koff(kap)];
+#pragma unroll
+      for (int nu = 0; nu < K; ++nu) {
+        V acc{0, 0};
+#pragma unroll
+        for (int kap = 0; kap < K; ++kap) cmac<T>(acc, Sr[nu * K + kap], in[kap], 0, 0);
+        tile[base | koff(nu)] = acc;
+      }
+    }
+    __syncthreads();
+  } else {
+    // K / kChainPer lanes per group, each accumulating kChainPer outputs while it
+    // streams the group's inputs; every lane reads before any lane writes
+    constexpr int NG = kChainTile / K;
+    const uint32_t base = deposit((uint32_t)(t % NG));
+    const int nu0 = (t / NG) * kChainPer;
+    const V *Sr = hrow(base) + nu0 * K;
+    V acc[kChainPer];
+#pragma unroll
+    for (int c = 0; c < kChainPer; ++c) acc[c] = V{0, 0};
+    for (int k0 = 0; k0 < K; k0 += 8) {
+      V in[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) in[j] = tile[base | koff(k0 + j)];
+#pragma unroll
+      for (int c = 0; c < kChainPer; ++c)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) cmac<T>(acc[c], Sr[c * K + k0 + j], in[j], 0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kChainPer; ++c) tile[base | koff(nu0 + c)] = acc[c];
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename Vec<T>::type *__restrict__ X0,
+                                                                  typename Vec<T>::type *__restrict__ Xm,
+                                                                  ChainArgs a) {
+  using V = typename Vec<T>::type;
+  __shared__ V tile[kChainTile];
+  __shared__ V sS[kChainMaxS];
+  const int t = threadIdx.x;
+  const int64_t ntiles = int64_t(1) << a.nouter;
+  for (int64_t w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    int64_t bi = 0, bo = 0;
+    for (int j = 0; j < a.nouter; ++j)
+      if ((w >> j) & 1) bi += a.oin[j], bo += a.oout[j];
+    for (int r = 0; r < kChainPer; ++r) {
+      const int e = t + kChainThreads * r;
+      int64_t oi = bi;
+      for (int b = 0; b < kChainTileBits; ++b)
+        if ((e >> b) & 1) oi += a.win[b];
+      tile[e] = X0[oi];
+    }
+    for (int j = 0; j < a.nsteps; ++j) {
+      const ChainStepArgs &s = a.st[j];
+      const V *S = (const V *)a.S[j];
+      const int kk = 1 << s.k, tot = (1 << s.h) * kk * kk;
+      for (int e = t; e < tot; e += kChainThreads) {
+        const int kap = e % kk, nu = (e / kk) % kk, h = e / (kk * kk);
+        int64_t o = 0;
+        for (int i = 0; i < s.k; ++i) {
+          if ((kap >> i) & 1) o += s.sk[i];
+          if ((nu >> i) & 1) o += s.sn[i];
+        }
+        for (int i = 0; i < s.h; ++i)
+          if ((h >> i) & 1) o += s.sh[i];
+        sS[e] = S[o];
+      }
+      __syncthreads();
+      switch (s.k) {
+        case 0: chain_apply<T, 1>(tile, sS, s); break;
+        case 1: chain_apply<T, 2>(tile, sS, s); break;
+        case 2: chain_apply<T, 4>(tile, sS, s); break;
+        case 3: chain_apply<T, 8>(tile, sS, s); break;
+        case 4: chain_apply<T, 16>(tile, sS, s); break;
+        default: chain_apply<T, 32>(tile, sS, s); break;
+      }
+    }
+    for (int r = 0; r < kChainPer; ++r) {
+      const int e = t + kChainThreads * r;
+      int64_t oo = bo;
+      int so = 0;
+      for (int b = 0; b < kChainTileBits; ++b)
+        if ((e >> b) & 1) {
+          oo += a.wout[a.pout[b]];
+          so |= 1 << a.pout[b];
+        }
+      Xm[oo] = tile[so];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Output sampling on the device.  Counter-based Philox4x64-10 (Salmon, Moraes,
 // Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC 2011): candidate j
 // of a stream with seed s draws its bits from the block at counter (j, 0, 0, 0),
@@ -1182,6 +1328,19 @@ void launch_reject(const double *amps, const char *cand, int64_t nb, int nch, ui
   hipLaunchKernelGGL(k_reject, dim3(1), dim3(256), 0, (hipStream_t)stream, (const double2 *)amps, cand, nb, nch, seed,
                      j0, scale, fix_m, num_samples, state, cnt, out_bits, (double2 *)out_amps);
   check_launch("reject");
+}
+
+void launch_contract_chain(int prec, const void *X0, void *Xm, const ChainArgs &args, void *stream) {
+  const int64_t ntiles = int64_t(1) << args.nouter;
+  const int g = (int)(ntiles < 256 * 4 ? ntiles : 256 * 4);
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == QX_C128)
+    hipLaunchKernelGGL(k_contract_chain<double>, dim3(g), dim3(kChainThreads), 0, s, (const double2 *)X0,
+                       (double2 *)Xm, args);
+  else
+    hipLaunchKernelGGL(k_contract_chain<float>, dim3(g), dim3(kChainThreads), 0, s, (const float2 *)X0,
+                       (float2 *)Xm, args);
+  check_launch("contract_chain");
 }
 
 }  // namespace qx

@@ -11,7 +11,9 @@
 //   * arena offsets come from a first-fit over liveness intervals.
 #include <algorithm>
 #include <climits>
+#include <map>
 #include <numeric>
+#include <set>
 
 #include "qx_internal.hpp"
 
@@ -68,6 +70,30 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
     return {mx, cap};
   };
   for (const Step &st : L.steps) {
+    if (st.kern == Kern::Chain) {
+      const ChainArgs &a = st.chain;
+      int64_t mi = 0, mo = 0;
+      for (int i = 0; i < kChainTileBits; ++i) mi += a.win[i], mo += a.wout[i];
+      for (int j = 0; j < a.nouter; ++j) mi += a.oin[j], mo += a.oout[j];
+      auto ex = extent(st.in_a), ec = extent(st.node);
+      if (ex.first + mi >= ex.second || ec.first + mo >= ec.second || mo + 1 != st.c_elems ||
+          (int64_t(1) << (kChainTileBits + a.nouter)) != st.c_elems)
+        throw Error(QX_ERR_STATE, "internal: chain '" + p.nodes[st.node].name + "' addresses outside its operands");
+      for (int j = 0; j < a.nsteps; ++j) {
+        const ChainStepArgs &c = a.st[j];
+        int64_t ms = 0;
+        for (int i = 0; i < c.k; ++i) ms += c.sk[i] + c.sn[i];
+        for (int i = 0; i < c.h; ++i) ms += c.sh[i];
+        auto es = extent(st.chain_small[j]);
+        if (es.first + ms >= es.second || (int64_t(1) << (c.h + 2 * c.k)) > kChainMaxS)
+          throw Error(QX_ERR_STATE, "internal: chain '" + p.nodes[st.node].name + "' operand out of range");
+        for (int i = 0; i < c.k; ++i)
+          if (c.q[i] < 0 || c.q[i] >= kChainTileBits) throw Error(QX_ERR_STATE, "internal: chain tile bit");
+        for (int i = 0; i < c.h; ++i)
+          if (c.hq[i] < 0 || c.hq[i] >= kChainTileBits) throw Error(QX_ERR_STATE, "internal: chain tile bit");
+      }
+      continue;
+    }
     int64_t ma = 0, mb = 0, mc = 0;
     for (auto &g : st.cl) {
       ma += (g.dim - 1) * g.sa;
@@ -275,7 +301,7 @@ void set_kernel_args(Step &st, int precision) {
 }
 
 bool force_kernel(Step &st, int precision, int choice) {
-  if (st.kern == Kern::Radix) return false;
+  if (st.kern == Kern::Radix || st.kern == Kern::Chain) return false;
   Step v = st;
   if (!configure_kernel(v, precision, choice)) return false;
   v.bits.conj_a = v.gemm.conj_a = v.rows.conj_a = st.bits.conj_a;
@@ -287,7 +313,7 @@ bool force_kernel(Step &st, int precision, int choice) {
 std::vector<Step> kernel_candidates(const Step &st, int precision) {
   std::vector<Step> out;
   out.push_back(st);
-  if (st.kern == Kern::Radix) return out;
+  if (st.kern == Kern::Radix || st.kern == Kern::Chain) return out;
   const StepBits sb = step_bits(st);
   std::vector<int> alts;
   if (sb.nk >= 2 && sb.nn >= 2 && sb.nm >= 4) alts.insert(alts.end(), {kGemm0, kGemm1, kGemm2, kMfma});
@@ -407,7 +433,258 @@ static void fused_shapes(const Program &p, uint64_t fused, std::vector<std::vect
   }
 }
 
-Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uint64_t fused) {
+namespace {
+
+// One step seen as X' = S . X with X the large operand: its legs by X stride.
+struct ChainView {
+  bool x_is_a = true;
+  int small = -1, big = -1;                         // nodes
+  std::vector<std::pair<int64_t, int64_t>> h, m;    // (X stride, C stride); h also S stride below
+  std::vector<int64_t> h_s;                         // S stride of each h leg
+  std::vector<std::pair<int64_t, int64_t>> k;       // (X stride, S stride)
+  std::vector<std::pair<int64_t, int64_t>> n;       // (S stride, C stride)
+};
+
+int64_t value_extent(const Program &p, const Value &v) {   // logical elements of a value
+  int64_t e = 1;
+  for (int64_t d : v.dims) e *= d;
+  (void)p;
+  return e;
+}
+
+// The step as a chain element, or false: sections 0/1, every leg of dim 2, a small
+// operand of at most kChainMaxS elements, as many new legs as contracted ones.
+bool chain_view(const Program &p, const Lowered &L, const Step &st, ChainView &cv) {
+  if (st.section == 2 || st.bits.conj_a || st.bits.conj_b) return false;
+  const Value &A = L.values[st.in_a], &B = L.values[st.in_b];
+  if (A.dep || B.dep) return false;
+  const int64_t ea = value_extent(p, A), eb = value_extent(p, B);
+  cv.x_is_a = ea >= eb;
+  cv.big = cv.x_is_a ? st.in_a : st.in_b;
+  cv.small = cv.x_is_a ? st.in_b : st.in_a;
+  if (std::min(ea, eb) > kChainMaxS || std::max(ea, eb) < (int64_t(1) << 14)) return false;
+  for (auto &g : st.cl) {
+    if (g.dim != 2) return false;
+    const int64_t sx = cv.x_is_a ? g.sa : g.sb, ss = cv.x_is_a ? g.sb : g.sa;
+    if (sx && ss) {
+      cv.h.push_back({sx, g.sc});
+      cv.h_s.push_back(ss);
+    } else if (sx) {
+      cv.m.push_back({sx, g.sc});
+    } else {
+      cv.n.push_back({ss, g.sc});
+    }
+  }
+  for (auto &g : st.kl) {
+    if (g.dim != 2) return false;
+    cv.k.push_back({cv.x_is_a ? g.sa : g.sb, cv.x_is_a ? g.sb : g.sa});
+  }
+  if (cv.k.size() != cv.n.size() || cv.k.size() > 5 || cv.h.size() > 3) return false;
+  if ((int64_t(1) << (cv.h.size() + 2 * cv.k.size())) > kChainMaxS) return false;
+  return true;
+}
+
+struct ChainBuild {
+  std::vector<size_t> members;              // step indices
+  int x0 = -1, xnode = -1, section = 0;
+  std::vector<int64_t> x0_stride;           // per slot (X_0 leg)
+  std::map<int64_t, int> slot_of;           // current X stride -> slot
+  std::set<int> touched;
+  struct S {
+    int small;
+    std::vector<int> kslot, hslot;
+    std::vector<int64_t> sk, sn, sh;
+  };
+  std::vector<S> steps;
+};
+
+// Apply one step to the slot map; false if it does not fit the chain.
+bool chain_extend(ChainBuild &cb, const ChainView &cv) {
+  const int tile_touch = kChainTileBits - kChainMinPassive;
+  std::map<int64_t, int> next;
+  ChainBuild::S s;
+  s.small = cv.small;
+  std::set<int> touched = cb.touched;
+  for (auto &hm : cv.h) {
+    auto it = cb.slot_of.find(hm.first);
+    if (it == cb.slot_of.end()) return false;
+    next[hm.second] = it->second;
+    s.hslot.push_back(it->second);
+    touched.insert(it->second);
+  }
+  s.sh = cv.h_s;
+  for (auto &mm : cv.m) {
+    auto it = cb.slot_of.find(mm.first);
+    if (it == cb.slot_of.end()) return false;
+    next[mm.second] = it->second;
+  }
+  for (size_t i = 0; i < cv.k.size(); ++i) {
+    auto it = cb.slot_of.find(cv.k[i].first);
+    if (it == cb.slot_of.end()) return false;
+    s.kslot.push_back(it->second);
+    s.sk.push_back(cv.k[i].second);
+    s.sn.push_back(cv.n[i].first);
+    next[cv.n[i].second] = it->second;
+    touched.insert(it->second);
+  }
+  for (auto v : {&s.sk, &s.sn, &s.sh})
+    for (int64_t x : *v)
+      if (x <= 0 || x > 32767) return false;
+  if ((int)touched.size() > tile_touch || next.size() != cb.x0_stride.size()) return false;
+  if (cb.steps.size() >= (size_t)kChainMaxSteps) return false;
+  cb.slot_of = std::move(next);
+  cb.touched = std::move(touched);
+  cb.steps.push_back(std::move(s));
+  return true;
+}
+
+// Turn a chain of at least two steps into one Kern::Chain step.
+Step chain_step(const Program &p, const Lowered &L, const ChainBuild &cb) {
+  const Step &last = L.steps[cb.members.back()];
+  const int nslot = (int)cb.x0_stride.size();
+  std::vector<int64_t> xm_stride(nslot, -1);
+  for (auto &kv : cb.slot_of) xm_stride[kv.second] = kv.first;
+  // tile: touched slots plus the fastest untouched ones, ordered by X_0 stride
+  std::vector<int> untouched, tile(cb.touched.begin(), cb.touched.end());
+  for (int sl = 0; sl < nslot; ++sl)
+    if (!cb.touched.count(sl)) untouched.push_back(sl);
+  auto by_in = [&](int a, int b) { return cb.x0_stride[a] < cb.x0_stride[b]; };
+  std::sort(untouched.begin(), untouched.end(), by_in);
+  size_t u = 0;
+  while ((int)tile.size() < kChainTileBits) tile.push_back(untouched[u++]);
+  std::sort(tile.begin(), tile.end(), by_in);
+  std::vector<int> outer(untouched.begin() + u, untouched.end());
+  std::vector<int> pos_of(nslot, -1);
+  Step st;
+  st.kern = Kern::Chain;
+  st.node = last.node;
+  st.section = last.section;
+  st.in_a = cb.x0;
+  st.in_b = cb.steps[0].small;
+  st.c_elems = last.c_elems;
+  ChainArgs &a = st.chain;
+  for (int i = 0; i < kChainTileBits; ++i) {
+    pos_of[tile[i]] = i;
+    a.win[i] = cb.x0_stride[tile[i]];
+    a.wout[i] = xm_stride[tile[i]];
+  }
+  std::vector<int> order(kChainTileBits);
+  for (int i = 0; i < kChainTileBits; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return a.wout[x] < a.wout[y]; });
+  for (int i = 0; i < kChainTileBits; ++i) a.pout[i] = (int8_t)order[i];
+  a.nouter = (int)outer.size();
+  if (a.nouter > kMaxBits) throw Error(QX_ERR_UNSUPPORTED, "chain: too many legs");
+  for (int j = 0; j < a.nouter; ++j) a.oin[j] = cb.x0_stride[outer[j]], a.oout[j] = xm_stride[outer[j]];
+  a.nsteps = (int)cb.steps.size();
+  double flops = 0, elems = 0;
+  for (size_t j = 0; j < cb.steps.size(); ++j) {
+    const auto &s = cb.steps[j];
+    ChainStepArgs &c = a.st[j];
+    c.k = (int8_t)s.kslot.size();
+    c.h = (int8_t)s.hslot.size();
+    for (int i = 0; i < c.k; ++i) c.q[i] = (int8_t)pos_of[s.kslot[i]], c.sk[i] = (int16_t)s.sk[i], c.sn[i] = (int16_t)s.sn[i];
+    for (int i = 0; i < c.h; ++i) c.hq[i] = (int8_t)pos_of[s.hslot[i]], c.sh[i] = (int16_t)s.sh[i];
+    st.chain_small.push_back(s.small);
+    const Step &m = L.steps[cb.members[j]];
+    flops += 8.0 * m.H * m.M * m.N * m.K;
+    elems += (double)value_extent(p, L.values[s.small]);
+  }
+  elems += (double)value_extent(p, L.values[cb.x0]) + (double)last.c_elems;
+  st.chain_flops = flops;
+  st.chain_elems = elems;
+  // shape summary for reports: one pass of X with the chain's contracted sizes
+  st.H = 1;
+  st.M = last.c_elems;
+  st.N = 1;
+  st.K = 1;
+  return st;
+}
+
+// Fold runs of chain steps (X_j = S_j . X_{j-1}, each intermediate used once) into
+// Kern::Chain steps placed where the run's last step was.  Returns the nodes whose
+// values are no longer materialised.
+std::vector<int> form_chains(const Program &p, Lowered &L) {
+  const size_t nn = p.nodes.size();
+  std::vector<int> uses(nn, 0);
+  for (const Node &n : p.nodes) {
+    if (n.op == Op::Ncon) uses[n.ia]++, uses[n.ib]++;
+    if (n.op == Op::View || n.op == Op::Save) uses[n.ia]++;
+  }
+  std::vector<Step> out;
+  std::vector<int> absorbed;
+  std::vector<char> drop(L.steps.size(), 0);
+  std::vector<std::pair<size_t, Step>> place;   // step index of a run's last member -> chain step
+  ChainBuild cb;
+  auto close = [&]() {
+    if (cb.members.size() >= 2) {
+      for (size_t j = 0; j + 1 < cb.members.size(); ++j) {
+        drop[cb.members[j]] = 1;
+        absorbed.push_back(L.steps[cb.members[j]].node);
+      }
+      place.push_back({cb.members.back(), chain_step(p, L, cb)});
+    }
+    cb = ChainBuild();
+  };
+  auto start = [&](size_t s, const ChainView &cv) {
+    cb = ChainBuild();
+    const Value &X = L.values[cv.big];
+    std::vector<int64_t> strides;
+    for (size_t i = 0; i < X.dims.size(); ++i)
+      if (X.dims[i] == 2) strides.push_back(X.strides[i]);
+      else if (X.dims[i] != 1) return;
+    for (size_t i = 0; i < strides.size(); ++i) {
+      if (cb.slot_of.count(strides[i])) return;   // aliased legs
+      cb.slot_of[strides[i]] = (int)i;
+    }
+    cb.x0_stride = strides;
+    cb.x0 = cv.big;
+    cb.section = L.steps[s].section;
+    if (!chain_extend(cb, cv)) {
+      cb = ChainBuild();
+      return;
+    }
+    cb.members.push_back(s);
+    cb.xnode = L.steps[s].node;
+  };
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    const Step &st = L.steps[s];
+    ChainView cv;
+    const bool ok = chain_view(p, L, st, cv);
+    const bool touches = cb.xnode >= 0 && (st.in_a == cb.xnode || st.in_b == cb.xnode);
+    if (touches) {
+      ChainBuild keep = cb;
+      if (ok && cv.big == cb.xnode && uses[cb.xnode] == 1 && st.section == cb.section && cb.xnode != p.save &&
+          chain_extend(cb, cv)) {
+        cb.members.push_back(s);
+        cb.xnode = st.node;
+        continue;
+      }
+      cb = keep;
+      close();
+    }
+    if (ok && cb.xnode < 0) start(s, cv);
+    else if (ok && !touches) {
+      // another chain could start here; keep the open one (its steps are interleaved)
+    }
+  }
+  close();
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    if (drop[s]) continue;
+    bool replaced = false;
+    for (auto &pl : place)
+      if (pl.first == s) {
+        out.push_back(pl.second);
+        replaced = true;
+      }
+    if (!replaced) out.push_back(std::move(L.steps[s]));
+  }
+  L.steps = std::move(out);
+  return absorbed;
+}
+
+}  // namespace
+
+Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uint64_t fused, bool chains) {
   if (!pow2(batch)) throw Error(QX_ERR_ARGUMENT, "batch must be a power of two");
   Lowered L;
   L.batch = batch;
@@ -490,6 +767,10 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
     L.steps.push_back(std::move(st));
   }
 
+  std::vector<char> absorbed(nn, 0);
+  if (chains && !keep_all)
+    for (int a : form_chains(p, L)) absorbed[a] = 1;
+
   // result elements per bitstring
   L.result_elems = 1;
   for (int64_t d : shape[p.save]) L.result_elems *= d;
@@ -535,7 +816,9 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
     sec_of[o] = 2;
   }
   for (size_t s = 0; s < L.steps.size(); ++s) {
-    for (int in : {L.steps[s].in_a, L.steps[s].in_b}) {
+    std::vector<int> ins = {L.steps[s].in_a, L.steps[s].in_b};
+    ins.insert(ins.end(), L.steps[s].chain_small.begin(), L.steps[s].chain_small.end());
+    for (int in : ins) {
       int r = root[in];
       if (r < 0) continue;
       int64_t e = pos_of_step[s] + 1;
@@ -551,7 +834,7 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
   std::vector<int> arena_nodes;
   std::vector<int64_t> S, E, Z, off;
   for (size_t i = 0; i < nn; ++i)
-    if (root[i] == (int)i) {
+    if (root[i] == (int)i && !absorbed[i]) {
       arena_nodes.push_back((int)i);
       S.push_back(st_pos[i]);
       E.push_back(std::max(en_pos[i], st_pos[i] + 1));

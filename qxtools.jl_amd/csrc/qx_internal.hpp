@@ -108,6 +108,31 @@ struct RadixArgs {
   int conj_a = 0, conj_b = 0;
 };
 
+// A chain of steps that each contract a small operand (a gate block: at most 2^10
+// elements) into one large tensor X, X_j = S_j . X_{j-1}, run as one pass over X:
+// every workgroup loads a tile of 2^kChainTileBits elements of X_0 (the bits of
+// the legs the chain touches plus the fastest untouched legs), applies every S_j
+// to it in LDS and stores the tile of X_m.  The intermediates never reach memory.
+constexpr int kChainTileBits = 13;
+constexpr int kChainMaxSteps = 16;
+constexpr int kChainMinPassive = 3;   // untouched fast legs kept in the tile
+constexpr int kChainMaxS = 1024;      // elements of one staged small operand
+
+struct ChainStepArgs {
+  int8_t k = 0, h = 0;          // contracted legs (each replaced by a new leg), batch legs
+  int8_t q[5] = {0}, hq[3] = {0};   // tile bit of each contracted / batch leg
+  int16_t sk[5] = {0}, sn[5] = {0}, sh[3] = {0};   // strides in S of contracted, new, batch legs
+};
+
+struct ChainArgs {
+  int nsteps = 0, nouter = 0;
+  int64_t win[kChainTileBits] = {0}, wout[kChainTileBits] = {0};   // tile bit -> X_0 / X_m stride
+  int8_t pout[kChainTileBits] = {0};   // store order: i-th fastest X_m tile bit -> tile bit
+  int64_t oin[kMaxBits] = {0}, oout[kMaxBits] = {0};   // outer bits (workgroup index) -> strides
+  ChainStepArgs st[kChainMaxSteps];
+  const void *S[kChainMaxSteps] = {nullptr};   // small operands, set at launch
+};
+
 // C[h, m, n] = sum_k A[h, m, k] B[h, k, n], every index mapped through bit
 // weights (all dims powers of two).  m bits are listed in increasing C weight,
 // likewise n bits; k bits in increasing A weight.
@@ -130,7 +155,7 @@ struct Leg {
   int64_t sa, sb, sc;
 };
 
-enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows };
+enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows, Chain };
 
 struct Step {
   int node = -1;           // ncon node
@@ -146,6 +171,9 @@ struct Step {
   GemmArgs gemm;           // Kern::Gemm
   RowArgs rows;            // Kern::Rows
   RadixArgs radix;         // Kern::Radix
+  ChainArgs chain;         // Kern::Chain: in_a = X_0, node = X_m
+  std::vector<int> chain_small;   // Kern::Chain: node of each S_j
+  double chain_flops = 0, chain_elems = 0;   // Kern::Chain: work and compulsory elements moved
 };
 
 struct ArenaSlot {
@@ -168,7 +196,7 @@ struct Lowered {
 };
 
 Lowered lower(const Program &p, int64_t batch, bool keep_all = false, int precision = QX_C128,
-              uint64_t fused = 0);
+              uint64_t fused = 0, bool chains = true);
 
 // C[clab] = sum A[alab] B[blab] over the given value layouts (C dense), kernel
 // chosen and arguments filled; conj flags conjugate an operand on load.
@@ -245,6 +273,7 @@ void launch_add_mapped(int prec, int dst_double, void *dst, const void *src, int
 // cotangent of the saved tensor from amp_cot rows [row0, row0 + nvalid)
 void launch_root_cot(int prec, void *dst, int dep, int64_t batch, int64_t nvalid, int64_t relems,
                      const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream);
+void launch_contract_chain(int prec, const void *X0, void *Xm, const ChainArgs &args, void *stream);
 // Output sampling: candidate bitstrings from Philox4x64-10 and the sequential
 // empirical-supremum rejection test over one round of candidates.
 void launch_gen_bits(char *bits, int64_t nb, int nch, uint64_t seed, int64_t j0, void *stream);

@@ -87,6 +87,7 @@ struct qx_plan {
   int64_t opt_profile = 0;
   int64_t opt_autotune = 0;
   int64_t opt_kernel = -1;                  // >= 0: force that kernel wherever it can run
+  int64_t opt_chains = 1;                   // fold runs of gate-block steps into chain passes
   bool finalized = false;
 
   std::vector<DevBuf> data;                 // per key
@@ -139,7 +140,7 @@ const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch, uint64_t fused = 0)
   auto key = std::make_pair(batch, fused);
   auto it = P->lowered.find(key);
   if (it != P->lowered.end()) return it->second;
-  qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, fused);
+  qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, fused, P->opt_chains != 0);
   if (P->opt_kernel >= 0)
     for (auto &st : L.steps)
       if (qx::force_kernel(st, P->prec, (int)P->opt_kernel)) L.ws_elems = std::max(L.ws_elems, st.ws_elems);
@@ -182,9 +183,19 @@ void *value_ptr(qx_plan *P, const qx::Value &v, const std::vector<int64_t> &varv
   return value_ptr(P, v, varval, b);
 }
 
-void launch_step(qx_plan *P, const qx::Step &st, void *A, void *B, void *C) {
+void launch_step(qx_plan *P, const qx::Lowered &L, const qx::Step &st, void *A, void *B, void *C,
+                 const std::vector<int64_t> &varval, const Bases *bs = nullptr) {
   hipStream_t stream = P->ctx->stream;
   switch (st.kern) {
+    case qx::Kern::Chain: {
+      qx::ChainArgs a = st.chain;
+      for (int j = 0; j < a.nsteps; ++j) {
+        const qx::Value &v = L.values[st.chain_small[j]];
+        a.S[j] = bs ? value_ptr(P, v, varval, *bs) : value_ptr(P, v, varval);
+      }
+      qx::launch_contract_chain(P->prec, A, C, a, stream);
+      break;
+    }
     case qx::Kern::Radix: qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, st.radix, stream); break;
     case qx::Kern::Gemm: qx::launch_contract_gemm(P->prec, A, B, C, P->ws.p, st.gemm, stream); break;
     case qx::Kern::Reduce: qx::launch_contract_reduce(P->prec, A, B, C, P->ws.p, st.c_elems, st.bits, stream); break;
@@ -210,7 +221,7 @@ void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int6
     P->event_step.push_back((int)s);
     hip_check(hipEventRecord(P->events[2 * ev], stream), "hipEventRecord");
   }
-  launch_step(P, st, A, B, C);
+  launch_step(P, L, st, A, B, C, varval);
   if (P->opt_profile) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
 }
 
@@ -234,12 +245,12 @@ void tune_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int
   void *C = value_ptr(P, L.values[st.node], varval);
   const auto &cs = T.cands[s];
   if (cs.size() < 2 || T.seen[s] >= Tuner::kTimedLaunches) {
-    launch_step(P, st, A, B, C);
+    launch_step(P, L, st, A, B, C, varval);
     return;
   }
   hipStream_t stream = P->ctx->stream;
   if (T.seen[s] == 0)   // first launch of every candidate untimed: code loading, cold caches
-    for (auto &c : cs) launch_step(P, c, A, B, C);
+    for (auto &c : cs) launch_step(P, L, c, A, B, C, varval);
   ++T.seen[s];
   for (size_t c = 0; c < cs.size(); ++c) {
     size_t ev = T.rec.size();
@@ -250,7 +261,7 @@ void tune_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int
     }
     T.rec.push_back({(int)s, (int)c, (int)ev});
     hip_check(hipEventRecord(P->tune_events[2 * ev], stream), "hipEventRecord");
-    launch_step(P, cs[c], A, B, C);
+    launch_step(P, L, cs[c], A, B, C, varval);
     hip_check(hipEventRecord(P->tune_events[2 * ev + 1], stream), "hipEventRecord");
   }
 }
@@ -320,7 +331,7 @@ uint64_t choose_fused(qx_plan *P, int64_t batch, bool for_vjp) {
         qx::VjpLowered V = qx::lower_vjp(P->prog, batch, P->prec, mask);
         need = (V.fwd.arena_elems + V.fwd.ws_elems + V.cot_elems + V.temp_elems) * es;
       } else {
-        qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, mask);
+        qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, mask, P->opt_chains != 0);
         need = (L.arena_elems + L.ws_elems) * es;
       }
       if (need <= budget) {
@@ -593,6 +604,7 @@ const char *kern_name(qx::Kern k) {
     case qx::Kern::Gemm: return "gemm";
     case qx::Kern::Reduce: return "reduce";
     case qx::Kern::Rows: return "rows";
+    case qx::Kern::Chain: return "chain";
     default: return "bits";
   }
 }
@@ -611,6 +623,7 @@ std::string kern_variant(const qx::Step &st) {
                                                                                    : "k_contract_gemm<64,64,16>";
     case qx::Kern::Reduce: return "k_contract_reduce";
     case qx::Kern::Radix: return "k_contract_radix";
+    case qx::Kern::Chain: return "k_contract_chain";
     default: return "k_contract_bits";
   }
 }
@@ -635,10 +648,23 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
     if (s) os << ",";
     double flops = 8.0 * st.H * st.M * st.N * st.K;
     double bytes = (double)es * st.H * (st.M * st.K + st.K * st.N + st.M * st.N);
-    os << "{\"name\":\"" << prog.nodes[st.node].name << "\",\"section\":" << st.section << ",\"B\":" << st.H
+    if (st.kern == qx::Kern::Chain) flops = st.chain_flops, bytes = (double)es * st.chain_elems;
+    os << "{\"name\":\"" << prog.nodes[st.node].name << "\",\"section\":" << st.section
+       << ",\"fused_steps\":" << (st.kern == qx::Kern::Chain ? st.chain.nsteps : 1) << ",\"B\":" << st.H
        << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K << ",\"flops\":" << flops
        << ",\"bytes\":" << bytes << ",\"kernel\":\"" << kern_name(st.kern) << "\",\"variant\":\""
-       << kern_variant(st) << "\",\"ws_elems\":" << st.ws_elems << ",\"candidates\":[";
+       << kern_variant(st) << "\",\"ws_elems\":" << st.ws_elems;
+    if (with_slots) {   // leg strides (elements) in A, B and C: free legs in C order, then contracted legs
+      os << ",\"legs\":[";
+      for (size_t j = 0; j < st.cl.size(); ++j)
+        os << (j ? "," : "") << "[" << st.cl[j].dim << "," << st.cl[j].sa << "," << st.cl[j].sb << ","
+           << st.cl[j].sc << "]";
+      os << "],\"klegs\":[";
+      for (size_t j = 0; j < st.kl.size(); ++j)
+        os << (j ? "," : "") << "[" << st.kl[j].dim << "," << st.kl[j].sa << "," << st.kl[j].sb << "]";
+      os << "]";
+    }
+    os << ",\"candidates\":[";
     auto cands = qx::kernel_candidates(st, es == 16 ? QX_C128 : QX_C64_4M);
     for (size_t c = 0; c < cands.size(); ++c)
       os << (c ? "," : "") << "\"" << kern_variant(cands[c]) << "\"";
@@ -747,8 +773,9 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
   auto run_fwd = [&](int sec) {
     for (const qx::Step &st : L.steps)
       if (st.section == sec)
-        launch_step(P, st, value_ptr(P, L.values[st.in_a], varval, bs), value_ptr(P, L.values[st.in_b], varval, bs),
-                    value_ptr(P, L.values[st.node], varval, bs));
+        launch_step(P, L, st, value_ptr(P, L.values[st.in_a], varval, bs),
+                    value_ptr(P, L.values[st.in_b], varval, bs), value_ptr(P, L.values[st.node], varval, bs), varval,
+                    &bs);
   };
   auto run_bwd = [&](int sec) {
     for (const qx::BwdStep &b : V.bsteps[sec]) {
@@ -758,7 +785,7 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
       void *A = xa ? value_ptr(P, cv, varval, bs) : value_ptr(P, L.values[cn.ia], varval, bs);
       void *B = xa ? value_ptr(P, L.values[cn.ib], varval, bs) : value_ptr(P, cv, varval, bs);
       void *C = b.direct ? value_ptr(P, V.cot[b.x], varval, bs) : (void *)bs.temp;
-      launch_step(P, b.step, A, B, C);
+      launch_step(P, L, b.step, A, B, C, varval, &bs);
       if (b.direct) continue;
       const int64_t xoff = value_offset(L.values[b.x], varval);
       if (b.dest == qx::Buf::Grad) {
@@ -813,7 +840,9 @@ std::string stats_json(const qx_plan *P) {
       if (s) os << ",";
       double flops = 8.0 * st.H * st.M * st.N * st.K;
       double bytes = (double)P->esize() * st.H * (st.M * st.K + st.K * st.N + st.M * st.N);
+      if (st.kern == qx::Kern::Chain) flops = st.chain_flops, bytes = (double)P->esize() * st.chain_elems;
       os << "{\"name\":\"" << P->prog.nodes[st.node].name << "\",\"section\":" << st.section
+         << ",\"fused_steps\":" << (st.kern == qx::Kern::Chain ? st.chain.nsteps : 1)
          << ",\"B\":" << st.H << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K
          << ",\"flops\":" << flops << ",\"bytes\":" << bytes << ",\"kernel\":\""
          << kern_name(st.kern) << "\",\"variant\":\"" << kern_variant(st) << "\"";
@@ -904,6 +933,8 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
       plan->opt_fuse = value;
     } else if (n == "autotune") {
       plan->opt_autotune = value;
+    } else if (n == "chains") {
+      plan->opt_chains = value;
     } else if (n == "kernel") {
       if (value < -1 || value > 6) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..6");
       plan->opt_kernel = value;

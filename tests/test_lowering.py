@@ -162,3 +162,16 @@ def test_autotune_candidates(name):
             assert {"k_contract_gemm<128,16,16>", "k_contract_gemm<128,32,16>", "k_contract_gemm<64,64,16>"} <= set(c)
         if prec == "c128":
             assert "k_contract_gemm_mfma" not in c
+
+
+def test_chains_fold_gate_block_runs():
+    """The sweep plan of the 5x5 depth-20 workload applies gate blocks to a 2^25
+    state; runs of them fold into chain passes (fewer passes over the state)."""
+    dsl, data, bits = workloads.WORKLOADS["rqc_5x5_d20"].load()
+    info = _native.inspect(dsl, batch=256)
+    chains = [s for s in info["steps"] if s["kernel"] == "chain"]
+    assert len(chains) >= 10 and all(s["fused_steps"] >= 2 for s in chains)
+    big = [s for s in info["steps"] if s["bytes"] > 1e8]
+    assert len(big) <= 32                      # 72 passes without chains
+    for s in chains:
+        assert s["candidates"] == ["k_contract_chain"]

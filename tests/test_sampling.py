@@ -129,6 +129,6 @@ def test_device_rejection_distribution_and_limits():
     with pytest.raises(QxError):                       # runs out of candidates
         ex.sample_rejection(100, 5, 1e6, True, 50)
     out = simulation.run_outputs(ex, simulation.output_params_dict(4, 7, "Uniform", seed=3))
-    assert list(out) == S.uniform_bitstrings(4, 7, 3)
+    assert list(out) == list(dict.fromkeys(S.uniform_bitstrings(4, 7, 3)))   # repeats collapse
     lst = simulation.run_outputs(ex, simulation.output_params_dict(4, bitstrings=["0101", "1111"]))
     assert np.allclose([lst["0101"], lst["1111"]], O.sv_amplitudes(psi, ["0101", "1111"]), atol=1e-14)
