@@ -10,6 +10,8 @@
 // every bitstring-dependent tensor, guarantees it for batched steps).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <vector>
 
 #include "qx_internal.hpp"
@@ -993,18 +995,128 @@ void launch_contract_radix(int prec, const void *A, const void *B, void *C, int6
 // only in its contracted tile bits q: out[nu] = sum_kappa S[h, kappa, nu] in[kappa],
 // the new legs taking the contracted legs' tile bits.  S_j is staged in LDS as
 // [h][nu][kappa].
-constexpr int kChainTile = 1 << kChainTileBits;
-constexpr int kChainThreads = 1024;
-constexpr int kChainPer = kChainTile / kChainThreads;   // tile elements per lane (8)
+// Calls f(std::integral_constant<int, i>) for i = 0 .. N-1 (indices stay compile-time
+// constants so per-lane arrays stay in registers).
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (N > 0) {
+    static_for<N - 1>(f);
+    f(std::integral_constant<int, N - 1>{});
+  }
+}
 
+constexpr int kChainTile = 1 << kChainTileBits;
+constexpr int kChainThreads = 512;
+constexpr int kChainPer = kChainTile / kChainThreads;   // tile elements per lane (16)
+
+// Workgroup barrier that orders LDS only: outstanding global loads (the next
+// tile's prefetch) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// 16x16x4 matrix-core products.  Both take A[i][k] from lane i + 16 k and B[k][j]
+// from lane j + 16 k; lane l holds D[row(l / 16, r)][l % 16] in element r, with
+// row = 4 (l / 16) + r for FP32 and 4 r + l / 16 for FP64.
+template <typename T> struct Mfma16;
+template <> struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * r + lk; }
+};
+template <> struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
+// Step j's S, packed densely as [h][nu][kappa] at Sp + a.st[j].soff, is read with
+// wave-uniform indices (the batch bits h are placed on wave-uniform group bits), so
+// its values come through the scalar cache rather than LDS.
 template <typename T, int K>
-__device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const typename Vec<T>::type *sS,
+__device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const typename Vec<T>::type *__restrict__ Sp,
                                             const ChainStepArgs &s) {
   using V = typename Vec<T>::type;
   constexpr int LK = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : K == 16 ? 4 : 5;
   const int t = threadIdx.x;
-  uint32_t kmask = 0;
+  uint32_t kmask = 0, hmask = 0;
   for (int i = 0; i < LK; ++i) kmask |= 1u << s.q[i];
+  for (int i = 0; i < s.h; ++i) hmask |= 1u << s.hq[i];
+  auto koff = [&](int kap) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < LK; ++i)
+      if ((kap >> i) & 1) o |= 1u << s.q[i];
+    return o;
+  };
+  // group index bits -> tile bits outside q: other positions first, batch bits last
+  auto deposit = [&](uint32_t g) {
+    uint32_t base = 0;
+    int j = 0;
+    for (int b = 0; b < kChainTileBits; ++b)
+      if (!(((kmask | hmask) >> b) & 1)) {
+        if ((g >> j) & 1) base |= 1u << b;
+        ++j;
+      }
+    for (int i = 0; i < s.h; ++i, ++j)
+      if ((g >> j) & 1) base |= 1u << s.hq[i];
+    return base;
+  };
+  auto srow = [&](uint32_t base) {
+    int h = 0;
+    for (int i = 0; i < s.h; ++i) h |= (int)((base >> s.hq[i]) & 1) << i;
+    h = __builtin_amdgcn_readfirstlane(h);
+    return Sp + s.soff + h * K * K;
+  };
+  static_assert(K <= kChainPer, "larger steps run on matrix cores");
+  // whole groups per lane, updated in place
+  for (int i = 0; i < kChainPer / K; ++i) {
+    const uint32_t base = deposit((uint32_t)(t + kChainThreads * i));
+    const V *Sr = srow(base);
+    V in[K];
+#pragma unroll
+    for (int kap = 0; kap < K; ++kap) in[kap] = tile[base | koff(kap)];
+#pragma unroll
+    for (int nu = 0; nu < K; ++nu) {
+      V acc{0, 0};
+#pragma unroll
+      for (int kap = 0; kap < K; ++kap) cmac<T>(acc, Sr[nu * K + kap], in[kap], 0, 0);
+      tile[base | koff(nu)] = acc;
+    }
+  }
+  lds_barrier();
+}
+
+// Steps with 16 or 32 contracted values on matrix cores: out[g][nu] = sum_kappa
+// in[g][kappa] S[kappa -> nu] as 16x16x4 blocks (rows = groups, columns = nu), the
+// complex product as four real products.  Wave w owns four row blocks of one nu
+// block; its 64 groups share the batch bits, so its S fragments (B operands: S at
+// nu = lane % 16, kappa = 4 kb + lane / 16) stay in registers for the whole step.
+// Operand and result layouts as in Mfma16.
+template <typename T, int K>
+__device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
+                                                 const typename Vec<T>::type *__restrict__ Sp,
+                                                 const ChainStepArgs &s) {
+  using V = typename Vec<T>::type;
+  using M = Mfma16<T>;
+  using v4 = typename M::v4;
+  constexpr int LK = K == 16 ? 4 : 5;
+  constexpr int NG = kChainTile / K, NB = K / 16, KB = K / 4;
+  constexpr int WAVES = kChainThreads / 64, RT = (NG / 16) * NB / WAVES;
+  static_assert(RT == 4, "row blocks per wave");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nb = wave % NB;
+  const int g0 = (wave / NB) * RT * 16;
+  uint32_t kmask = 0, hmask = 0;
+  for (int i = 0; i < LK; ++i) kmask |= 1u << s.q[i];
+  for (int i = 0; i < s.h; ++i) hmask |= 1u << s.hq[i];
   auto koff = [&](int kap) {
     uint32_t o = 0;
 #pragma unroll
@@ -1016,117 +1128,198 @@ __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const t
     uint32_t base = 0;
     int j = 0;
     for (int b = 0; b < kChainTileBits; ++b)
-      if (!((kmask >> b) & 1)) {
+      if (!(((kmask | hmask) >> b) & 1)) {
         if ((g >> j) & 1) base |= 1u << b;
         ++j;
       }
+    for (int i = 0; i < s.h; ++i, ++j)
+      if ((g >> j) & 1) base |= 1u << s.hq[i];
     return base;
   };
-  auto hrow = [&](uint32_t base) {
-    int h = 0;
-    for (int i = 0; i < s.h; ++i) h |= (int)((base >> s.hq[i]) & 1) << i;
-    return sS + h * K * K;
-  };
-  if (K <= kChainPer) {
-    // whole groups per lane, updated in place
-    for (int i = 0; i < kChainPer / K; ++i) {
-      const uint32_t base = deposit((uint32_t)(t + kChainThreads * i));
-      const V *Sr = hrow(base);
-      V in[K];
+  int h = 0;
+  {
+    const uint32_t b0 = deposit((uint32_t)g0);
+    for (int i = 0; i < s.h; ++i) h |= (int)((b0 >> s.hq[i]) & 1) << i;
+  }
+  const V *Sr = Sp + s.soff + h * K * K + (16 * nb + li) * K + lk;
+  T bre[KB], bim[KB];
 #pragma unroll
-      for (int kap = 0; kap < K; ++kap) in[kap] = tile[base | koff(kap)];
+  for (int kb = 0; kb < KB; ++kb) {
+    const V v = Sr[4 * kb];
+    bre[kb] = v.x;
+    bim[kb] = v.y;
+  }
+  v4 cre[RT], cim[RT];
 #pragma unroll
-      for (int nu = 0; nu < K; ++nu) {
-        V acc{0, 0};
+  for (int rt = 0; rt < RT; ++rt) {
+    const uint32_t base = deposit((uint32_t)(g0 + 16 * rt + li));
+    v4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
 #pragma unroll
-        for (int kap = 0; kap < K; ++kap) cmac<T>(acc, Sr[nu * K + kap], in[kap], 0, 0);
-        tile[base | koff(nu)] = acc;
+    for (int kb = 0; kb < KB; ++kb) {
+      const V a = tile[base | koff(4 * kb + lk)];
+      xr = M::mma(a.x, bre[kb], xr);
+      xr = M::mma(-a.y, bim[kb], xr);
+      xi = M::mma(a.x, bim[kb], xi);
+      xi = M::mma(a.y, bre[kb], xi);
+    }
+    cre[rt] = xr;
+    cim[rt] = xi;
+  }
+  // waves of other nu blocks read these groups: all reads before any write
+  lds_barrier();
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t base = deposit((uint32_t)(g0 + 16 * rt + M::row(lk, r)));
+      V o;
+      o.x = cre[rt][r];
+      o.y = cim[rt][r];
+      tile[base | koff(16 * nb + li)] = o;
+    }
+  lds_barrier();
+}
+
+// Gather every S_j of a chain into the dense [h][nu][kappa] blocks the chain
+// kernel reads.
+template <typename T>
+__global__ void k_chain_pack(typename Vec<T>::type *__restrict__ Sp, ChainArgs a) {
+  using V = typename Vec<T>::type;
+  for (int j = 0; j < a.nsteps; ++j) {
+    const ChainStepArgs &s = a.st[j];
+    const V *S = (const V *)a.S[j];
+    const int kk = 1 << s.k, tot = (1 << s.h) * kk * kk;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += gridDim.x * blockDim.x) {
+      const int kap = e % kk, nu = (e / kk) % kk, h = e / (kk * kk);
+      int64_t o = 0;
+      for (int i = 0; i < s.k; ++i) {
+        if ((kap >> i) & 1) o += s.sk[i];
+        if ((nu >> i) & 1) o += s.sn[i];
       }
+      for (int i = 0; i < s.h; ++i)
+        if ((h >> i) & 1) o += s.sh[i];
+      Sp[s.soff + e] = S[o];
     }
-    __syncthreads();
-  } else {
-    // K / kChainPer lanes per group, each accumulating kChainPer outputs while it
-    // streams the group's inputs; every lane reads before any lane writes
-    constexpr int NG = kChainTile / K;
-    const uint32_t base = deposit((uint32_t)(t % NG));
-    const int nu0 = (t / NG) * kChainPer;
-    const V *Sr = hrow(base) + nu0 * K;
-    V acc[kChainPer];
-#pragma unroll
-    for (int c = 0; c < kChainPer; ++c) acc[c] = V{0, 0};
-    for (int k0 = 0; k0 < K; k0 += 8) {
-      V in[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) in[j] = tile[base | koff(k0 + j)];
-#pragma unroll
-      for (int c = 0; c < kChainPer; ++c)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) cmac<T>(acc[c], Sr[c * K + k0 + j], in[j], 0, 0);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < kChainPer; ++c) tile[base | koff(nu0 + c)] = acc[c];
-    __syncthreads();
   }
 }
 
 template <typename T>
 __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename Vec<T>::type *__restrict__ X0,
                                                                   typename Vec<T>::type *__restrict__ Xm,
+                                                                  const typename Vec<T>::type *__restrict__ Sp,
                                                                   ChainArgs a) {
   using V = typename Vec<T>::type;
   __shared__ V tile[kChainTile];
-  __shared__ V sS[kChainMaxS];
   const int t = threadIdx.x;
   const int64_t ntiles = int64_t(1) << a.nouter;
-  for (int64_t w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    int64_t bi = 0, bo = 0;
+  auto in_base = [&](int64_t w) {
+    int64_t bi = 0;
     for (int j = 0; j < a.nouter; ++j)
-      if ((w >> j) & 1) bi += a.oin[j], bo += a.oout[j];
-    for (int r = 0; r < kChainPer; ++r) {
-      const int e = t + kChainThreads * r;
-      int64_t oi = bi;
-      for (int b = 0; b < kChainTileBits; ++b)
-        if ((e >> b) & 1) oi += a.win[b];
-      tile[e] = X0[oi];
+      if ((w >> j) & 1) bi += a.oin[j];
+    return bi;
+  };
+  // lane t holds tile elements t + kChainThreads * r: the low tile bits come from t,
+  // the top four from r
+  constexpr int RB = 4;
+  static_assert(kChainPer == 1 << RB && kChainThreads == 1 << (kChainTileBits - RB), "tile split");
+  int64_t lin = 0, lout = 0;
+  int sout = 0;
+  for (int b = 0; b < kChainTileBits - RB; ++b)
+    if ((t >> b) & 1) {
+      lin += a.win[b];
+      lout += a.wout[a.pout[b]];
+      sout |= 1 << a.pout[b];
     }
+  // offsets of element r of this lane: the top RB tile bits come from r (a
+  // compile-time constant, so these fold to sums of kernel arguments)
+  auto rin = [&](auto r) {
+    int64_t o = lin;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      if ((r >> i) & 1) o += a.win[kChainTileBits - RB + i];
+    return o;
+  };
+  auto rout = [&](auto r) {
+    int64_t o = lout;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      if ((r >> i) & 1) o += a.wout[a.pout[kChainTileBits - RB + i]];
+    return o;
+  };
+  auto rso = [&](auto r) {
+    int o = sout;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      if ((r >> i) & 1) o |= 1 << a.pout[kChainTileBits - RB + i];
+    return o;
+  };
+  // prefetch registers as native two-element vectors (a struct copy would keep them
+  // in private memory)
+  typedef T E __attribute__((ext_vector_type(2)));
+  const E *X0e = reinterpret_cast<const E *>(X0);
+  E *tile_e = reinterpret_cast<E *>(tile);
+  E pf[kChainPer];
+  int64_t w = blockIdx.x;
+  if (w < ntiles) {
+    const int64_t bi = in_base(w);
+    static_for<kChainPer>([&](auto r) { pf[r] = X0e[bi + rin(r)]; });
+  }
+  for (; w < ntiles; w += gridDim.x) {
+    static_for<kChainPer>([&](auto r) { tile_e[t + kChainThreads * r] = pf[r]; });
+    // the next tile's loads overlap this tile's steps
+    if (w + gridDim.x < ntiles) {
+      const int64_t bi = in_base(w + gridDim.x);
+      static_for<kChainPer>([&](auto r) { pf[r] = X0e[bi + rin(r)]; });
+    }
+    lds_barrier();
     for (int j = 0; j < a.nsteps; ++j) {
       const ChainStepArgs &s = a.st[j];
-      const V *S = (const V *)a.S[j];
-      const int kk = 1 << s.k, tot = (1 << s.h) * kk * kk;
-      for (int e = t; e < tot; e += kChainThreads) {
-        const int kap = e % kk, nu = (e / kk) % kk, h = e / (kk * kk);
-        int64_t o = 0;
-        for (int i = 0; i < s.k; ++i) {
-          if ((kap >> i) & 1) o += s.sk[i];
-          if ((nu >> i) & 1) o += s.sn[i];
-        }
-        for (int i = 0; i < s.h; ++i)
-          if ((h >> i) & 1) o += s.sh[i];
-        sS[e] = S[o];
-      }
-      __syncthreads();
       switch (s.k) {
-        case 0: chain_apply<T, 1>(tile, sS, s); break;
-        case 1: chain_apply<T, 2>(tile, sS, s); break;
-        case 2: chain_apply<T, 4>(tile, sS, s); break;
-        case 3: chain_apply<T, 8>(tile, sS, s); break;
-        case 4: chain_apply<T, 16>(tile, sS, s); break;
-        default: chain_apply<T, 32>(tile, sS, s); break;
+        case 0: chain_apply<T, 1>(tile, Sp, s); break;
+        case 1: chain_apply<T, 2>(tile, Sp, s); break;
+        case 2: chain_apply<T, 4>(tile, Sp, s); break;
+        case 3: chain_apply<T, 8>(tile, Sp, s); break;
+        case 4:
+          if (a.mfma16)
+            chain_apply_mfma<T, 16>(tile, Sp, s);
+          else
+            chain_apply<T, 16>(tile, Sp, s);
+          break;
+        default: chain_apply_mfma<T, 32>(tile, Sp, s); break;
       }
     }
-    for (int r = 0; r < kChainPer; ++r) {
-      const int e = t + kChainThreads * r;
-      int64_t oo = bo;
-      int so = 0;
-      for (int b = 0; b < kChainTileBits; ++b)
-        if ((e >> b) & 1) {
-          oo += a.wout[a.pout[b]];
-          so |= 1 << a.pout[b];
-        }
-      Xm[oo] = tile[so];
+    int64_t bo = 0;
+    for (int j = 0; j < a.nouter; ++j)
+      if ((w >> j) & 1) bo += a.oout[j];
+    E *Xme = reinterpret_cast<E *>(Xm);
+    static_for<kChainPer>([&](auto r) { Xme[bo + rout(r)] = tile_e[rso(r)]; });
+    lds_barrier();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small-step batch: workgroup i contracts the step of descriptor i (lanes along C,
+// the K loop in registers).  The steps of one launch are independent.
+template <typename T>
+__global__ __launch_bounds__(256) void k_small_batch(const SmallDesc *__restrict__ d) {
+  using V = typename Vec<T>::type;
+  const SmallDesc &s = d[blockIdx.x];
+  const V *A = (const V *)s.A;
+  const V *B = (const V *)s.B;
+  V *C = (V *)s.C;
+  const int nc = 1 << s.nbc, nk = 1 << s.nbk;
+  for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+    int64_t oa = 0, ob = 0;
+    for (int b = 0; b < s.nbc; ++b)
+      if ((c >> b) & 1) oa += s.wa[b], ob += s.wb[b];
+    V acc{0, 0};
+    for (int k = 0; k < nk; ++k) {
+      int64_t xa = oa, xb = ob;
+      for (int b = 0; b < s.nbk; ++b)
+        if ((k >> b) & 1) xa += s.ka[b], xb += s.kb[b];
+      cmac<T>(acc, A[xa], B[xb], s.conj_a, s.conj_b);
     }
-    __syncthreads();
+    C[c] = acc;
   }
 }
 
@@ -1330,17 +1523,30 @@ void launch_reject(const double *amps, const char *cand, int64_t nb, int nch, ui
   check_launch("reject");
 }
 
-void launch_contract_chain(int prec, const void *X0, void *Xm, const ChainArgs &args, void *stream) {
+void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args, void *stream) {
   const int64_t ntiles = int64_t(1) << args.nouter;
-  const int g = (int)(ntiles < 256 * 4 ? ntiles : 256 * 4);
+  const int g = (int)(ntiles < args.grid ? ntiles : args.grid);
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == QX_C128) {
+    hipLaunchKernelGGL(k_chain_pack<double>, dim3(8), dim3(256), 0, s, (double2 *)ws, args);
+    hipLaunchKernelGGL(k_contract_chain<double>, dim3(g), dim3(kChainThreads), 0, s, (const double2 *)X0,
+                       (double2 *)Xm, (const double2 *)ws, args);
+  } else {
+    hipLaunchKernelGGL(k_chain_pack<float>, dim3(8), dim3(256), 0, s, (float2 *)ws, args);
+    hipLaunchKernelGGL(k_contract_chain<float>, dim3(g), dim3(kChainThreads), 0, s, (const float2 *)X0,
+                       (float2 *)Xm, (const float2 *)ws, args);
+  }
+  check_launch("contract_chain");
+}
+
+void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream) {
+  if (n <= 0) return;
   hipStream_t s = (hipStream_t)stream;
   if (prec == QX_C128)
-    hipLaunchKernelGGL(k_contract_chain<double>, dim3(g), dim3(kChainThreads), 0, s, (const double2 *)X0,
-                       (double2 *)Xm, args);
+    hipLaunchKernelGGL(k_small_batch<double>, dim3(n), dim3(256), 0, s, d_desc);
   else
-    hipLaunchKernelGGL(k_contract_chain<float>, dim3(g), dim3(kChainThreads), 0, s, (const float2 *)X0,
-                       (float2 *)Xm, args);
-  check_launch("contract_chain");
+    hipLaunchKernelGGL(k_small_batch<float>, dim3(n), dim3(256), 0, s, d_desc);
+  check_launch("small_batch");
 }
 
 }  // namespace qx

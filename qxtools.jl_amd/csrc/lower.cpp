@@ -313,7 +313,20 @@ bool force_kernel(Step &st, int precision, int choice) {
 std::vector<Step> kernel_candidates(const Step &st, int precision) {
   std::vector<Step> out;
   out.push_back(st);
-  if (st.kern == Kern::Radix || st.kern == Kern::Chain) return out;
+  if (st.kern == Kern::Chain) {
+    bool k16 = false;
+    for (int j = 0; j < st.chain.nsteps; ++j) k16 |= st.chain.st[j].k == 4;
+    for (int grid : {256, 512, 1024})
+      for (int m : {0, 1}) {
+        if ((m && !k16) || (grid == st.chain.grid && m == st.chain.mfma16)) continue;
+        Step v = st;
+        v.chain.grid = grid;
+        v.chain.mfma16 = m;
+        out.push_back(std::move(v));
+      }
+    return out;
+  }
+  if (st.kern == Kern::Radix) return out;
   const StepBits sb = step_bits(st);
   std::vector<int> alts;
   if (sb.nk >= 2 && sb.nn >= 2 && sb.nm >= 4) alts.insert(alts.end(), {kGemm0, kGemm1, kGemm2, kMfma});
@@ -584,6 +597,8 @@ Step chain_step(const Program &p, const Lowered &L, const ChainBuild &cb) {
     c.h = (int8_t)s.hslot.size();
     for (int i = 0; i < c.k; ++i) c.q[i] = (int8_t)pos_of[s.kslot[i]], c.sk[i] = (int16_t)s.sk[i], c.sn[i] = (int16_t)s.sn[i];
     for (int i = 0; i < c.h; ++i) c.hq[i] = (int8_t)pos_of[s.hslot[i]], c.sh[i] = (int16_t)s.sh[i];
+    c.soff = (int32_t)st.ws_elems;
+    st.ws_elems += (int64_t(1) << (c.h + 2 * c.k));
     st.chain_small.push_back(s.small);
     const Step &m = L.steps[cb.members[j]];
     flops += 8.0 * m.H * m.M * m.N * m.K;
@@ -684,6 +699,82 @@ std::vector<int> form_chains(const Program &p, Lowered &L) {
 
 }  // namespace
 
+namespace {
+
+// Whether a step may run inside a small-step batch: slice-free, power-of-two legs
+// with at most kSmallMaxBits C and K bits each, a few thousand MACs, and operand
+// offsets that fit 32 bits.
+bool small_step(const Step &st) {
+  if (st.section != 0 || st.kern == Kern::Chain || st.kern == Kern::Radix) return false;
+  if (st.c_elems > 4096 || st.K > 4096 || st.c_elems * st.K > (int64_t(1) << 18)) return false;
+  int nbc = 0, nbk = 0;
+  int64_t wmax = 0;
+  for (auto &g : st.cl) {
+    if (!pow2(g.dim)) return false;
+    nbc += ilog2(g.dim);
+    wmax = std::max({wmax, g.sa * (g.dim - 1), g.sb * (g.dim - 1)});
+  }
+  for (auto &g : st.kl) {
+    if (!pow2(g.dim)) return false;
+    nbk += ilog2(g.dim);
+    wmax = std::max({wmax, g.sa * (g.dim - 1), g.sb * (g.dim - 1)});
+  }
+  return nbc <= kSmallMaxBits && nbk <= kSmallMaxBits && wmax < (int64_t(1) << 30);
+}
+
+// Hoist the slice-free small steps whose inputs are data or other such steps to
+// the front of section 0, grouped by dependency level (a level's steps are
+// independent of each other).  The order stays topological; liveness is computed
+// on the new order afterwards.
+void level_small_steps(const Program &p, Lowered &L) {
+  const size_t nn = p.nodes.size();
+  std::vector<int> root(nn, -1), prod(nn, -1);
+  for (size_t i = 0; i < nn; ++i) {
+    const Node &n = p.nodes[i];
+    root[i] = (n.op == Op::View || n.op == Op::Save) ? root[n.ia] : (int)i;
+  }
+  for (size_t s = 0; s < L.steps.size(); ++s) prod[L.steps[s].node] = (int)s;
+  std::vector<int> level(L.steps.size(), -1);
+  int maxlev = -1;
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    const Step &st = L.steps[s];
+    if (!small_step(st)) continue;
+    int lv = 0;
+    bool ok = true;
+    for (int in : {st.in_a, st.in_b}) {
+      const int r = root[in];
+      if (r < 0) continue;
+      const int ps = prod[r];
+      if (ps < 0) continue;                       // data or bitstring output
+      if (L.steps[ps].section != 0) continue;     // cannot happen for section 0 inputs
+      if (level[ps] < 0) {
+        ok = false;
+        break;
+      }
+      lv = std::max(lv, level[ps] + 1);
+    }
+    if (!ok) continue;
+    level[s] = lv;
+    maxlev = std::max(maxlev, lv);
+  }
+  if (maxlev < 0) return;
+  std::vector<int> count(maxlev + 1, 0);
+  for (int lv : level)
+    if (lv >= 0) count[lv]++;
+  std::vector<Step> out;
+  for (int lv = 0; lv <= maxlev; ++lv)
+    for (size_t s = 0; s < L.steps.size(); ++s)
+      if (level[s] == lv) {
+        out.push_back(L.steps[s]);
+        out.back().level = count[lv] > 1 ? lv : -1;   // a lone step keeps its own kernel
+      }
+  for (size_t s = 0; s < L.steps.size(); ++s)
+    if (level[s] < 0) out.push_back(std::move(L.steps[s]));
+  L.steps = std::move(out);
+}
+
+}  // namespace
+
 Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uint64_t fused, bool chains) {
   if (!pow2(batch)) throw Error(QX_ERR_ARGUMENT, "batch must be a power of two");
   Lowered L;
@@ -768,8 +859,11 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
   }
 
   std::vector<char> absorbed(nn, 0);
-  if (chains && !keep_all)
+  if (chains && !keep_all) {
     for (int a : form_chains(p, L)) absorbed[a] = 1;
+    for (const Step &st : L.steps) L.ws_elems = std::max(L.ws_elems, st.ws_elems);
+    level_small_steps(p, L);
+  }
 
   // result elements per bitstring
   L.result_elems = 1;
@@ -785,11 +879,20 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
   {
     int64_t pos = 0;
     bool filled = false;
+    int prev_level = -1;
     for (int s : order) {
       if (!filled && L.steps[s].section == 2) {
         fill_pos = pos++;
         filled = true;
       }
+      // the steps of one batched level run concurrently: one schedule position, so
+      // none of them reuses memory another one reads or writes
+      const int lv = L.steps[s].level;
+      if (lv >= 0 && lv == prev_level) {
+        pos_of_step[s] = pos - 1;
+        continue;
+      }
+      prev_level = lv;
       pos_of_step[s] = pos++;
     }
     if (!filled) fill_pos = pos++;

@@ -108,6 +108,17 @@ struct RadixArgs {
   int conj_a = 0, conj_b = 0;
 };
 
+// Small slice-free steps (gate-block building) run many per launch: one workgroup
+// per step, its bit weights read from a device table of these descriptors.
+constexpr int kSmallMaxBits = 12;
+struct SmallDesc {
+  const void *A = nullptr, *B = nullptr;
+  void *C = nullptr;
+  int32_t nbc = 0, nbk = 0, conj_a = 0, conj_b = 0;
+  int32_t wa[kSmallMaxBits] = {0}, wb[kSmallMaxBits] = {0};   // C bits
+  int32_t ka[kSmallMaxBits] = {0}, kb[kSmallMaxBits] = {0};   // K bits
+};
+
 // A chain of steps that each contract a small operand (a gate block: at most 2^10
 // elements) into one large tensor X, X_j = S_j . X_{j-1}, run as one pass over X:
 // every workgroup loads a tile of 2^kChainTileBits elements of X_0 (the bits of
@@ -122,10 +133,13 @@ struct ChainStepArgs {
   int8_t k = 0, h = 0;          // contracted legs (each replaced by a new leg), batch legs
   int8_t q[5] = {0}, hq[3] = {0};   // tile bit of each contracted / batch leg
   int16_t sk[5] = {0}, sn[5] = {0}, sh[3] = {0};   // strides in S of contracted, new, batch legs
+  int32_t soff = 0;             // offset of S packed as [h][nu][kappa] in the workspace
 };
 
 struct ChainArgs {
   int nsteps = 0, nouter = 0;
+  int grid = 256;          // workgroups (each walks tiles); tuned
+  int mfma16 = 0;          // steps with 16 contracted values on matrix cores (else vector FMAs)
   int64_t win[kChainTileBits] = {0}, wout[kChainTileBits] = {0};   // tile bit -> X_0 / X_m stride
   int8_t pout[kChainTileBits] = {0};   // store order: i-th fastest X_m tile bit -> tile bit
   int64_t oin[kMaxBits] = {0}, oout[kMaxBits] = {0};   // outer bits (workgroup index) -> strides
@@ -167,6 +181,7 @@ struct Step {
   int64_t ws_elems = 0;    // workspace for K-split partials
   Kern kern = Kern::Bits;
   int section = 0;         // 0 shared slice-free, 1 shared per slice, 2 per bitstring batch
+  int level = -1;          // >= 0: small slice-free step run with the others of its level in one launch
   BitArgs bits;            // Kern::Bits / Kern::Reduce
   GemmArgs gemm;           // Kern::Gemm
   RowArgs rows;            // Kern::Rows
@@ -273,7 +288,8 @@ void launch_add_mapped(int prec, int dst_double, void *dst, const void *src, int
 // cotangent of the saved tensor from amp_cot rows [row0, row0 + nvalid)
 void launch_root_cot(int prec, void *dst, int dep, int64_t batch, int64_t nvalid, int64_t relems,
                      const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream);
-void launch_contract_chain(int prec, const void *X0, void *Xm, const ChainArgs &args, void *stream);
+void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream);
+void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args, void *stream);
 // Output sampling: candidate bitstrings from Philox4x64-10 and the sequential
 // empirical-supremum rejection test over one round of candidates.
 void launch_gen_bits(char *bits, int64_t nb, int nch, uint64_t seed, int64_t j0, void *stream);

@@ -103,6 +103,12 @@ struct qx_plan {
   std::vector<int64_t> step_calls;
   std::vector<hipEvent_t> events;           // pool: 2 per profiled launch
   std::vector<hipEvent_t> tune_events;      // pool: 2 per timed candidate launch
+  // descriptors of the batched small steps of one lowering (device table)
+  DevBuf small_desc;
+  const qx::Lowered *small_for = nullptr;
+  void *small_arena = nullptr;
+  std::vector<int> small_index;             // per step: descriptor index, or -1
+  std::vector<int> small_count;             // per step that starts a level: steps in the level
   // captured device work of qx_amplitudes per (batch, nb, slice_begin, slice_end)
   std::map<std::tuple<int64_t, int64_t, int64_t, int64_t>, hipGraphExec_t> graphs;
   void drop_graphs() {
@@ -128,6 +134,7 @@ struct qx_plan {
     grad.release();
     ampcot.release();
     ws.release();
+    small_desc.release();
     for (auto e : events) (void)hipEventDestroy(e);
     for (auto e : tune_events) (void)hipEventDestroy(e);
     drop_graphs();
@@ -193,7 +200,7 @@ void launch_step(qx_plan *P, const qx::Lowered &L, const qx::Step &st, void *A, 
         const qx::Value &v = L.values[st.chain_small[j]];
         a.S[j] = bs ? value_ptr(P, v, varval, *bs) : value_ptr(P, v, varval);
       }
-      qx::launch_contract_chain(P->prec, A, C, a, stream);
+      qx::launch_contract_chain(P->prec, A, C, P->ws.p, a, stream);
       break;
     }
     case qx::Kern::Radix: qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, st.radix, stream); break;
@@ -202,6 +209,73 @@ void launch_step(qx_plan *P, const qx::Lowered &L, const qx::Step &st, void *A, 
     case qx::Kern::Rows: qx::launch_contract_rows(P->prec, A, B, C, st.rows, stream); break;
     default: qx::launch_contract_bits(P->prec, A, B, C, P->ws.p, st.c_elems, st.bits, stream); break;
   }
+}
+
+// Build and upload the descriptor table of L's batched small steps (slice-free, so
+// their operand addresses depend only on the arena and data buffers).
+void prepare_small(qx_plan *P, const qx::Lowered &L) {
+  if (P->small_for == &L && P->small_arena == P->arena.p) return;
+  std::vector<qx::SmallDesc> d;
+  P->small_index.assign(L.steps.size(), -1);
+  P->small_count.assign(L.steps.size(), 0);
+  const std::vector<int64_t> zero(P->prog.vars.size(), 0);
+  int first = -1;
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    const qx::Step &st = L.steps[s];
+    if (st.level < 0) continue;
+    if (first < 0 || L.steps[first].level != st.level) first = (int)s;
+    P->small_count[first] += 1;
+    P->small_index[s] = (int)d.size();
+    qx::SmallDesc x;
+    x.A = value_ptr(P, L.values[st.in_a], zero);
+    x.B = value_ptr(P, L.values[st.in_b], zero);
+    x.C = value_ptr(P, L.values[st.node], zero);
+    x.conj_a = st.bits.conj_a;
+    x.conj_b = st.bits.conj_b;
+    for (auto &g : st.cl)
+      for (int64_t m = 1; m < g.dim; m <<= 1) {
+        if (x.nbc >= qx::kSmallMaxBits) throw qx::Error(QX_ERR_STATE, "internal: small step bits");
+        x.wa[x.nbc] = (int32_t)(g.sa * m);
+        x.wb[x.nbc] = (int32_t)(g.sb * m);
+        ++x.nbc;
+      }
+    for (auto &g : st.kl)
+      for (int64_t m = 1; m < g.dim; m <<= 1) {
+        if (x.nbk >= qx::kSmallMaxBits) throw qx::Error(QX_ERR_STATE, "internal: small step bits");
+        x.ka[x.nbk] = (int32_t)(g.sa * m);
+        x.kb[x.nbk] = (int32_t)(g.sb * m);
+        ++x.nbk;
+      }
+    d.push_back(x);
+  }
+  if (!d.empty()) {
+    P->small_desc.reserve(d.size() * sizeof(qx::SmallDesc));
+    hip_check(hipMemcpy(P->small_desc.p, d.data(), d.size() * sizeof(qx::SmallDesc), hipMemcpyHostToDevice),
+              "copy descriptors");
+  }
+  P->small_for = &L;
+  P->small_arena = P->arena.p;
+  P->drop_graphs();
+}
+
+// Launch the level of batched small steps that starts at step s.
+void run_small_level(qx_plan *P, const qx::Lowered &L, size_t s) {
+  hipStream_t stream = P->ctx->stream;
+  size_t ev = 0;
+  if (P->opt_profile) {
+    ev = P->event_step.size();
+    while (P->events.size() < 2 * (ev + 1)) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      P->events.push_back(e);
+    }
+    P->event_step.push_back((int)s);
+    hip_check(hipEventRecord(P->events[2 * ev], stream), "hipEventRecord");
+  }
+  const qx::SmallDesc *d = (const qx::SmallDesc *)P->small_desc.p + P->small_index[s];
+  qx::launch_small_batch(P->prec, d, P->small_count[s], stream);
+  if (P->opt_profile) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
+  (void)L;
 }
 
 void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int64_t> &varval) {
@@ -226,6 +300,14 @@ void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int6
 }
 
 std::string kern_variant(const qx::Step &st);
+
+// Distinct name of a candidate configuration (kernel plus launch choices).
+std::string cand_label(const qx::Step &st) {
+  std::string v = kern_variant(st);
+  if (st.kern == qx::Kern::Chain)
+    v += "/g" + std::to_string(st.chain.grid) + (st.chain.mfma16 ? "/mfma" : "/fma");
+  return v;
+}
 
 // The autotuner's state for one call: every step's candidate configurations and
 // the launches timed so far.  Candidates of a step all write the same C, so
@@ -283,7 +365,7 @@ void finish_tuning(qx_plan *P, qx::Lowered &L, Tuner &T) {
     os.precision(4);
     for (size_t c = 0; c < T.cands[s].size(); ++c) {
       if (T.ms[s][c] < T.ms[s][best]) best = c;
-      os << (c ? "," : "") << "\"" << kern_variant(T.cands[s][c]) << "\":" << T.ms[s][c] / T.seen[s];
+      os << (c ? "," : "") << "\"" << cand_label(T.cands[s][c]) << "\":" << T.ms[s][c] / T.seen[s];
     }
     L.tune_log[s] = os.str();
     L.steps[s] = T.cands[s][best];
@@ -456,6 +538,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   moved |= P->acc.reserve((size_t)nb * R * 32);
   moved |= P->outbuf.reserve((size_t)nb * R * 16);
   if (moved) P->drop_graphs();
+  prepare_small(P, L);
 
   P->event_step.clear();
 
@@ -467,6 +550,10 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     auto run_section = [&](int sec) {
       for (size_t s = 0; s < L.steps.size(); ++s) {
         if (L.steps[s].section != sec) continue;
+        if (L.steps[s].level >= 0) {   // batched small steps: one launch per level
+          if (P->small_count[s] > 0) run_small_level(P, L, s);
+          continue;
+        }
         if (tuner)
           tune_step(P, L, s, varval, *tuner);
         else
@@ -598,8 +685,9 @@ void sample_rejection(qx_plan *P, int64_t num_samples, uint64_t seed, double M, 
                                       " of " + std::to_string(num_samples) + " samples");
 }
 
-const char *kern_name(qx::Kern k) {
-  switch (k) {
+const char *kern_name(const qx::Step &st) {
+  if (st.level >= 0) return "small";
+  switch (st.kern) {
     case qx::Kern::Radix: return "radix";
     case qx::Kern::Gemm: return "gemm";
     case qx::Kern::Reduce: return "reduce";
@@ -612,6 +700,7 @@ const char *kern_name(qx::Kern k) {
 // Name of the kernel instantiation a step launches, as rocprofv3 reports it
 // (template arguments after the precision).
 std::string kern_variant(const qx::Step &st) {
+  if (st.level >= 0) return "k_small_batch";
   switch (st.kern) {
     case qx::Kern::Rows: {
       int nc = st.rows.nn >= 8 ? 8 : st.rows.nn;
@@ -652,8 +741,17 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
     os << "{\"name\":\"" << prog.nodes[st.node].name << "\",\"section\":" << st.section
        << ",\"fused_steps\":" << (st.kern == qx::Kern::Chain ? st.chain.nsteps : 1) << ",\"B\":" << st.H
        << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K << ",\"flops\":" << flops
-       << ",\"bytes\":" << bytes << ",\"kernel\":\"" << kern_name(st.kern) << "\",\"variant\":\""
-       << kern_variant(st) << "\",\"ws_elems\":" << st.ws_elems;
+       << ",\"bytes\":" << bytes << ",\"kernel\":\"" << kern_name(st) << "\",\"variant\":\""
+       << kern_variant(st) << "\",\"config\":\"" << cand_label(st) << "\",\"ws_elems\":" << st.ws_elems;
+    if (with_slots) {   // arena values read (views resolved), and the small-step level
+      auto root_name = [&](int n) {
+        while (prog.nodes[n].op == qx::Op::View || prog.nodes[n].op == qx::Op::Save) n = prog.nodes[n].ia;
+        return prog.nodes[n].op == qx::Op::Ncon ? prog.nodes[n].name : std::string();
+      };
+      os << ",\"level\":" << st.level << ",\"reads\":[\"" << root_name(st.in_a) << "\",\"" << root_name(st.in_b) << "\"";
+      for (int c : st.chain_small) os << ",\"" << root_name(c) << "\"";
+      os << "]";
+    }
     if (with_slots) {   // leg strides (elements) in A, B and C: free legs in C order, then contracted legs
       os << ",\"legs\":[";
       for (size_t j = 0; j < st.cl.size(); ++j)
@@ -665,9 +763,9 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
       os << "]";
     }
     os << ",\"candidates\":[";
-    auto cands = qx::kernel_candidates(st, es == 16 ? QX_C128 : QX_C64_4M);
+    auto cands = st.level >= 0 ? std::vector<qx::Step>{st} : qx::kernel_candidates(st, es == 16 ? QX_C128 : QX_C64_4M);
     for (size_t c = 0; c < cands.size(); ++c)
-      os << (c ? "," : "") << "\"" << kern_variant(cands[c]) << "\"";
+      os << (c ? "," : "") << "\"" << cand_label(cands[c]) << "\"";
     os << "]}";
   }
   os << "]}";
@@ -845,7 +943,8 @@ std::string stats_json(const qx_plan *P) {
          << ",\"fused_steps\":" << (st.kern == qx::Kern::Chain ? st.chain.nsteps : 1)
          << ",\"B\":" << st.H << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K
          << ",\"flops\":" << flops << ",\"bytes\":" << bytes << ",\"kernel\":\""
-         << kern_name(st.kern) << "\",\"variant\":\"" << kern_variant(st) << "\"";
+         << kern_name(st) << "\",\"variant\":\"" << kern_variant(st) << "\",\"config\":\"" << cand_label(st)
+         << "\"";
       if (P->opt_profile && &L == P->last_lowered && s < P->step_ms.size())
         os << ",\"ms\":" << P->step_ms[s] << ",\"calls\":" << P->step_calls[s];
       if (s < L.tune_log.size() && !L.tune_log[s].empty()) os << ",\"tuning_ms\":{" << L.tune_log[s] << "}";

@@ -66,3 +66,32 @@ def rel_err(a, r):
     a = np.asarray(a)
     r = np.asarray(r)
     return np.abs(a - r) / np.maximum(np.abs(r), 1e-300)
+
+
+def gate_sequence_program(n: int, gates: list, seed: int = 0):
+    """DSL of dense gates applied to a random 2^n tensor: gate j (k legs, a
+    2^k x 2^k block) replaces the tensor's legs `gates[j]` (qubit positions) with
+    new ones; the result keeps all n legs.  Returns (dsl, data)."""
+    rng = np.random.default_rng(seed)
+    qlab = list(range(1, n + 1))
+    order = list(range(1, n + 1))
+    nxt = n + 1
+    lines = ["# version: 0.4.0", "load X data_x " + ",".join(["2"] * n)]
+    data = {"data_x": rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)}
+    cur = "X"
+    for j, legs in enumerate(gates):
+        k = len(legs)
+        data[f"data_g{j}"] = (rng.normal(size=4 ** k) + 1j * rng.normal(size=4 ** k)) / 2 ** k
+        lines.append(f"load G{j} data_g{j} " + ",".join(["2"] * (2 * k)))
+        ins = [qlab[q] for q in legs]
+        outs = list(range(nxt, nxt + k))
+        nxt += k
+        new = [l for l in order if l not in ins] + outs
+        lines.append(f"ncon I{j} {','.join(map(str, new))} {cur} {','.join(map(str, order))} "
+                     f"G{j} {','.join(map(str, ins + outs))}")
+        for q, o in zip(legs, outs):
+            qlab[q] = o
+        order = new
+        cur = f"I{j}"
+    lines.append(f"save output {cur}")
+    return "\n".join(lines) + "\n", data

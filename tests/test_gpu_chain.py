@@ -55,3 +55,28 @@ def test_chain_with_graph_replay_and_autotune():
     ex = Executor(dsl, g.data, batch=8, graph=1, autotune=1)
     for _ in range(3):
         assert np.max(qxtest.rel_err(ex.amplitudes(bits), ref)) <= 1e-12
+
+
+GATE_RUNS = {
+    "k1-k5 mixed": [(0,), (1, 2), (3, 4, 5), (6, 7, 8, 9), (2, 3, 4, 5, 6)],
+    "k4 x3 overlapping": [(10, 11, 12, 13), (13, 14, 15, 0), (0, 1, 2, 3)],
+    "k5 x2 high legs": [(11, 12, 13, 14, 15), (6, 7, 8, 9, 10)],
+    "k2 x4 low legs": [(0, 1), (1, 2), (2, 3), (0, 3)],
+}
+
+
+@pytest.mark.parametrize("name", sorted(GATE_RUNS))
+@pytest.mark.parametrize("precision,tol", [("c128", 1e-12), ("c64_4m", 2e-5)])
+def test_chain_gate_runs(name, precision, tol):
+    """Every contracted-leg count (vector path for 1-8 values, matrix cores for 16
+    and 32) on a 2^16 tensor, against the oracle and the unchained lowering."""
+    dsl, data = qxtest.gate_sequence_program(16, GATE_RUNS[name], seed=len(name))
+    info = _native.inspect(dsl, precision=precision)
+    assert [s["kernel"] for s in info["steps"]] == ["chain"] * len(info["steps"])
+    ex = Executor(dsl, data, precision=precision, batch=1)
+    out = np.asarray(ex.amplitudes([""])).reshape(-1)
+    ref = O.amplitudes(dsl, data, [""])[0].reshape(-1, order="F")
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol
+    plain = Executor(dsl, data, precision=precision, batch=1, chains=0)
+    outp = np.asarray(plain.amplitudes([""])).reshape(-1)
+    assert np.linalg.norm(out - outp) / np.linalg.norm(outp) < tol

@@ -182,6 +182,6 @@ def test_autotune_keeps_results(nslice):
     act = [x for x in ex.stats()["batches"] if x["active"]][0]
     assert act["tuned"]
     timed = [s for s in act["steps"] if "tuning_ms" in s]
-    assert timed and all(s["variant"] in s["tuning_ms"] for s in timed)
-    for s in timed:   # the kept variant is the fastest measured
-        assert s["tuning_ms"][s["variant"]] <= min(s["tuning_ms"].values()) + 1e-12
+    assert timed and all(s["config"] in s["tuning_ms"] for s in timed)
+    for s in timed:   # the kept configuration is the fastest measured
+        assert s["tuning_ms"][s["config"]] <= min(s["tuning_ms"].values()) + 1e-12

@@ -157,7 +157,7 @@ def test_autotune_candidates(name):
     info = _native.inspect(dsl, precision=prec, batch=64)
     for st in info["steps"]:
         c = st["candidates"]
-        assert c[0] == st["variant"] and len(set(c)) == len(c)
+        assert c[0] == st["config"] and len(set(c)) == len(c)
         if st["kernel"] == "gemm":
             assert {"k_contract_gemm<128,16,16>", "k_contract_gemm<128,32,16>", "k_contract_gemm<64,64,16>"} <= set(c)
         if prec == "c128":
@@ -173,5 +173,26 @@ def test_chains_fold_gate_block_runs():
     assert len(chains) >= 10 and all(s["fused_steps"] >= 2 for s in chains)
     big = [s for s in info["steps"] if s["bytes"] > 1e8]
     assert len(big) <= 32                      # 72 passes without chains
-    for s in chains:
-        assert s["candidates"] == ["k_contract_chain"]
+    for s in chains:   # launch configurations of the one chain kernel
+        assert all(c.startswith("k_contract_chain/") for c in s["candidates"]) and len(s["candidates"]) >= 3
+
+
+def test_batched_small_levels_do_not_share_memory():
+    """Small slice-free steps of one dependency level run in one launch, so within a
+    level no step may write memory another step of the level reads or writes."""
+    dsl, data, bits = workloads.WORKLOADS["rqc_5x5_d20"].load()
+    info = _native.inspect(dsl, batch=256)
+    slot = {s["name"]: (s["offset"], s["offset"] + s["elems"]) for s in info["slots"]}
+    levels = {}
+    for st in info["steps"]:
+        if st["level"] >= 0:
+            levels.setdefault(st["level"], []).append(st)
+    assert len(levels) >= 5 and sum(len(v) for v in levels.values()) >= 300
+    overlap = lambda a, b: a[0] < b[1] and b[0] < a[1]
+    for lv, sts in levels.items():
+        outs = [slot[s["name"]] for s in sts]
+        reads = [slot[r] for s in sts for r in s["reads"] if r]
+        for i, o in enumerate(outs):
+            assert not any(overlap(o, x) for j, x in enumerate(outs) if j != i)
+            assert not any(overlap(o, r) for r in reads)
+        assert all(s["variant"] == "k_small_batch" for s in sts)
