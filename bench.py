@@ -79,6 +79,23 @@ def cpu_baseline(dsl: str, data: dict, bits: list, seconds: float) -> dict:
                       f"by the oracle ({el:.1f} s)"}
 
 
+def measured_traffic(workload: str, prec: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a copy of known size), or None."""
+    import glob
+    import re
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path in glob.glob(os.path.join(here, "profiles", "pmc_traffic_*.md")):
+        m = re.search(r"```json\s*(\{.*?\})\s*```", open(path).read(), re.S)
+        if not m:
+            continue
+        d = json.loads(m.group(1))
+        if d.get("workload") == workload and d.get("precision") == prec:
+            return d["traffic_bytes_per_launch"].get(kernel)
+    return None
+
+
 def c64_comparison(dsl: str, data: dict, bits: list, ref: np.ndarray, a, precs=("c64_4m", "c64_3m")) -> dict:
     """Other precisions on the same plan: throughput and per-amplitude relative error
     against `ref` (the complex128 GPU result)."""
@@ -232,7 +249,7 @@ def main() -> int:
         ach = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
                 "frac": ach / PEAK_TFLOPS[prec]}
-    roof.update({"traffic": None, "kernel": dom_name, "launches": dom["launches"],
+    roof.update({"traffic": measured_traffic(w.name, prec, dom_name), "kernel": dom_name, "launches": dom["launches"],
                  "avg_launch_us": 1e3 * dom["ms"] / dom["launches"],
                  "algorithmic_bytes_per_launch": dom["bytes"] / dom["launches"],
                  "algorithmic_flops_per_launch": dom["flops"] / dom["launches"]})
