@@ -162,9 +162,20 @@ def grad_bench(a) -> dict:
         loss, grad, _ = pc.loss_and_grad(bits)
     torch.cuda.synchronize()
     both = (time.perf_counter() - t0) / a.steps
+    amps = pc.amplitudes(bits)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        kg = pc.executor.vjp(bits, 2.0 * amps)
+    torch.cuda.synchronize()
+    pull = (time.perf_counter() - t0) / a.steps
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        pc.param_grad(kg)
+    chain = (time.perf_counter() - t0) / a.steps
     return {"metric": "gradient time / forward time", "value": (both - fwd) / fwd, "unit": "ratio",
             "higher_is_better": False, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
             "forward_ms": 1e3 * fwd, "forward_plus_gradient_ms": 1e3 * both,
+            "pullback_ms": 1e3 * pull, "host_chain_rule_ms": 1e3 * chain,
             "forward_amplitudes_per_s": len(bits) / fwd, "params": int(pc.num_params),
             "dtype": "c128" if prec == "c128" else "c64", "data": "synthetic",
             "config": {"workload": w.name, "qubits": w.qubits, "depth": w.depth, "bitstrings": len(bits),

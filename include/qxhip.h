@@ -44,9 +44,10 @@ typedef int32_t qx_status;
 #define QX_ERR_UNSUPPORTED (-8) /* valid DSL outside what this build executes   */
 
 /* Arithmetic of a plan.  QX_C128: complex128 storage and arithmetic.  The C64
- * variants store complex64; their products are formed as 4 real products (4M),
- * 3 real products (3M, Gauss), or, on large steps, as three bf16 MFMA products of
- * a hi/lo split (BF16X3); accumulation is FP32, the cross-slice sum FP64. */
+ * variants store complex64 and accumulate in FP32 (the cross-slice sum in FP64);
+ * large GEMM steps run on FP32 matrix cores with 4 real products (4M) or 3 (3M,
+ * Gauss).  QX_C64_BF16X3 (three bf16 products of a hi/lo split) is reserved:
+ * qx_plan_from_dsl returns QX_ERR_UNSUPPORTED for it. */
 enum qx_precision {
   QX_C128 = 0,
   QX_C64_4M = 1,

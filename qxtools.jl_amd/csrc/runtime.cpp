@@ -111,9 +111,13 @@ struct qx_plan {
   std::vector<int> small_count;             // per step that starts a level: steps in the level
   // captured device work of qx_amplitudes per (batch, nb, slice_begin, slice_end)
   std::map<std::tuple<int64_t, int64_t, int64_t, int64_t>, hipGraphExec_t> graphs;
+  // captured device work of qx_amplitudes_vjp per (batch, nb, -, fused slice vars)
+  std::map<std::tuple<int64_t, int64_t, int64_t, int64_t>, hipGraphExec_t> vjp_graphs;
   void drop_graphs() {
     for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
     graphs.clear();
+    for (auto &kv : vjp_graphs) (void)hipGraphExecDestroy(kv.second);
+    vjp_graphs.clear();
   }
   std::vector<int> event_step;              // step of each recorded pair this call
 
@@ -808,22 +812,21 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
   const int64_t es = (int64_t)P->esize();
   const int64_t nch = P->prog.num_chars;
   const int64_t R = L.result_elems;
-  P->varena.reserve(std::max<size_t>((size_t)L.arena_elems * es, 256));
-  if (P->ws.reserve(std::max<size_t>((size_t)L.ws_elems * es, 256))) P->drop_graphs();
-  P->cot.reserve(std::max<size_t>((size_t)V.cot_elems * es, 256));
-  P->temp.reserve(std::max<size_t>((size_t)V.temp_elems * es, 256));
+  bool moved = P->varena.reserve(std::max<size_t>((size_t)L.arena_elems * es, 256));
+  moved |= P->ws.reserve(std::max<size_t>((size_t)L.ws_elems * es, 256));
+  moved |= P->cot.reserve(std::max<size_t>((size_t)V.cot_elems * es, 256));
+  moved |= P->temp.reserve(std::max<size_t>((size_t)V.temp_elems * es, 256));
   std::vector<int64_t> key_off;
   int64_t ke = 0;
   for (auto &k : P->prog.keys) {
     key_off.push_back(ke);
     ke += P->prog.key_elems.at(k);
   }
-  P->grad.reserve(std::max<size_t>((size_t)ke * 16, 16));
-  qx::launch_zero(P->grad.p, ke * 16, stream);
-  P->ampcot.reserve((size_t)nb * R * 16);
+  moved |= P->grad.reserve(std::max<size_t>((size_t)ke * 16, 16));
+  moved |= P->ampcot.reserve((size_t)nb * R * 16);
   hip_check(hipMemcpyAsync(P->ampcot.p, amp_cot, (size_t)nb * R * 16, hipMemcpyHostToDevice, stream), "copy");
   size_t nbits = (size_t)nb * nch;
-  P->bits.reserve(std::max<size_t>(nbits, 16));
+  moved |= P->bits.reserve(std::max<size_t>(nbits, 16));
   if (nbits) hip_check(hipMemcpyAsync(P->bits.p, bits, nbits, hipMemcpyHostToDevice, stream), "copy");
   std::vector<int64_t> ooff;
   std::vector<int> oq;
@@ -832,8 +835,8 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
     oq.push_back(P->prog.nodes[o].qubit);
   }
   const int nout = (int)ooff.size();
-  P->out_off.reserve(std::max<size_t>(ooff.size() * 8, 16));
-  P->out_q.reserve(std::max<size_t>(oq.size() * 4, 16));
+  moved |= P->out_off.reserve(std::max<size_t>(ooff.size() * 8, 16));
+  moved |= P->out_q.reserve(std::max<size_t>(oq.size() * 4, 16));
   if (nout) {
     hip_check(hipMemcpyAsync(P->out_off.p, ooff.data(), ooff.size() * 8, hipMemcpyHostToDevice, stream), "copy");
     hip_check(hipMemcpyAsync(P->out_q.p, oq.data(), oq.size() * 4, hipMemcpyHostToDevice, stream), "copy");
@@ -852,8 +855,9 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
       roff[e] = o;
     }
   }
-  P->relem_off.reserve(std::max<size_t>(R * 8, 16));
+  moved |= P->relem_off.reserve(std::max<size_t>(R * 8, 16));
   hip_check(hipMemcpyAsync(P->relem_off.p, roff.data(), R * 8, hipMemcpyHostToDevice, stream), "copy");
+  if (moved) P->drop_graphs();
 
   Bases bs;
   bs.arena = (char *)P->varena.p;
@@ -897,6 +901,9 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
     }
   };
   const int64_t nslices = looped_slices(P, fused);
+  // the device work of one call, from zeroing the gradient to the last backward step
+  auto enqueue = [&]() {
+  qx::launch_zero(P->grad.p, ke * 16, stream);
   run_fwd(0);
   zero_cots(0);
   for (int64_t sl = 0; sl < nslices; ++sl) {
@@ -916,6 +923,31 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
     run_bwd(1);
   }
   run_bwd(0);
+  };
+  if (P->opt_graph && !P->opt_profile) {
+    auto key = std::make_tuple(batch, nb, int64_t(-1), (int64_t)fused);
+    auto it = P->vjp_graphs.find(key);
+    if (it == P->vjp_graphs.end()) {
+      hipGraph_t g = nullptr;
+      hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        enqueue();
+      } catch (...) {
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+      hipGraphExec_t ex = nullptr;
+      hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      hip_check(e, "hipGraphInstantiate");
+      it = P->vjp_graphs.emplace(key, ex).first;
+    }
+    hip_check(hipGraphLaunch(it->second, stream), "hipGraphLaunch");
+  } else {
+    enqueue();
+  }
   hip_check(hipMemcpyAsync(grad, P->grad.p, (size_t)ke * 16, hipMemcpyDeviceToHost, stream), "copy out");
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
@@ -991,6 +1023,7 @@ qx_status qx_plan_from_dsl(qx_ctx *ctx, const char *dsl, size_t len, int precisi
     if (!ctx || !out) throw qx::Error(QX_ERR_ARGUMENT, "null context or output pointer");
     *out = nullptr;
     if (precision < QX_C128 || precision > QX_C64_BF16X3) throw qx::Error(QX_ERR_ARGUMENT, "unknown precision");
+    if (precision == QX_C64_BF16X3) throw qx::Error(QX_ERR_UNSUPPORTED, "precision QX_C64_BF16X3 is not implemented");
     auto P = std::make_unique<qx_plan>();
     P->ctx = ctx;
     P->prec = precision;
@@ -1169,6 +1202,7 @@ qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t bat
   return guard([&] {
     if (!json || cap == 0) throw qx::Error(QX_ERR_ARGUMENT, "null output buffer");
     if (precision < QX_C128 || precision > QX_C64_BF16X3) throw qx::Error(QX_ERR_ARGUMENT, "unknown precision");
+    if (precision == QX_C64_BF16X3) throw qx::Error(QX_ERR_UNSUPPORTED, "precision QX_C64_BF16X3 is not implemented");
     qx::Program prog = qx::parse_program(dsl, len);
     qx::Lowered L = qx::lower(prog, batch, false, precision, fused);
     std::ostringstream os;

@@ -25,7 +25,7 @@ class ParamCircuit:
     """A parameterised circuit compiled to a saved plan with per-gate data keys."""
 
     def __init__(self, circ: Circuit, planner: str = "sweep", nslice: int = 0, precision: str = "c128",
-                 batch: int | None = None, device: int = 0, **plan_kwargs):
+                 batch: int | None = None, device: int = 0, graph: int = 1, **plan_kwargs):
         self.circ = circ
         tnc, names = gate_tensor_map(circ)
         tnc.add_input()
@@ -49,18 +49,28 @@ class ParamCircuit:
             if d.shape != self.graph.data[key].shape:
                 raise ValueError(f"derivative of gate {g.name} has a different tensor structure")
             self.terms.append((g.param, key, d))
+        # the chain rule over all parameters as one segmented reduction
+        self._tp = np.array([p for p, _, _ in self.terms], dtype=np.int64)
+        self._tkeys = [k for _, k, _ in self.terms]
+        self._td = np.concatenate([d.reshape(-1, order="F") for _, _, d in self.terms]) if self.terms else np.zeros(0)
+        self._tstart = np.cumsum([0] + [d.size for _, _, d in self.terms[:-1]]).astype(np.int64)
         self.dsl = self.graph.to_dsl()
-        self.executor = Executor(self.dsl, self.graph.data, precision=precision, device=device, batch=batch)
+        self.executor = Executor(self.dsl, self.graph.data, precision=precision, device=device, batch=batch,
+                                 graph=graph)
         self.num_params = circ.num_params
 
     def amplitudes(self, bits) -> np.ndarray:
         return self.executor.amplitudes(bits)
 
     def param_grad(self, key_grad: dict) -> np.ndarray:
+        """dL/dtheta_p = sum over the gate tensors T of parameter p of
+        Re <dL/dT, dT/dtheta_p> (the cotangent is conj-linear)."""
         out = np.zeros(self.num_params)
-        for p, key, d in self.terms:
-            g = key_grad[key].reshape(d.shape, order="F")
-            out[p] += float(np.real(np.sum(np.conj(g) * d)))
+        if not self.terms:
+            return out
+        g = np.concatenate([key_grad[k] for k in self._tkeys])
+        per_term = np.add.reduceat(np.real(np.conj(g) * self._td), self._tstart)
+        np.add.at(out, self._tp, per_term)
         return out
 
     def loss_and_grad(self, bits) -> tuple:

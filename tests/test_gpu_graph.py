@@ -30,3 +30,22 @@ def test_graph_replays_match_eager(planner, nslice, nb, batch):
         _ = eager.amplitudes(bits[:1])          # eager work on the same device between replays
     part = graphed.amplitudes(bits, 0, 1)      # another slice range: another graph
     assert np.array_equal(part, eager.amplitudes(bits, 0, 1))
+
+
+@pytest.mark.parametrize("planner,nslice,nb,batch", [("sweep", 0, 8, 8), ("greedy", 2, 6, 4)])
+def test_vjp_graph_replays_match_eager(planner, nslice, nb, batch):
+    """qx_amplitudes_vjp replayed from its captured graph equals the eager pullback,
+    with forward calls and other bitstrings between the replays."""
+    circ = circuits.create_rqc_circuit(3, 3, 8, seed=5)
+    t, plan, g = qxtest.graph_for(circ, planner=planner, nslice=nslice)
+    dsl = g.to_dsl()
+    eager = Executor(dsl, g.data, batch=batch)
+    graphed = Executor(dsl, g.data, batch=batch, graph=1)
+    for rep in range(3):
+        bits = simulation.amplitudes_uniform(9, 200 + rep, nb)
+        cot = np.random.default_rng(rep).normal(size=nb) + 1j
+        want = eager.vjp(bits, cot)
+        got = graphed.vjp(bits, cot)
+        for k in want:
+            assert np.array_equal(got[k], want[k]), (rep, k)
+        _ = graphed.amplitudes(bits)             # forward graph between vjp replays

@@ -196,3 +196,9 @@ def test_batched_small_levels_do_not_share_memory():
             assert not any(overlap(o, x) for j, x in enumerate(outs) if j != i)
             assert not any(overlap(o, r) for r in reads)
         assert all(s["variant"] == "k_small_batch" for s in sts)
+
+
+def test_reserved_precision_is_rejected():
+    with pytest.raises(_native.QxError) as e:
+        _native.inspect("load A data_1 2\nsave output A\n", precision="c64_bf16x3")
+    assert e.value.status == -8
