@@ -288,6 +288,14 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = a.copy_mib << 20
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        # read + write bytes per copy / time: the practical HBM ceiling of a stream
+        out["copy_gbps"] = 2 * (a.copy_mib << 20) * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
     if not a.no_c64 and prec == "c128":
         out["c64"] = c64_comparison(dsl, data, bits, np.asarray(amps), a)
     elif not a.no_c64:
