@@ -90,7 +90,7 @@ qx_status qx_plan_set_tensor(qx_plan *plan, const char *key, const double *z, in
  *                 every step (same results), times them with HIP events and keeps
  *                 the fastest per step for later calls; default 0
  *   "kernel"      -1 (default) = heuristic kernel choice; 0 gather, 1 rows, 2-4
- *                 FP tile GEMMs, 5 matrix-core GEMM (complex64), 6 reduce = use
+ *                 FP tile GEMMs, 5 / 7 matrix-core GEMM (K chunks of 16 / 32), 6 reduce = use
  *                 that kernel on every step it can run (testing) */
 qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value);
 

@@ -339,30 +339,50 @@ __global__ __launch_bounds__(256) void k_contract_gemm(const typename Vec<T>::ty
 }
 
 
-// ---------------------------------------------------------------------------
-// Complex64 GEMM on FP32 matrix cores (v_mfma_f32_16x16x4f32), same bit-weight
-// tile gathers as k_contract_gemm.  A 64x64 complex C tile per workgroup, 4 waves
-// of 32x32 (2x2 MFMA blocks of 16x16).  4M: Re += Ar.Br - Ai.Bi and Im += Ar.Bi +
-// Ai.Br as four real products per block; 3M (Gauss): RR = Ar.Br, II = Ai.Bi,
-// SS = (Ar+Ai).(Br+Bi), Re = RR - II, Im = SS - RR - II with three.  FP32
-// accumulation.  MFMA operand layouts: A[i][k] from lane i + 16k, B[k][j] from lane
-// j + 16k, D[i][j] in lane j + 16(i/4), register i%4.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+// 16x16x4 matrix-core products.  Both take A[i][k] from lane i + 16 k and B[k][j]
+// from lane j + 16 k; lane l holds D[row(l / 16, r)][l % 16] in element r, with
+// row = 4 (l / 16) + r for FP32 and 4 r + l / 16 for FP64.
+template <typename T> struct Mfma16;
+template <> struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * r + lk; }
+};
+template <> struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
 
-template <bool GAUSS>
-__global__ __launch_bounds__(256) void k_contract_gemm_mfma(const float2 *__restrict__ A, const float2 *__restrict__ B,
-                                                            float2 *__restrict__ C, GemmArgs a) {
-  constexpr int TM = 64, TN = 64, TK = 16;
+// ---------------------------------------------------------------------------
+// Complex GEMM on 16x16x4 matrix cores (FP32 for complex64, FP64 for complex128),
+// same bit-weight tile gathers as k_contract_gemm.  A 64x64 complex C tile per
+// workgroup, 4 waves of 32x32 (2x2 blocks of 16x16).  4M: Re += Ar.Br - Ai.Bi and
+// Im += Ar.Bi + Ai.Br as four real products per block; 3M (Gauss, complex64 only):
+// RR = Ar.Br, II = Ai.Bi, SS = (Ar+Ai).(Br+Bi), Re = RR - II, Im = SS - RR - II.
+// Accumulation in the operand precision.  Operand and result layouts as in Mfma16.
+template <typename T, bool GAUSS, int TK>
+__global__ __launch_bounds__(256) void k_contract_gemm_mfma(const typename Vec<T>::type *__restrict__ A,
+                                                            const typename Vec<T>::type *__restrict__ B,
+                                                            typename Vec<T>::type *__restrict__ C, GemmArgs a) {
+  using V = typename Vec<T>::type;
+  using M = Mfma16<T>;
+  using f32x4 = typename M::v4;
+  constexpr int TM = 64, TN = 64;
   constexpr int PA = TM + 1, PB = TN + 1, PC = TN + 1;
   constexpr int LDS_AB = TK * (PA + PB), LDS_C = TM * PC;
   constexpr int LDS = LDS_AB > LDS_C ? LDS_AB : LDS_C;
   constexpr int EA = TM * TK / 256, EB = TN * TK / 256;
-  __shared__ float2 lds[LDS];
+  __shared__ V lds[LDS];
   __shared__ int64_t t_am[TM], t_cm[TM], t_bn[TN], t_cn[TN], t_ak[TK], t_bk[TK];
-  float2 *sA = lds;
-  float2 *sB = lds + TK * PA;
+  V *sA = lds;
+  V *sB = lds + TK * PA;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lm = 6, ln = 6, lk = 4;
+  const int lm = 6, ln = 6, lk = TK == 16 ? 4 : 5;
   const int rows = a.nbm < lm ? (1 << a.nbm) : TM;
   const int cols = a.nbn < ln ? (1 << a.nbn) : TN;
   const int kk = a.nbk < lk ? (1 << a.nbk) : TK;
@@ -437,7 +457,7 @@ __global__ __launch_bounds__(256) void k_contract_gemm_mfma(const float2 *__rest
       if (((nt << ln) >> b) & 1) ob += a.nb[b], oc += a.nc[b];
     oc += split * a.c_elems;
   };
-  float2 ra[EA], rb[EB];
+  V ra[EA], rb[EB];
   auto fetch = [&](int64_t item) {
     int64_t unit = t_begin + item / chunks, oa, ob, oc;
     int64_t kc = (unit % a.ksplit) * chunks + item % chunks;
@@ -445,9 +465,9 @@ __global__ __launch_bounds__(256) void k_contract_gemm_mfma(const float2 *__rest
     for (int b = lk; b < a.nbk; ++b)
       if (((kc << lk) >> b) & 1) oa += a.ka[b], ob += a.kb[b];
 #pragma unroll
-    for (int j = 0; j < EA; ++j) ra[j] = oka[j] ? A[oa + offa[j]] : float2{0, 0};
+    for (int j = 0; j < EA; ++j) ra[j] = oka[j] ? A[oa + offa[j]] : V{0, 0};
 #pragma unroll
-    for (int j = 0; j < EB; ++j) rb[j] = okb[j] ? B[ob + offb[j]] : float2{0, 0};
+    for (int j = 0; j < EB; ++j) rb[j] = okb[j] ? B[ob + offb[j]] : V{0, 0};
   };
   // wave (wm, wn) owns rows 32*wm.., cols 32*wn..; block (bi, bj) of 16x16
   const int wm = wave & 1, wn = wave >> 1;
@@ -469,13 +489,13 @@ __global__ __launch_bounds__(256) void k_contract_gemm_mfma(const float2 *__rest
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < EA; ++j) {
-      float2 v = ra[j];
+      V v = ra[j];
       if (a.conj_a) v.y = -v.y;
       sA[slota[j]] = v;
     }
 #pragma unroll
     for (int j = 0; j < EB; ++j) {
-      float2 v = rb[j];
+      V v = rb[j];
       if (a.conj_b) v.y = -v.y;
       sB[slotb[j]] = v;
     }
@@ -483,7 +503,7 @@ __global__ __launch_bounds__(256) void k_contract_gemm_mfma(const float2 *__rest
     if (it + 1 < items) fetch(it + 1);
 #pragma unroll
     for (int k0 = 0; k0 < TK; k0 += 4) {
-      float2 fa[2], fb[2];
+      V fa[2], fb[2];
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi) fa[bi] = sA[(k0 + lq) * PA + 32 * wm + 16 * bi + li];
 #pragma unroll
@@ -492,14 +512,13 @@ __global__ __launch_bounds__(256) void k_contract_gemm_mfma(const float2 *__rest
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) {
-          acc_rr[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[bi].x, fb[bj].x, acc_rr[bi][bj], 0, 0, 0);
-          acc_ii[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[bi].y, fb[bj].y, acc_ii[bi][bj], 0, 0, 0);
+          acc_rr[bi][bj] = M::mma(fa[bi].x, fb[bj].x, acc_rr[bi][bj]);
+          acc_ii[bi][bj] = M::mma(fa[bi].y, fb[bj].y, acc_ii[bi][bj]);
           if (GAUSS) {
-            acc_x[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[bi].x + fa[bi].y, fb[bj].x + fb[bj].y,
-                                                                  acc_x[bi][bj], 0, 0, 0);
+            acc_x[bi][bj] = M::mma(fa[bi].x + fa[bi].y, fb[bj].x + fb[bj].y, acc_x[bi][bj]);
           } else {
-            acc_x[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[bi].x, fb[bj].y, acc_x[bi][bj], 0, 0, 0);
-            acc_x[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[bi].y, fb[bj].x, acc_x[bi][bj], 0, 0, 0);
+            acc_x[bi][bj] = M::mma(fa[bi].x, fb[bj].y, acc_x[bi][bj]);
+            acc_x[bi][bj] = M::mma(fa[bi].y, fb[bj].x, acc_x[bi][bj]);
           }
         }
     }
@@ -507,19 +526,19 @@ __global__ __launch_bounds__(256) void k_contract_gemm_mfma(const float2 *__rest
       int64_t oa, ob, oc;
       tile_offsets(t_begin + it / chunks, oa, ob, oc);
       __syncthreads();
-      float2 *sC = lds;
+      V *sC = lds;
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = 32 * wm + 16 * bi + 4 * lq + r, j = 32 * wn + 16 * bj + li;
-            const float rr = acc_rr[bi][bj][r], ii = acc_ii[bi][bj][r], x = acc_x[bi][bj][r];
-            sC[i * PC + j] = GAUSS ? float2{rr - ii, x - rr - ii} : float2{rr - ii, x};
+            const int i = 32 * wm + 16 * bi + M::row(lq, r), j = 32 * wn + 16 * bj + li;
+            const T rr = acc_rr[bi][bj][r], ii = acc_ii[bi][bj][r], x = acc_x[bi][bj][r];
+            sC[i * PC + j] = GAUSS ? V{rr - ii, x - rr - ii} : V{rr - ii, x};
           }
       __syncthreads();
-      float2 *Cb = C + oc;
+      V *Cb = C + oc;
       for (int e = tid; e < TM * TN; e += 256) {
         int i, j;
         if (a.c_nfast) i = e / TN, j = e % TN;
@@ -896,16 +915,22 @@ void launch_contract_gemm(int prec, const void *A, const void *B, void *C, void 
                           void *stream) {
   hipStream_t s = (hipStream_t)stream;
   void *dst = args.ksplit > 1 ? ws : C;
-  if (args.tile == 3) {   // complex64 on FP32 matrix cores, 64x64 tiles
+  if (args.tile >= 3) {   // matrix cores, 64x64 tiles, K chunks of 16 (tile 3) or 32 (tile 4)
     int64_t tiles = (args.nbm > 6 ? (int64_t(1) << (args.nbm - 6)) : 1) *
                     (args.nbn > 6 ? (int64_t(1) << (args.nbn - 6)) : 1) * (int64_t(1) << args.nbh) * args.ksplit;
     int g = (int)(tiles < 1024 ? tiles : 1024);
-    if (prec == QX_C64_3M)
-      hipLaunchKernelGGL((k_contract_gemm_mfma<true>), dim3(g), dim3(256), 0, s, (const float2 *)A,
-                         (const float2 *)B, (float2 *)dst, args);
-    else
-      hipLaunchKernelGGL((k_contract_gemm_mfma<false>), dim3(g), dim3(256), 0, s, (const float2 *)A,
-                         (const float2 *)B, (float2 *)dst, args);
+#define QX_MFMA_GEMM(TT, V2, GA, TKK)                                                                   \
+  hipLaunchKernelGGL((k_contract_gemm_mfma<TT, GA, TKK>), dim3(g), dim3(256), 0, s, (const V2 *)A, \
+                     (const V2 *)B, (V2 *)dst, args)
+    const bool k32 = args.tile == 4;
+    if (prec == QX_C128) {
+      if (k32) QX_MFMA_GEMM(double, double2, false, 32); else QX_MFMA_GEMM(double, double2, false, 16);
+    } else if (prec == QX_C64_3M) {
+      if (k32) QX_MFMA_GEMM(float, float2, true, 32); else QX_MFMA_GEMM(float, float2, true, 16);
+    } else {
+      if (k32) QX_MFMA_GEMM(float, float2, false, 32); else QX_MFMA_GEMM(float, float2, false, 16);
+    }
+#undef QX_MFMA_GEMM
   } else switch (args.tile) {
     case 0: gemm_launch<128, 16, 16>(prec, A, B, dst, args, s); break;
     case 1: gemm_launch<128, 32, 16>(prec, A, B, dst, args, s); break;
@@ -1016,25 +1041,6 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
-
-// 16x16x4 matrix-core products.  Both take A[i][k] from lane i + 16 k and B[k][j]
-// from lane j + 16 k; lane l holds D[row(l / 16, r)][l % 16] in element r, with
-// row = 4 (l / 16) + r for FP32 and 4 r + l / 16 for FP64.
-template <typename T> struct Mfma16;
-template <> struct Mfma16<double> {
-  typedef double v4 __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lk, int r) { return 4 * r + lk; }
-};
-template <> struct Mfma16<float> {
-  typedef float v4 __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
-};
 
 // Step j's S, packed densely as [h][nu][kappa] at Sp + a.st[j].soff, is read with
 // wave-uniform indices (the batch bits h are placed on wave-uniform group bits), so

@@ -114,7 +114,8 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
 }
 
 // Kernel configurations a power-of-two step can run with.
-enum Choice { kBits = 0, kRows = 1, kGemm0 = 2, kGemm1 = 3, kGemm2 = 4, kMfma = 5, kReduce = 6, kNumChoices = 7 };
+enum Choice { kBits = 0, kRows = 1, kGemm0 = 2, kGemm1 = 3, kGemm2 = 4, kMfma = 5, kReduce = 6, kMfma32 = 7,
+              kNumChoices = 8 };
 
 namespace {
 struct CB {
@@ -215,8 +216,8 @@ bool configure_kernel(Step &st, int precision, int choice) {
     case kGemm0:
     case kGemm1:
     case kGemm2:
-    case kMfma: {
-      if (choice == kMfma && precision == QX_C128) return false;
+    case kMfma:
+    case kMfma32: {
       st.kern = Kern::Gemm;
       GemmArgs &g = st.gemm;
       g = GemmArgs{};
@@ -234,8 +235,8 @@ bool configure_kernel(Step &st, int precision, int choice) {
       for (int i = 0; i < nm; ++i) g.ma[i] = mb[i].wa, g.mc[i] = mb[i].wc;
       for (int i = 0; i < nn; ++i) g.nb[i] = nbits[i].wb, g.nc[i] = nbits[i].wc;
       for (int i = 0; i < nk; ++i) g.ka[i] = kbits[i].first, g.kb[i] = kbits[i].second;
-      g.tile = choice - kGemm0;
-      const int lm = g.tile >= 2 ? 6 : 7, ln = g.tile == 0 ? 4 : g.tile == 1 ? 5 : 6, lk = 4;
+      g.tile = choice == kMfma32 ? 4 : choice - kGemm0;
+      const int lm = g.tile >= 2 ? 6 : 7, ln = g.tile == 0 ? 4 : g.tile == 1 ? 5 : 6, lk = g.tile == 4 ? 5 : 4;
       auto min_w = [](const int64_t *w, int n) {
         int64_t m = INT64_MAX;
         for (int i = 0; i < n; ++i) m = std::min(m, w[i]);
@@ -330,6 +331,7 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
   const StepBits sb = step_bits(st);
   std::vector<int> alts;
   if (sb.nk >= 2 && sb.nn >= 2 && sb.nm >= 4) alts.insert(alts.end(), {kGemm0, kGemm1, kGemm2, kMfma});
+  if (sb.nk >= 5 && sb.nn >= 4 && sb.nm >= 4) alts.push_back(kMfma32);
   if (sb.nr >= 8 && rows_ok(sb)) alts.push_back(kRows);
   if (st.c_elems <= (int64_t(1) << 16) && sb.nk >= 6) alts.push_back(kReduce);
   if (sb.nk <= 6 || st.c_elems <= (int64_t(1) << 16)) alts.push_back(kBits);

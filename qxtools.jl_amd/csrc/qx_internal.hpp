@@ -125,7 +125,7 @@ struct SmallDesc {
 // the legs the chain touches plus the fastest untouched legs), applies every S_j
 // to it in LDS and stores the tile of X_m.  The intermediates never reach memory.
 constexpr int kChainTileBits = 13;
-constexpr int kChainMaxSteps = 16;
+constexpr int kChainMaxSteps = 32;
 constexpr int kChainMinPassive = 3;   // untouched fast legs kept in the tile
 constexpr int kChainMaxS = 1024;      // elements of one staged small operand
 

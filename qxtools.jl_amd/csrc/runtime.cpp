@@ -712,6 +712,7 @@ std::string kern_variant(const qx::Step &st) {
     }
     case qx::Kern::Gemm:
       if (st.gemm.tile == 3) return "k_contract_gemm_mfma";
+      if (st.gemm.tile == 4) return "k_contract_gemm_mfma<32>";
       return st.gemm.tile == 0 ? "k_contract_gemm<128,16,16>" : st.gemm.tile == 1 ? "k_contract_gemm<128,32,16>"
                                                                                    : "k_contract_gemm<64,64,16>";
     case qx::Kern::Reduce: return "k_contract_reduce";
@@ -1068,7 +1069,7 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
     } else if (n == "chains") {
       plan->opt_chains = value;
     } else if (n == "kernel") {
-      if (value < -1 || value > 6) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..6");
+      if (value < -1 || value > 7) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..7");
       plan->opt_kernel = value;
     } else if (n == "profile") {
       // may be toggled at any time; every set clears the accumulated step times

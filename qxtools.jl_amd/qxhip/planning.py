@@ -324,17 +324,56 @@ def fuse_gates(gates: list, max_qubits: int) -> list:
     return emitted
 
 
-def sweep_contraction_plan(tnc, max_block_qubits: int = 5, output_groups: int = 2) -> list:
+def stage_blocks(blocks: list, stage_qubits: int) -> list:
+    """Reorder gate blocks [(names, qubits)] into stages that each touch at most
+    `stage_qubits` qubits, keeping every qubit's blocks in their order.  Greedy: a
+    stage takes ready blocks that fit, most already-touched qubits first (then the
+    earliest).  Consecutive blocks of a stage can then run as one pass over the
+    state (the executor's chain passes touch at most that many legs per pass)."""
+    n = len(blocks)
+    last: dict = {}
+    preds = []
+    for i, (_, qs) in enumerate(blocks):
+        preds.append({last[q] for q in qs if q in last})
+        for q in qs:
+            last[q] = i
+    done = [False] * n
+    left = set(range(n))
+    order = []
+    while left:
+        touched: set = set()
+        while True:
+            ready = [i for i in left if all(done[p] for p in preds[i])]
+            fit = [i for i in ready if len(touched | set(blocks[i][1])) <= stage_qubits]
+            if not fit:
+                if not touched and ready:          # a block wider than a stage runs alone
+                    fit = [min(ready)]
+                else:
+                    break
+            pick = max(fit, key=lambda i: (len(touched & set(blocks[i][1])), -i))
+            order.append(pick)
+            done[pick] = True
+            left.discard(pick)
+            touched |= set(blocks[pick][1])
+    return [blocks[i] for i in order]
+
+
+def sweep_contraction_plan(tnc, max_block_qubits: int = 5, output_groups: int = 2,
+                           stage_qubits: int | None = 10) -> list:
     """Circuit-aware plan for a closed network: fuse gates into blocks of at most
-    `max_block_qubits` qubits, contract each block, apply blocks in time order to the
-    growing state (merging state components they touch), then absorb the output
-    tensors last in `output_groups` outer-product groups.  Bitstring-independent
-    work (everything before the outputs) is thereby one shared subtree."""
+    `max_block_qubits` qubits, contract each block, apply blocks to the growing
+    state (merging state components they touch), then absorb the output tensors
+    last in `output_groups` outer-product groups.  Bitstring-independent work
+    (everything before the outputs) is thereby one shared subtree.  Blocks are
+    applied in stages of at most `stage_qubits` qubits (None: time order)."""
     b = _PlanBuilder(tnc)
     comp_of: dict = {}        # qubit -> current state component name
     for q, name in enumerate(tnc.input_tensors, start=1):
         comp_of[q] = name
-    for names, qubits in fuse_gates(tnc.gate_tensors, max_block_qubits):
+    blocks = fuse_gates(tnc.gate_tensors, max_block_qubits)
+    if stage_qubits:
+        blocks = stage_blocks(blocks, stage_qubits)
+    for names, qubits in blocks:
         blk = b.chain(list(names))
         comps = []
         for q in qubits:

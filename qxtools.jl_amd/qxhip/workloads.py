@@ -72,7 +72,7 @@ WORKLOADS = {w.name: w for w in [
     # plain method, small: 3x3 grid, depth 16, 64 bitstrings, complex128, no slicing
     Workload("rqc_3x3_d16", 3, 3, 16, 64, "c128"),
     # plain method at scale: 5x5 grid, depth 20, 256 bitstrings, complex128, no slicing
-    Workload("rqc_5x5_d20", 5, 5, 20, 256, "c128"),
+    Workload("rqc_5x5_d20", 5, 5, 20, 256, "c128", plan_kwargs={"max_block_qubits": 4}),
     # precision: 6x6 grid, complex64 storage, 2^6 slices, 512 bitstrings.  Depth 12
     # instead of 24: at depth 24 the planners here reach width 39 (2^58 flops per
     # amplitude), beyond a single-GPU benchmark.

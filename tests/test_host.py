@@ -174,7 +174,7 @@ def test_committed_workloads_match_their_generator():
         dsl, data, bits = w.load()
         assert bits == w.bitstrings()
         tnc = convert_to_tnc(w.circuit())
-        plan = planning.sweep_contraction_plan(tnc)
+        plan = planning.sweep_contraction_plan(tnc, **w.plan_kwargs)
         body = lambda t: [ln for ln in t.splitlines() if ln and not ln.startswith("#")]
         assert body(build_compute_graph(tnc, plan).to_dsl()) == body(dsl)
 
@@ -185,3 +185,25 @@ def test_config1_workload_oracle_vs_statevector():
     amps = O.amplitudes(dsl, data, bits[:16])
     sv = qxtest.statevector_amps(w.circuit(), bits[:16])
     assert np.max(np.abs(amps - sv) / np.abs(sv)) < 1e-12
+
+
+def test_stage_blocks_keeps_qubit_order_and_limits_stages():
+    circ = circuits.create_rqc_circuit(4, 4, 12, seed=3)
+    tnc = convert_to_tnc(circ)
+    blocks = planning.fuse_gates(tnc.gate_tensors, 4)
+    staged = planning.stage_blocks(blocks, 8)
+    assert sorted(map(str, staged)) == sorted(map(str, blocks))
+    pos = {tuple(b[0]): i for i, b in enumerate(staged)}
+    last = {}
+    for names, qs in blocks:                     # every qubit keeps its block order
+        for q in qs:
+            if q in last:
+                assert pos[last[q]] < pos[tuple(names)]
+            last[q] = tuple(names)
+    # greedy stages: a new stage starts when the next block does not fit
+    touched = set()
+    for names, qs in staged:
+        if len(touched | set(qs)) > 8:
+            touched = set()
+        touched |= set(qs)
+        assert len(touched) <= 8

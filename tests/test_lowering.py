@@ -159,9 +159,8 @@ def test_autotune_candidates(name):
         c = st["candidates"]
         assert c[0] == st["config"] and len(set(c)) == len(c)
         if st["kernel"] == "gemm":
-            assert {"k_contract_gemm<128,16,16>", "k_contract_gemm<128,32,16>", "k_contract_gemm<64,64,16>"} <= set(c)
-        if prec == "c128":
-            assert "k_contract_gemm_mfma" not in c
+            assert {"k_contract_gemm<128,16,16>", "k_contract_gemm<128,32,16>", "k_contract_gemm<64,64,16>",
+                    "k_contract_gemm_mfma"} <= set(c)
 
 
 def test_chains_fold_gate_block_runs():
@@ -172,7 +171,7 @@ def test_chains_fold_gate_block_runs():
     chains = [s for s in info["steps"] if s["kernel"] == "chain"]
     assert len(chains) >= 10 and all(s["fused_steps"] >= 2 for s in chains)
     big = [s for s in info["steps"] if s["bytes"] > 1e8]
-    assert len(big) <= 32                      # 72 passes without chains
+    assert len(big) <= 20                      # 72 passes without chains or stages
     for s in chains:   # launch configurations of the one chain kernel
         assert all(c.startswith("k_contract_chain/") for c in s["candidates"]) and len(s["candidates"]) >= 3
 
