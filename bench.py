@@ -195,6 +195,8 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=1, help="replay each call from a captured hipGraph")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64 comparison")
     ap.add_argument("--grad", action="store_true", help="gradient / forward time of the parameterised workload")
+    ap.add_argument("--batch-last", type=int, default=None,
+                    help="lay the bitstring batch leg out slowest in intermediates (default: per workload)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="time every candidate kernel per step in one untimed call first, keep the fastest")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -218,8 +220,9 @@ def main() -> int:
     prec = a.precision or w.precision
     dsl, data, bits = w.load()
     log(f"workload {w.name}: {len(bits)} bitstrings, precision {prec}")
+    batch_last = w.batch_last if a.batch_last is None else a.batch_last
     ex = Executor(dsl, data, precision=prec, device=0, batch=a.batch or min(256, len(bits)), graph=a.graph,
-                  autotune=a.autotune)
+                  autotune=a.autotune, batch_last=batch_last)
     if a.autotune:
         t_tune = time.perf_counter()
         ex.amplitudes(bits)
@@ -274,6 +277,7 @@ def main() -> int:
         "config": {"workload": w.name, "qubits": w.qubits, "depth": w.depth, "bitstrings": len(bits),
                    "amplitudes_per_step": per_call, "note": w.note or None,
                    "sliced_bonds": w.sliced, "precision": prec, "autotune": bool(a.autotune),
+                   "batch_leg_last": bool(batch_last),
                    "batch": next(b for b in stats["batches"] if b.get("active"))["batch"]},
         "roofline": roof,
         "plan_roofline_frac": plan_roofline(stats, prec, ms_step, a.steps),

@@ -732,7 +732,7 @@ __global__ void k_add_mapped(typename Vec<TD>::type *__restrict__ dst, const typ
 }
 
 template <typename T>
-__global__ void k_root_cot(typename Vec<T>::type *__restrict__ dst, int dep, int64_t batch, int64_t nvalid,
+__global__ void k_root_cot(typename Vec<T>::type *__restrict__ dst, int64_t dep, int64_t batch, int64_t nvalid,
                            int64_t relems, const int64_t *__restrict__ relem_off, const double *__restrict__ amp,
                            int64_t row0) {
   using V = typename Vec<T>::type;
@@ -745,7 +745,7 @@ __global__ void k_root_cot(typename Vec<T>::type *__restrict__ dst, int dep, int
         const double *p = amp + 2 * ((row0 + beta) * relems + e);
         v = V{(T)p[0], (T)p[1]};
       }
-      dst[relem_off[e] + beta] = v;
+      dst[relem_off[e] + beta * dep] = v;
     }
   } else {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < relems;
@@ -795,7 +795,7 @@ __device__ __forceinline__ void neumaier(double &s, double &c, double x) {
 }
 
 template <typename T>
-__global__ void k_accumulate(const typename Vec<T>::type *__restrict__ root, int root_dep, int64_t batch,
+__global__ void k_accumulate(const typename Vec<T>::type *__restrict__ root, int64_t root_bstride, int64_t batch,
                              int64_t nvalid, int64_t relems, const int64_t *__restrict__ relem_off,
                              double *__restrict__ acc, int64_t row0) {
   int64_t total = nvalid * relems;
@@ -803,7 +803,7 @@ __global__ void k_accumulate(const typename Vec<T>::type *__restrict__ root, int
        t += (int64_t)gridDim.x * blockDim.x) {
     int64_t beta = t % nvalid;
     int64_t e = t / nvalid;
-    auto v = root[relem_off[e] + (root_dep ? beta : 0)];
+    auto v = root[relem_off[e] + beta * root_bstride];
     double *a = acc + ((row0 + beta) * relems + e) * 4;
     double s0 = a[0], s1 = a[1], c0 = a[2], c1 = a[3];
     neumaier(s0, c0, (double)v.x);
@@ -1454,15 +1454,15 @@ void launch_fill_outputs(int prec, void *arena, const int64_t *d_out_off, int n_
   check_launch("fill_outputs");
 }
 
-void launch_accumulate(int prec, const void *root, int root_dep, int64_t batch, int64_t nvalid, int64_t relems,
+void launch_accumulate(int prec, const void *root, int64_t root_bstride, int64_t batch, int64_t nvalid, int64_t relems,
                        const int64_t *d_relem_off, double *acc, int64_t row0, void *stream) {
   int g = grid_for(nvalid * relems, 256);
   hipStream_t s = (hipStream_t)stream;
   if (prec == QX_C128)
-    hipLaunchKernelGGL(k_accumulate<double>, dim3(g), dim3(256), 0, s, (const double2 *)root, root_dep, batch,
+    hipLaunchKernelGGL(k_accumulate<double>, dim3(g), dim3(256), 0, s, (const double2 *)root, root_bstride, batch,
                        nvalid, relems, d_relem_off, acc, row0);
   else
-    hipLaunchKernelGGL(k_accumulate<float>, dim3(g), dim3(256), 0, s, (const float2 *)root, root_dep, batch,
+    hipLaunchKernelGGL(k_accumulate<float>, dim3(g), dim3(256), 0, s, (const float2 *)root, root_bstride, batch,
                        nvalid, relems, d_relem_off, acc, row0);
   check_launch("accumulate");
 }
@@ -1489,7 +1489,7 @@ void launch_add_mapped(int prec, int dst_double, void *dst, const void *src, int
   check_launch("add_mapped");
 }
 
-void launch_root_cot(int prec, void *dst, int dep, int64_t batch, int64_t nvalid, int64_t relems,
+void launch_root_cot(int prec, void *dst, int64_t dep, int64_t batch, int64_t nvalid, int64_t relems,
                      const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream) {
   int g = grid_for(dep ? batch * relems : relems, 256);
   hipStream_t s = (hipStream_t)stream;

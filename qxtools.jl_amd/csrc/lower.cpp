@@ -777,7 +777,8 @@ void level_small_steps(const Program &p, Lowered &L) {
 
 }  // namespace
 
-Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uint64_t fused, bool chains) {
+Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uint64_t fused, bool chains,
+              bool batch_last) {
   if (!pow2(batch)) throw Error(QX_ERR_ARGUMENT, "batch must be a power of two");
   Lowered L;
   L.batch = batch;
@@ -829,13 +830,17 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
         v.dims = shape[i];
         v.dep = n.dep;
         int64_t s = 1;
-        if (n.dep) {
+        if (n.dep && !batch_last) {
           v.beta_stride = 1;
           s = batch;
         }
         for (int64_t d : shape[i]) {
           v.strides.push_back(d > 1 ? s : 0);
           if (d > 1) s *= d;
+        }
+        if (n.dep && batch_last) {
+          v.beta_stride = s;
+          s *= batch;
         }
         v.elems = s;
         break;
@@ -957,9 +962,9 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
   return L;
 }
 
-VjpLowered lower_vjp(const Program &p, int64_t batch, int precision, uint64_t fused) {
+VjpLowered lower_vjp(const Program &p, int64_t batch, int precision, uint64_t fused, bool batch_last) {
   VjpLowered V;
-  V.fwd = lower(p, batch, true, precision, fused);
+  V.fwd = lower(p, batch, true, precision, fused, false, batch_last);
   const Lowered &L = V.fwd;
   const size_t nn = p.nodes.size();
   // every tensor must feed exactly one consumer (a tree), so each cotangent has

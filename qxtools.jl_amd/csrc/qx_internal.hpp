@@ -210,8 +210,10 @@ struct Lowered {
   int64_t result_elems = 1;         // complex values per bitstring
 };
 
+// batch_last: lay the batched bitstring leg out slowest in bitstring-dependent
+// intermediates (default fastest); outputs keep it fastest.
 Lowered lower(const Program &p, int64_t batch, bool keep_all = false, int precision = QX_C128,
-              uint64_t fused = 0, bool chains = true);
+              uint64_t fused = 0, bool chains = true, bool batch_last = false);
 
 // C[clab] = sum A[alab] B[blab] over the given value layouts (C dense), kernel
 // chosen and arguments filled; conj flags conjugate an operand on load.
@@ -246,7 +248,8 @@ struct VjpLowered {
   int root = -1;
 };
 
-VjpLowered lower_vjp(const Program &p, int64_t batch, int precision = QX_C128, uint64_t fused = 0);
+VjpLowered lower_vjp(const Program &p, int64_t batch, int precision = QX_C128, uint64_t fused = 0,
+                     bool batch_last = false);
 
 // First-fit offsets for [start, end) intervals of given sizes (elements), 256-byte
 // aligned for element size `esize`.  Returns the total size; offsets written out.
@@ -277,7 +280,7 @@ void launch_contract_radix(int prec, const void *A, const void *B, void *C, int6
 void launch_fill_outputs(int prec, void *arena, const int64_t *d_out_off, int n_out,
                          const int *d_out_q, const char *d_bits, int64_t nvalid, int64_t batch,
                          int64_t num_chars, void *stream);
-void launch_accumulate(int prec, const void *root, int root_dep, int64_t batch, int64_t nvalid,
+void launch_accumulate(int prec, const void *root, int64_t root_bstride, int64_t batch, int64_t nvalid,
                        int64_t relems, const int64_t *d_relem_off, double *acc, int64_t row0,
                        void *stream);
 void launch_finalize(const double *acc, double *out, int64_t n, void *stream);
@@ -286,7 +289,7 @@ void launch_add_mapped(int prec, int dst_double, void *dst, const void *src, int
                        const std::vector<int64_t> &dims, const std::vector<int64_t> &strides,
                        void *stream);
 // cotangent of the saved tensor from amp_cot rows [row0, row0 + nvalid)
-void launch_root_cot(int prec, void *dst, int dep, int64_t batch, int64_t nvalid, int64_t relems,
+void launch_root_cot(int prec, void *dst, int64_t dep, int64_t batch, int64_t nvalid, int64_t relems,
                      const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream);
 void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream);
 void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args, void *stream);

@@ -88,6 +88,7 @@ struct qx_plan {
   int64_t opt_autotune = 0;
   int64_t opt_kernel = -1;                  // >= 0: force that kernel wherever it can run
   int64_t opt_chains = 1;                   // fold runs of gate-block steps into chain passes
+  int64_t opt_batch_last = 0;               // batched leg slowest in bitstring-dependent values
   bool finalized = false;
 
   std::vector<DevBuf> data;                 // per key
@@ -151,7 +152,7 @@ const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch, uint64_t fused = 0)
   auto key = std::make_pair(batch, fused);
   auto it = P->lowered.find(key);
   if (it != P->lowered.end()) return it->second;
-  qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, fused, P->opt_chains != 0);
+  qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, fused, P->opt_chains != 0, P->opt_batch_last != 0);
   if (P->opt_kernel >= 0)
     for (auto &st : L.steps)
       if (qx::force_kernel(st, P->prec, (int)P->opt_kernel)) L.ws_elems = std::max(L.ws_elems, st.ws_elems);
@@ -414,10 +415,10 @@ uint64_t choose_fused(qx_plan *P, int64_t batch, bool for_vjp) {
     try {
       int64_t need;
       if (for_vjp) {
-        qx::VjpLowered V = qx::lower_vjp(P->prog, batch, P->prec, mask);
+        qx::VjpLowered V = qx::lower_vjp(P->prog, batch, P->prec, mask, P->opt_batch_last != 0);
         need = (V.fwd.arena_elems + V.fwd.ws_elems + V.cot_elems + V.temp_elems) * es;
       } else {
-        qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, mask, P->opt_chains != 0);
+        qx::Lowered L = qx::lower(P->prog, batch, false, P->prec, mask, P->opt_chains != 0, P->opt_batch_last != 0);
         need = (L.arena_elems + L.ws_elems) * es;
       }
       if (need <= budget) {
@@ -573,7 +574,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
         qx::launch_fill_outputs(P->prec, P->arena.p, (const int64_t *)P->out_off.p, nout, (const int *)P->out_q.p,
                                 (const char *)P->bits.p + row0 * nch, nvalid, batch, nch, stream);
         run_section(2);
-        qx::launch_accumulate(P->prec, value_ptr(P, rv, varval), rv.dep ? 1 : 0, batch, nvalid, R,
+        qx::launch_accumulate(P->prec, value_ptr(P, rv, varval), rv.dep ? rv.beta_stride : 0, batch, nvalid, R,
                               (const int64_t *)P->relem_off.p, (double *)P->acc.p, row0, stream);
       }
     }
@@ -780,7 +781,7 @@ const qx::VjpLowered &ensure_vjp(qx_plan *P, int64_t batch, uint64_t fused) {
   auto key = std::make_pair(batch, fused);
   auto it = P->vjp_lowered.find(key);
   if (it != P->vjp_lowered.end()) return it->second;
-  qx::VjpLowered V = qx::lower_vjp(P->prog, batch, P->prec, fused);
+  qx::VjpLowered V = qx::lower_vjp(P->prog, batch, P->prec, fused, P->opt_batch_last != 0);
   if (P->opt_kernel >= 0) {
     auto force = [&](qx::Step &st) {
       if (qx::force_kernel(st, P->prec, (int)P->opt_kernel)) V.fwd.ws_elems = std::max(V.fwd.ws_elems, st.ws_elems);
@@ -917,7 +918,7 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
       qx::launch_fill_outputs(P->prec, bs.arena, (const int64_t *)P->out_off.p, nout, (const int *)P->out_q.p,
                               (const char *)P->bits.p + row0 * nch, nvalid, batch, nch, stream);
       run_fwd(2);
-      qx::launch_root_cot(P->prec, value_ptr(P, rc, varval, bs), rc.dep ? 1 : 0, batch, nvalid, R,
+      qx::launch_root_cot(P->prec, value_ptr(P, rc, varval, bs), rc.dep ? rc.beta_stride : 0, batch, nvalid, R,
                           (const int64_t *)P->relem_off.p, (const double *)P->ampcot.p, row0, stream);
       run_bwd(2);
     }
@@ -1068,6 +1069,8 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
       plan->opt_autotune = value;
     } else if (n == "chains") {
       plan->opt_chains = value;
+    } else if (n == "batch_last") {
+      plan->opt_batch_last = value;
     } else if (n == "kernel") {
       if (value < -1 || value > 7) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..7");
       plan->opt_kernel = value;

@@ -37,6 +37,7 @@ class Workload:
     plan_kwargs: dict = field(default_factory=dict)
     open_outputs: tuple = ()          # qubits left open: 2^m amplitudes per bitstring
     note: str = ""
+    batch_last: int = 0               # executor layout option (batched leg slowest)
 
     @property
     def qubits(self) -> int:
@@ -78,12 +79,12 @@ WORKLOADS = {w.name: w for w in [
     # amplitude), beyond a single-GPU benchmark.
     Workload("rqc_6x6_d12_s6", 6, 6, 12, 512, "c64_4m", sliced=6,
              plan_kwargs={"trials": 16, "min_fill_trials": 4},
-             note="depth reduced from 24 to 12"),
+             note="depth reduced from 24 to 12", batch_last=1),
     # batching: 7x7 grid, 2^10 slices, 5 output legs left open so each of 32 bitstring
     # prefixes yields 32 amplitudes (1024 in all).  Depth 12 instead of 24 (width).
     Workload("rqc_7x7_d12_s10_open5", 7, 7, 12, 32, "c64_4m", sliced=10,
              plan_kwargs={"trials": 16, "min_fill_trials": 4}, open_outputs=(45, 46, 47, 48, 49),
-             note="depth reduced from 24 to 12"),
+             note="depth reduced from 24 to 12", batch_last=1),
 ]}
 
 # differentiation: parameterised 6x6 grid, Rz Ry Rz per qubit per cycle, 2^4 slices,

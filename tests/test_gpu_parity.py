@@ -185,3 +185,22 @@ def test_autotune_keeps_results(nslice):
     assert timed and all(s["config"] in s["tuning_ms"] for s in timed)
     for s in timed:   # the kept configuration is the fastest measured
         assert s["tuning_ms"][s["config"]] <= min(s["tuning_ms"].values()) + 1e-12
+
+
+@pytest.mark.parametrize("planner,nslice", [("sweep", 0), ("greedy", 3)])
+def test_batch_leg_last_layout(planner, nslice):
+    """The batched bitstring leg laid out slowest in intermediates gives the same
+    amplitudes and the same pullback."""
+    circ = circuits.create_rqc_circuit(3, 3, 10, seed=17)
+    t, plan, g = qxtest.graph_for(circ, planner=planner, nslice=nslice)
+    dsl = g.to_dsl()
+    bits = simulation.amplitudes_uniform(9, 3, 20)
+    ref = O.amplitudes(dsl, g.data, bits)
+    a = Executor(dsl, g.data, batch=8, batch_last=1)
+    b = Executor(dsl, g.data, batch=8)
+    out = a.amplitudes(bits)
+    assert np.max(qxtest.rel_err(out, ref)) <= TOL64
+    cot = np.conj(out)
+    ga, gb = a.vjp(bits, cot), b.vjp(bits, cot)
+    for k in gb:
+        assert np.allclose(ga[k], gb[k], rtol=1e-11, atol=1e-13), k
