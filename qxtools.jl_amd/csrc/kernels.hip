@@ -1329,6 +1329,28 @@ __global__ __launch_bounds__(256) void k_small_batch(const SmallDesc *__restrict
   }
 }
 
+// Batched scatter-add: workgroup i adds descriptor i's dense temp into its FP64
+// destination through the destination's strides.  Destinations of one launch are
+// disjoint.
+template <typename TS>
+__global__ __launch_bounds__(256) void k_add_batch(const AddDesc *__restrict__ d) {
+  using VS = typename Vec<TS>::type;
+  const AddDesc &a = d[blockIdx.x];
+  const VS *src = (const VS *)a.src;
+  double2 *dst = (double2 *)a.dst;
+  for (int64_t e = threadIdx.x; e < a.n; e += blockDim.x) {
+    int64_t r = e, o = 0;
+    for (int j = 0; j < a.nd; ++j) {
+      const int64_t q = r % a.dim[j];
+      r /= a.dim[j];
+      o += q * a.st[j];
+    }
+    const VS v = src[e];
+    dst[o].x += (double)v.x;
+    dst[o].y += (double)v.y;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Output sampling on the device.  Counter-based Philox4x64-10 (Salmon, Moraes,
 // Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC 2011): candidate j
@@ -1553,6 +1575,16 @@ void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream) 
   else
     hipLaunchKernelGGL(k_small_batch<float>, dim3(n), dim3(256), 0, s, d_desc);
   check_launch("small_batch");
+}
+
+void launch_add_batch(int prec, const AddDesc *d_desc, int n, void *stream) {
+  if (n <= 0) return;
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == QX_C128)
+    hipLaunchKernelGGL(k_add_batch<double>, dim3(n), dim3(256), 0, s, d_desc);
+  else
+    hipLaunchKernelGGL(k_add_batch<float>, dim3(n), dim3(256), 0, s, d_desc);
+  check_launch("add_batch");
 }
 
 }  // namespace qx

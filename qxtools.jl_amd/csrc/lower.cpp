@@ -707,7 +707,7 @@ namespace {
 // with at most kSmallMaxBits C and K bits each, a few thousand MACs, and operand
 // offsets that fit 32 bits.
 bool small_step(const Step &st) {
-  if (st.section != 0 || st.kern == Kern::Chain || st.kern == Kern::Radix) return false;
+  if (st.section == 1 || st.kern == Kern::Chain || st.kern == Kern::Radix) return false;
   if (st.c_elems > 4096 || st.K > 4096 || st.c_elems * st.K > (int64_t(1) << 18)) return false;
   int nbc = 0, nbk = 0;
   int64_t wmax = 0;
@@ -724,9 +724,12 @@ bool small_step(const Step &st) {
   return nbc <= kSmallMaxBits && nbk <= kSmallMaxBits && wmax < (int64_t(1) << 30);
 }
 
-// Hoist the slice-free small steps whose inputs are data or other such steps to
-// the front of section 0, grouped by dependency level (a level's steps are
-// independent of each other).  The order stays topological; liveness is computed
+// Hoist the small steps of section 0 (once per call) and of section 2 (once per
+// bitstring batch) whose inputs are available at the section's start (data, values
+// of earlier sections, bitstring tensors, or other such steps) to the front of
+// their section, grouped by dependency level (a level's steps are independent of
+// each other).  Steps reading slice-dependent views are left alone (their
+// addresses change per slice).  The order stays topological; liveness is computed
 // on the new order afterwards.
 void level_small_steps(const Program &p, Lowered &L) {
   const size_t nn = p.nodes.size();
@@ -741,14 +744,15 @@ void level_small_steps(const Program &p, Lowered &L) {
   for (size_t s = 0; s < L.steps.size(); ++s) {
     const Step &st = L.steps[s];
     if (!small_step(st)) continue;
+    if (!L.values[st.in_a].var_terms.empty() || !L.values[st.in_b].var_terms.empty()) continue;
     int lv = 0;
     bool ok = true;
     for (int in : {st.in_a, st.in_b}) {
       const int r = root[in];
       if (r < 0) continue;
       const int ps = prod[r];
-      if (ps < 0) continue;                       // data or bitstring output
-      if (L.steps[ps].section != 0) continue;     // cannot happen for section 0 inputs
+      if (ps < 0) continue;                               // data or bitstring output
+      if (L.steps[ps].section != st.section) continue;   // an earlier section's value
       if (level[ps] < 0) {
         ok = false;
         break;
@@ -760,18 +764,23 @@ void level_small_steps(const Program &p, Lowered &L) {
     maxlev = std::max(maxlev, lv);
   }
   if (maxlev < 0) return;
-  std::vector<int> count(maxlev + 1, 0);
-  for (int lv : level)
-    if (lv >= 0) count[lv]++;
-  std::vector<Step> out;
-  for (int lv = 0; lv <= maxlev; ++lv)
-    for (size_t s = 0; s < L.steps.size(); ++s)
-      if (level[s] == lv) {
-        out.push_back(L.steps[s]);
-        out.back().level = count[lv] > 1 ? lv : -1;   // a lone step keeps its own kernel
-      }
+  // levels are numbered per section; section 2's are offset past section 0's
+  const int off2 = maxlev + 1;
+  std::vector<int> count(2 * (maxlev + 1), 0);
   for (size_t s = 0; s < L.steps.size(); ++s)
-    if (level[s] < 0) out.push_back(std::move(L.steps[s]));
+    if (level[s] >= 0) count[level[s] + (L.steps[s].section == 2 ? off2 : 0)]++;
+  std::vector<Step> out;
+  for (int sec : {0, 1, 2}) {
+    for (int lv = 0; lv <= maxlev; ++lv)
+      for (size_t s = 0; s < L.steps.size(); ++s)
+        if (L.steps[s].section == sec && level[s] == lv) {
+          const int key = lv + (sec == 2 ? off2 : 0);
+          out.push_back(L.steps[s]);
+          out.back().level = count[key] > 1 ? key : -1;   // a lone step keeps its own kernel
+        }
+    for (size_t s = 0; s < L.steps.size(); ++s)
+      if (L.steps[s].section == sec && level[s] < 0) out.push_back(L.steps[s]);
+  }
   L.steps = std::move(out);
 }
 
@@ -869,8 +878,8 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
   if (chains && !keep_all) {
     for (int a : form_chains(p, L)) absorbed[a] = 1;
     for (const Step &st : L.steps) L.ws_elems = std::max(L.ws_elems, st.ws_elems);
-    level_small_steps(p, L);
   }
+  if (chains || keep_all) level_small_steps(p, L);
 
   // result elements per bitstring
   L.result_elems = 1;
@@ -1097,6 +1106,61 @@ VjpLowered lower_vjp(const Program &p, int64_t batch, int precision, uint64_t fu
       V.fwd.ws_elems = std::max(V.fwd.ws_elems, b.step.ws_elems);
       V.bsteps[fs.section].push_back(std::move(b));
     }
+  }
+
+  // Backward steps of sections 0 and 2 in dependency levels: a step reads the
+  // cotangent written one level earlier.  Small ones whose operands have no
+  // slice-dependent offsets and that write a cotangent directly, or a temp added
+  // into a data key's gradient that no other step adds into, run batched per level
+  // (each with its own temp); the order is stable per level.
+  std::vector<int> key_uses(p.keys.size(), 0);
+  for (auto &sec : V.bsteps)
+    for (auto &b : sec)
+      if (!b.direct && b.dest == Buf::Grad) key_uses[b.dest_key]++;
+  int level_base = 0;
+  for (int sec : {0, 2}) {
+    auto &bv = V.bsteps[sec];
+    std::map<int, size_t> prod;   // node -> backward step writing its cotangent
+    for (size_t i = 0; i < bv.size(); ++i)
+      if (bv[i].direct) prod[bv[i].x] = i;
+    std::vector<int> lev(bv.size(), 0);
+    int maxlev = 0;
+    for (size_t i = 0; i < bv.size(); ++i) {
+      auto it = prod.find(bv[i].step.node);
+      lev[i] = it == prod.end() ? 0 : lev[it->second] + 1;
+      maxlev = std::max(maxlev, lev[i]);
+    }
+    auto no_slice_terms = [&](const BwdStep &b) {
+      const Node &cn = p.nodes[b.step.node];
+      return L.values[cn.ia].var_terms.empty() && L.values[cn.ib].var_terms.empty() &&
+             L.values[b.x].var_terms.empty();
+    };
+    std::vector<size_t> order(bv.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return lev[a] < lev[b]; });
+    std::vector<BwdStep> sorted;
+    std::map<int, int64_t> level_temp;
+    std::map<int, int> level_count;
+    for (size_t i : order) {
+      BwdStep b = std::move(bv[i]);
+      const bool ok = small_step(b.step) && no_slice_terms(b) &&
+                      (b.direct || (b.dest == Buf::Grad && key_uses[b.dest_key] == 1 &&
+                                    b.add_dims.size() <= (size_t)kSmallMaxBits));
+      if (ok) {
+        b.level = level_base + lev[i];
+        level_count[b.level]++;
+        if (!b.direct) {
+          b.temp_off = level_temp[b.level];
+          level_temp[b.level] += (b.add_elems + 31) / 32 * 32;
+          V.temp_elems = std::max(V.temp_elems, level_temp[b.level]);
+        }
+      }
+      sorted.push_back(std::move(b));
+    }
+    for (auto &b : sorted)   // a lone step keeps its own launch
+      if (b.level >= 0 && level_count[b.level] < 2) b.level = -1, b.temp_off = 0;
+    bv = std::move(sorted);
+    level_base += maxlev + 1;
   }
   return V;
 }

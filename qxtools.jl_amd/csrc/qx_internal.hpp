@@ -119,6 +119,16 @@ struct SmallDesc {
   int32_t ka[kSmallMaxBits] = {0}, kb[kSmallMaxBits] = {0};   // K bits
 };
 
+// Batched scatter-add of small temps into FP64 gradients: dst[map(e)] += src[e],
+// map(e) = sum of digit_j(e) * st[j] over the dims (first fastest).
+struct AddDesc {
+  void *dst = nullptr;
+  const void *src = nullptr;
+  int64_t n = 0;
+  int32_t nd = 0, pad = 0;
+  int64_t dim[kSmallMaxBits] = {0}, st[kSmallMaxBits] = {0};
+};
+
 // A chain of steps that each contract a small operand (a gate block: at most 2^10
 // elements) into one large tensor X, X_j = S_j . X_{j-1}, run as one pass over X:
 // every workgroup loads a tile of 2^kChainTileBits elements of X_0 (the bits of
@@ -237,6 +247,8 @@ struct BwdStep {
   int dest_key = -1;          // data key (Grad)
   std::vector<int64_t> add_dims, add_strides;   // temp (dense) -> destination
   int64_t add_elems = 0;
+  int level = -1;             // >= 0: runs batched with the other small steps of its level
+  int64_t temp_off = 0;       // offset (elements) of its temp in the temp buffer
 };
 
 struct VjpLowered {
@@ -292,6 +304,7 @@ void launch_add_mapped(int prec, int dst_double, void *dst, const void *src, int
 void launch_root_cot(int prec, void *dst, int64_t dep, int64_t batch, int64_t nvalid, int64_t relems,
                      const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream);
 void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream);
+void launch_add_batch(int prec, const AddDesc *d_desc, int n, void *stream);
 void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args, void *stream);
 // Output sampling: candidate bitstrings from Philox4x64-10 and the sequential
 // empirical-supremum rejection test over one round of candidates.

@@ -105,11 +105,22 @@ struct qx_plan {
   std::vector<hipEvent_t> events;           // pool: 2 per profiled launch
   std::vector<hipEvent_t> tune_events;      // pool: 2 per timed candidate launch
   // descriptors of the batched small steps of one lowering (device table)
-  DevBuf small_desc;
-  const qx::Lowered *small_for = nullptr;
-  void *small_arena = nullptr;
-  std::vector<int> small_index;             // per step: descriptor index, or -1
-  std::vector<int> small_count;             // per step that starts a level: steps in the level
+  struct SmallTable {
+    DevBuf desc;
+    void *arena = nullptr;                  // arena base the descriptors point into
+    std::vector<int> index;                 // per step: descriptor index, or -1
+    std::vector<int> count;                 // per step that starts a level: steps in the level
+  };
+  std::map<const qx::Lowered *, SmallTable> small;   // per lowering (forward or pullback)
+  // batched slice-free backward levels of one pullback lowering
+  struct BwdTable {
+    DevBuf desc, adds;
+    std::vector<char *> bases;              // arena, cot, temp, grad the tables point into
+    std::vector<int> first[3];              // per section and backward step: its level's group, or -1
+    struct Group { int desc0, ndesc, add0, nadd; };
+    std::vector<Group> groups;
+  };
+  std::map<const qx::VjpLowered *, BwdTable> bwd;
   // captured device work of qx_amplitudes per (batch, nb, slice_begin, slice_end)
   std::map<std::tuple<int64_t, int64_t, int64_t, int64_t>, hipGraphExec_t> graphs;
   // captured device work of qx_amplitudes_vjp per (batch, nb, -, fused slice vars)
@@ -139,7 +150,8 @@ struct qx_plan {
     grad.release();
     ampcot.release();
     ws.release();
-    small_desc.release();
+    for (auto &kv : small) kv.second.desc.release();
+    for (auto &kv : bwd) kv.second.desc.release(), kv.second.adds.release();
     for (auto e : events) (void)hipEventDestroy(e);
     for (auto e : tune_events) (void)hipEventDestroy(e);
     drop_graphs();
@@ -218,23 +230,24 @@ void launch_step(qx_plan *P, const qx::Lowered &L, const qx::Step &st, void *A, 
 
 // Build and upload the descriptor table of L's batched small steps (slice-free, so
 // their operand addresses depend only on the arena and data buffers).
-void prepare_small(qx_plan *P, const qx::Lowered &L) {
-  if (P->small_for == &L && P->small_arena == P->arena.p) return;
+void prepare_small(qx_plan *P, const qx::Lowered &L, const Bases &bs) {
+  qx_plan::SmallTable &T = P->small[&L];
+  if (T.arena == bs.arena && !T.index.empty()) return;
   std::vector<qx::SmallDesc> d;
-  P->small_index.assign(L.steps.size(), -1);
-  P->small_count.assign(L.steps.size(), 0);
+  T.index.assign(L.steps.size(), -1);
+  T.count.assign(L.steps.size(), 0);
   const std::vector<int64_t> zero(P->prog.vars.size(), 0);
   int first = -1;
   for (size_t s = 0; s < L.steps.size(); ++s) {
     const qx::Step &st = L.steps[s];
     if (st.level < 0) continue;
     if (first < 0 || L.steps[first].level != st.level) first = (int)s;
-    P->small_count[first] += 1;
-    P->small_index[s] = (int)d.size();
+    T.count[first] += 1;
+    T.index[s] = (int)d.size();
     qx::SmallDesc x;
-    x.A = value_ptr(P, L.values[st.in_a], zero);
-    x.B = value_ptr(P, L.values[st.in_b], zero);
-    x.C = value_ptr(P, L.values[st.node], zero);
+    x.A = value_ptr(P, L.values[st.in_a], zero, bs);
+    x.B = value_ptr(P, L.values[st.in_b], zero, bs);
+    x.C = value_ptr(P, L.values[st.node], zero, bs);
     x.conj_a = st.bits.conj_a;
     x.conj_b = st.bits.conj_b;
     for (auto &g : st.cl)
@@ -254,18 +267,18 @@ void prepare_small(qx_plan *P, const qx::Lowered &L) {
     d.push_back(x);
   }
   if (!d.empty()) {
-    P->small_desc.reserve(d.size() * sizeof(qx::SmallDesc));
-    hip_check(hipMemcpy(P->small_desc.p, d.data(), d.size() * sizeof(qx::SmallDesc), hipMemcpyHostToDevice),
+    T.desc.reserve(d.size() * sizeof(qx::SmallDesc));
+    hip_check(hipMemcpy(T.desc.p, d.data(), d.size() * sizeof(qx::SmallDesc), hipMemcpyHostToDevice),
               "copy descriptors");
   }
-  P->small_for = &L;
-  P->small_arena = P->arena.p;
+  T.arena = bs.arena;
   P->drop_graphs();
 }
 
 // Launch the level of batched small steps that starts at step s.
 void run_small_level(qx_plan *P, const qx::Lowered &L, size_t s) {
   hipStream_t stream = P->ctx->stream;
+  const qx_plan::SmallTable &T = P->small.at(&L);
   size_t ev = 0;
   if (P->opt_profile) {
     ev = P->event_step.size();
@@ -277,10 +290,9 @@ void run_small_level(qx_plan *P, const qx::Lowered &L, size_t s) {
     P->event_step.push_back((int)s);
     hip_check(hipEventRecord(P->events[2 * ev], stream), "hipEventRecord");
   }
-  const qx::SmallDesc *d = (const qx::SmallDesc *)P->small_desc.p + P->small_index[s];
-  qx::launch_small_batch(P->prec, d, P->small_count[s], stream);
+  const qx::SmallDesc *d = (const qx::SmallDesc *)T.desc.p + T.index[s];
+  qx::launch_small_batch(P->prec, d, T.count[s], stream);
   if (P->opt_profile) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
-  (void)L;
 }
 
 void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int64_t> &varval) {
@@ -543,7 +555,11 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   moved |= P->acc.reserve((size_t)nb * R * 32);
   moved |= P->outbuf.reserve((size_t)nb * R * 16);
   if (moved) P->drop_graphs();
-  prepare_small(P, L);
+  {
+    Bases bs;
+    bs.arena = (char *)P->arena.p;
+    prepare_small(P, L, bs);
+  }
 
   P->event_step.clear();
 
@@ -556,7 +572,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
       for (size_t s = 0; s < L.steps.size(); ++s) {
         if (L.steps[s].section != sec) continue;
         if (L.steps[s].level >= 0) {   // batched small steps: one launch per level
-          if (P->small_count[s] > 0) run_small_level(P, L, s);
+          if (P->small.at(&L).count[s] > 0) run_small_level(P, L, s);
           continue;
         }
         if (tuner)
@@ -874,21 +890,104 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
       qx::launch_zero(bs.cot + c.slot * es, elems * es, stream);
     }
   };
+  prepare_small(P, L, bs);
   auto run_fwd = [&](int sec) {
-    for (const qx::Step &st : L.steps)
-      if (st.section == sec)
-        launch_step(P, L, st, value_ptr(P, L.values[st.in_a], varval, bs),
-                    value_ptr(P, L.values[st.in_b], varval, bs), value_ptr(P, L.values[st.node], varval, bs), varval,
-                    &bs);
+    for (size_t s = 0; s < L.steps.size(); ++s) {
+      const qx::Step &st = L.steps[s];
+      if (st.section != sec) continue;
+      if (st.level >= 0) {   // batched small steps: one launch per level
+        if (P->small.at(&L).count[s] > 0) run_small_level(P, L, s);
+        continue;
+      }
+      launch_step(P, L, st, value_ptr(P, L.values[st.in_a], varval, bs),
+                  value_ptr(P, L.values[st.in_b], varval, bs), value_ptr(P, L.values[st.node], varval, bs), varval,
+                  &bs);
+    }
   };
+  // operands of a backward step: (A, B, C) of its contraction
+  auto bwd_ptrs = [&](const qx::BwdStep &b, void *&A, void *&B, void *&C) {
+    const qx::Node &cn = P->prog.nodes[b.step.node];
+    const qx::Value &cv = V.cot[b.step.node];
+    const bool xa = b.x == cn.ia;
+    A = xa ? value_ptr(P, cv, varval, bs) : value_ptr(P, L.values[cn.ia], varval, bs);
+    B = xa ? value_ptr(P, L.values[cn.ib], varval, bs) : value_ptr(P, cv, varval, bs);
+    C = b.direct ? value_ptr(P, V.cot[b.x], varval, bs) : (void *)(bs.temp + b.temp_off * es);
+  };
+  // descriptor tables of the batched backward levels (sections 0 and 2; their
+  // operands have no slice-dependent offsets)
+  qx_plan::BwdTable &BT = P->bwd[&V];
+  {
+    std::vector<char *> bases = {bs.arena, bs.cot, bs.temp, (char *)P->grad.p};
+    if (BT.bases != bases) {
+      std::vector<qx::SmallDesc> d;
+      std::vector<qx::AddDesc> ad;
+      BT.groups.clear();
+      for (int sec = 0; sec < 3; ++sec) {
+        const auto &bv = V.bsteps[sec];
+        BT.first[sec].assign(bv.size(), -1);
+        int cur_level = -1;
+        for (size_t i = 0; i < bv.size(); ++i) {
+          const qx::BwdStep &b = bv[i];
+          if (b.level < 0) continue;
+          if (b.level != cur_level) {   // the level's first batched step launches the group
+            cur_level = b.level;
+            BT.groups.push_back({(int)d.size(), 0, (int)ad.size(), 0});
+            BT.first[sec][i] = (int)BT.groups.size() - 1;
+          }
+          auto &g = BT.groups.back();
+          qx::SmallDesc x;
+          void *A, *B, *C;
+          bwd_ptrs(b, A, B, C);
+          x.A = A, x.B = B, x.C = C;
+          x.conj_a = b.step.bits.conj_a;
+          x.conj_b = b.step.bits.conj_b;
+          for (auto &l : b.step.cl)
+            for (int64_t m = 1; m < l.dim; m <<= 1)
+              x.wa[x.nbc] = (int32_t)(l.sa * m), x.wb[x.nbc] = (int32_t)(l.sb * m), ++x.nbc;
+          for (auto &l : b.step.kl)
+            for (int64_t m = 1; m < l.dim; m <<= 1)
+              x.ka[x.nbk] = (int32_t)(l.sa * m), x.kb[x.nbk] = (int32_t)(l.sb * m), ++x.nbk;
+          d.push_back(x);
+          g.ndesc++;
+          if (!b.direct) {
+            qx::AddDesc a;
+            a.src = C;
+            a.dst = (char *)P->grad.p + (key_off[b.dest_key] + value_offset(L.values[b.x], varval)) * 16;
+            a.n = b.add_elems;
+            a.nd = (int32_t)b.add_dims.size();
+            for (int j = 0; j < a.nd; ++j) a.dim[j] = b.add_dims[j], a.st[j] = b.add_strides[j];
+            ad.push_back(a);
+            g.nadd++;
+          }
+        }
+      }
+      if (!d.empty()) {
+        BT.desc.reserve(d.size() * sizeof(qx::SmallDesc));
+        hip_check(hipMemcpy(BT.desc.p, d.data(), d.size() * sizeof(qx::SmallDesc), hipMemcpyHostToDevice), "copy");
+      }
+      if (!ad.empty()) {
+        BT.adds.reserve(ad.size() * sizeof(qx::AddDesc));
+        hip_check(hipMemcpy(BT.adds.p, ad.data(), ad.size() * sizeof(qx::AddDesc), hipMemcpyHostToDevice), "copy");
+      }
+      BT.bases = bases;
+      P->drop_graphs();
+    }
+  }
   auto run_bwd = [&](int sec) {
-    for (const qx::BwdStep &b : V.bsteps[sec]) {
+    for (size_t bi = 0; bi < V.bsteps[sec].size(); ++bi) {
+      const qx::BwdStep &b = V.bsteps[sec][bi];
+      if (b.level >= 0) {   // batched level: contractions, then adds into gradients
+        if (BT.first[sec][bi] >= 0) {
+          const auto &g = BT.groups[BT.first[sec][bi]];
+          qx::launch_small_batch(P->prec, (const qx::SmallDesc *)BT.desc.p + g.desc0, g.ndesc, stream);
+          qx::launch_add_batch(P->prec, (const qx::AddDesc *)BT.adds.p + g.add0, g.nadd, stream);
+        }
+        continue;
+      }
       const qx::Node &cn = P->prog.nodes[b.step.node];
-      const qx::Value &cv = V.cot[b.step.node];
-      const bool xa = b.x == cn.ia;
-      void *A = xa ? value_ptr(P, cv, varval, bs) : value_ptr(P, L.values[cn.ia], varval, bs);
-      void *B = xa ? value_ptr(P, L.values[cn.ib], varval, bs) : value_ptr(P, cv, varval, bs);
-      void *C = b.direct ? value_ptr(P, V.cot[b.x], varval, bs) : (void *)bs.temp;
+      void *A, *B, *C;
+      bwd_ptrs(b, A, B, C);
+      (void)cn;
       launch_step(P, L, b.step, A, B, C, varval, &bs);
       if (b.direct) continue;
       const int64_t xoff = value_offset(L.values[b.x], varval);
