@@ -1142,11 +1142,29 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
       if ((g >> j) & 1) base |= 1u << s.hq[i];
     return base;
   };
-  int h = 0;
+  // the low four group bits (a row inside a 16-row block) go to the first four
+  // free tile bits: a row's base is its block's base plus this scatter
+  uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
   {
-    const uint32_t b0 = deposit((uint32_t)g0);
-    for (int i = 0; i < s.h; ++i) h |= (int)((b0 >> s.hq[i]) & 1) << i;
+    int j = 0;
+    for (int b = 0; b < kChainTileBits && j < 4; ++b)
+      if (!(((kmask | hmask) >> b) & 1)) {
+        const uint32_t bit = 1u << b;
+        if (j == 0) p0 = bit;
+        else if (j == 1) p1 = bit;
+        else if (j == 2) p2 = bit;
+        else p3 = bit;
+        ++j;
+      }
   }
+  auto scatter4 = [&](int row) {
+    return ((row & 1) ? p0 : 0u) | ((row & 2) ? p1 : 0u) | ((row & 4) ? p2 : 0u) | ((row & 8) ? p3 : 0u);
+  };
+  uint32_t rbase[RT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) rbase[rt] = deposit((uint32_t)(g0 + 16 * rt));
+  int h = 0;
+  for (int i = 0; i < s.h; ++i) h |= (int)((rbase[0] >> s.hq[i]) & 1) << i;
   const V *Sr = Sp + s.soff + h * K * K + (16 * nb + li) * K + lk;
   T bre[KB], bim[KB];
 #pragma unroll
@@ -1155,10 +1173,11 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
     bre[kb] = v.x;
     bim[kb] = v.y;
   }
+  const uint32_t arow = scatter4(li);
   v4 cre[RT], cim[RT];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) {
-    const uint32_t base = deposit((uint32_t)(g0 + 16 * rt + li));
+    const uint32_t base = rbase[rt] | arow;
     v4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
@@ -1171,17 +1190,18 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
     cre[rt] = xr;
     cim[rt] = xi;
   }
-  // waves of other nu blocks read these groups: all reads before any write
-  lds_barrier();
+  // waves of other nu blocks read these groups: all reads before any write (with
+  // one nu block every wave owns its groups alone)
+  if (NB > 1) lds_barrier();
+  const uint32_t ocol = koff(16 * nb + li);
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const uint32_t base = deposit((uint32_t)(g0 + 16 * rt + M::row(lk, r)));
       V o;
       o.x = cre[rt][r];
       o.y = cim[rt][r];
-      tile[base | koff(16 * nb + li)] = o;
+      tile[rbase[rt] | scatter4(M::row(lk, r)) | ocol] = o;
     }
   lds_barrier();
 }
