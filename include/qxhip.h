@@ -89,6 +89,11 @@ qx_status qx_plan_set_tensor(qx_plan *plan, const char *key, const double *z, in
  *   "autotune"    1 = the next call of each lowering runs every candidate kernel of
  *                 every step (same results), times them with HIP events and keeps
  *                 the fastest per step for later calls; default 0
+ *   "chains"      1 (default) = fold runs of gate-block steps on one large tensor
+ *                 into single passes and batch small steps per dependency level;
+ *                 0 = one launch per step
+ *   "batch_last"  1 = lay the bitstring batch leg out slowest in bitstring-dependent
+ *                 intermediates (default 0: fastest)
  *   "kernel"      -1 (default) = heuristic kernel choice; 0 gather, 1 rows, 2-4
  *                 FP tile GEMMs, 5 / 7 matrix-core GEMM (K chunks of 16 / 32), 6 reduce = use
  *                 that kernel on every step it can run (testing) */
