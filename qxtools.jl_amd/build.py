@@ -46,7 +46,7 @@ def build(debug: bool = False, verbose: bool = False) -> str:
     objs = [os.path.join(bdir, f + ".o") for f in HOST_SRC + HIP_SRC]
     tmp = OUT + ".tmp"
     _run([f"{ROCM}/bin/hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp,
-          f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
+          f"-L{ROCM}/lib", "-lrocblas", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, OUT)
     return OUT
 

@@ -1285,19 +1285,22 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   const E *X0e = reinterpret_cast<const E *>(X0);
   E *tile_e = reinterpret_cast<E *>(tile);
   E pf[kChainPer];
+  // The next tile's loads are issued before this tile's steps and waited for after
+  // this tile's stores, so the stores stay in flight across that wait (one in-order
+  // counter covers loads and stores); the loop header sees the same pending loads
+  // from the prologue and from the loop body.
   int64_t w = blockIdx.x;
   if (w < ntiles) {
     const int64_t bi = in_base(w);
     static_for<kChainPer>([&](auto r) { pf[r] = X0e[bi + rin(r)]; });
-  }
-  for (; w < ntiles; w += gridDim.x) {
     static_for<kChainPer>([&](auto r) { tile_e[t + kChainThreads * r] = pf[r]; });
-    // the next tile's loads overlap this tile's steps
     if (w + gridDim.x < ntiles) {
-      const int64_t bi = in_base(w + gridDim.x);
-      static_for<kChainPer>([&](auto r) { pf[r] = X0e[bi + rin(r)]; });
+      const int64_t bn = in_base(w + gridDim.x);
+      static_for<kChainPer>([&](auto r) { pf[r] = X0e[bn + rin(r)]; });
     }
     lds_barrier();
+  }
+  for (; w < ntiles; w += gridDim.x) {
     for (int j = 0; j < a.nsteps; ++j) {
       const ChainStepArgs &s = a.st[j];
       switch (s.k) {
@@ -1320,6 +1323,14 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
     E *Xme = reinterpret_cast<E *>(Xm);
     static_for<kChainPer>([&](auto r) { Xme[bo + rout(r)] = tile_e[rso(r)]; });
     lds_barrier();
+    if (w + gridDim.x < ntiles) {
+      static_for<kChainPer>([&](auto r) { tile_e[t + kChainThreads * r] = pf[r]; });
+      if (w + 2 * gridDim.x < ntiles) {
+        const int64_t bn = in_base(w + 2 * gridDim.x);
+        static_for<kChainPer>([&](auto r) { pf[r] = X0e[bn + rin(r)]; });
+      }
+      lds_barrier();
+    }
   }
 }
 

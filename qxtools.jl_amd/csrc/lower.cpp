@@ -115,7 +115,7 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
 
 // Kernel configurations a power-of-two step can run with.
 enum Choice { kBits = 0, kRows = 1, kGemm0 = 2, kGemm1 = 3, kGemm2 = 4, kMfma = 5, kReduce = 6, kMfma32 = 7,
-              kNumChoices = 8 };
+              kBlas = 8, kNumChoices = 9 };
 
 namespace {
 struct CB {
@@ -154,6 +154,83 @@ bool rows_ok(const StepBits &b) {
 }
 }  // namespace
 
+namespace {
+// Bits of one class, weights in two operands, ordered by the first; true when the
+// weights in both form dense runs w, 2w, 4w, ... (one matrix dimension each).
+bool dense_pair(std::vector<std::pair<int64_t, int64_t>> w, int64_t &lo1, int64_t &lo2) {
+  std::sort(w.begin(), w.end());
+  lo1 = lo2 = 0;
+  if (w.empty()) return true;
+  for (size_t i = 1; i < w.size(); ++i)
+    if (w[i].first != 2 * w[i - 1].first || w[i].second != 2 * w[i - 1].second) return false;
+  lo1 = w[0].first;
+  lo2 = w[0].second;
+  return true;
+}
+
+// Logical matrix Z (rows, cols) stored with dense runs starting at lo_r / lo_c:
+// column-major as stored (op 0, ld = lo_c) or transposed (op 1, ld = lo_r).
+bool matrix_op(int64_t rows, int64_t cols, int64_t lo_r, int64_t lo_c, int &op, int64_t &ld) {
+  if (lo_r == 1 || rows == 1) {
+    op = 0;
+    ld = cols == 1 ? std::max<int64_t>(rows, 1) : lo_c;
+    return ld >= rows;
+  }
+  if (lo_c == 1 || cols == 1) {
+    op = 1;
+    ld = rows == 1 ? std::max<int64_t>(cols, 1) : lo_r;
+    return ld >= cols;
+  }
+  return false;
+}
+
+// A step that the vendor BLAS can run as one strided-batched GEMM without copies.
+bool plain_gemm(const Step &st, BlasArgs &b) {
+  if (st.radix.conj_a || st.radix.conj_b) return false;   // conj flags survive reconfiguration here
+  std::vector<std::pair<int64_t, int64_t>> am_cm, bn_cn, ak_bk, ha_hb, ha_hc;
+  for (auto &g : st.cl) {
+    if (!pow2(g.dim)) return false;
+    for (int i = 0; i < ilog2(g.dim); ++i) {
+      const int64_t wa = g.sa << i, wb = g.sb << i, wc = g.sc << i;
+      if (wa && wb) ha_hb.push_back({wa, wb}), ha_hc.push_back({wa, wc});
+      else if (wa) am_cm.push_back({wa, wc});
+      else bn_cn.push_back({wb, wc});
+    }
+  }
+  for (auto &g : st.kl) {
+    if (!pow2(g.dim)) return false;
+    for (int i = 0; i < ilog2(g.dim); ++i) ak_bk.push_back({g.sa << i, g.sb << i});
+  }
+  const int nm = (int)am_cm.size(), nn = (int)bn_cn.size(), nk = (int)ak_bk.size();
+  if (nm < 4 || nn < 4 || nk < 4) return false;
+  int64_t am, cm, bn, cn, ak, bk, ha, hb, hc, ha2;
+  if (!dense_pair(am_cm, am, cm) || !dense_pair(bn_cn, bn, cn) || !dense_pair(ak_bk, ak, bk) ||
+      !dense_pair(ha_hb, ha, hb) || !dense_pair(ha_hc, ha2, hc))
+    return false;
+  const int64_t M = int64_t(1) << nm, N = int64_t(1) << nn, K = int64_t(1) << nk;
+  b.batch = int64_t(1) << ha_hb.size();
+  int opx, opy;
+  int64_t ldx, ldy;
+  if (cm == 1) {   // C = M x N column-major: X = A (M x K), Y = B (K x N)
+    b.swap = 0;
+    b.r = M, b.s = N, b.ldc = cn;
+    if (!matrix_op(M, K, am, ak, opx, ldx) || !matrix_op(K, N, bk, bn, opy, ldy)) return false;
+    b.sx = ha, b.sy = hb;
+  } else if (cn == 1) {   // C^T = N x M: X = B^T (N x K), Y = A^T (K x M)
+    b.swap = 1;
+    b.r = N, b.s = M, b.ldc = cm;
+    if (!matrix_op(N, K, bn, bk, opx, ldx) || !matrix_op(K, M, ak, am, opy, ldy)) return false;
+    b.sx = hb, b.sy = ha;
+  } else {
+    return false;
+  }
+  if (b.ldc < b.r) return false;
+  b.opx = opx, b.opy = opy, b.ldx = ldx, b.ldy = ldy, b.k = K, b.sc = hc;
+  const int64_t lim = int64_t(1) << 31;   // rocblas_int sizes and leading dimensions
+  return b.r < lim && b.s < lim && b.k < lim && b.ldx < lim && b.ldy < lim && b.ldc < lim && b.batch < lim;
+}
+}  // namespace
+
 // Configure `st` for kernel configuration `choice`; false if that kernel cannot
 // run this step.  Every configuration computes the same C.
 bool configure_kernel(Step &st, int precision, int choice) {
@@ -167,6 +244,13 @@ bool configure_kernel(Step &st, int precision, int choice) {
   for (int i = 0; i < nk; ++i) b.ka[i] = sb.kbits[i].first, b.kb[i] = sb.kbits[i].second;
   st.ws_elems = 0;
   switch (choice) {
+    case kBlas: {
+      BlasArgs ba;
+      if (!plain_gemm(st, ba)) return false;
+      st.kern = Kern::Blas;
+      st.blas = ba;
+      return true;
+    }
     case kBits: {
       st.kern = Kern::Bits;
       // small C, long K: split K over workgroups (lanes walk C)
@@ -334,6 +418,7 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
   if (sb.nk >= 5 && sb.nn >= 4 && sb.nm >= 4) alts.push_back(kMfma32);
   if (sb.nr >= 8 && rows_ok(sb)) alts.push_back(kRows);
   if (st.c_elems <= (int64_t(1) << 16) && sb.nk >= 6) alts.push_back(kReduce);
+  if (sb.nk >= 4 && sb.nn >= 4 && sb.nm >= 4) alts.push_back(kBlas);
   if (sb.nk <= 6 || st.c_elems <= (int64_t(1) << 16)) alts.push_back(kBits);
   const int def = default_choice(st, precision);
   for (int c : alts) {
@@ -586,7 +671,7 @@ Step chain_step(const Program &p, const Lowered &L, const ChainBuild &cb) {
   std::vector<int> order(kChainTileBits);
   for (int i = 0; i < kChainTileBits; ++i) order[i] = i;
   std::sort(order.begin(), order.end(), [&](int x, int y) { return a.wout[x] < a.wout[y]; });
-  for (int i = 0; i < kChainTileBits; ++i) a.pout[i] = (int8_t)order[i];
+  for (int i = 0; i < kChainTileBits; ++i) a.pout[i] = order[i];
   a.nouter = (int)outer.size();
   if (a.nouter > kMaxBits) throw Error(QX_ERR_UNSUPPORTED, "chain: too many legs");
   for (int j = 0; j < a.nouter; ++j) a.oin[j] = cb.x0_stride[outer[j]], a.oout[j] = xm_stride[outer[j]];
@@ -595,10 +680,10 @@ Step chain_step(const Program &p, const Lowered &L, const ChainBuild &cb) {
   for (size_t j = 0; j < cb.steps.size(); ++j) {
     const auto &s = cb.steps[j];
     ChainStepArgs &c = a.st[j];
-    c.k = (int8_t)s.kslot.size();
-    c.h = (int8_t)s.hslot.size();
-    for (int i = 0; i < c.k; ++i) c.q[i] = (int8_t)pos_of[s.kslot[i]], c.sk[i] = (int16_t)s.sk[i], c.sn[i] = (int16_t)s.sn[i];
-    for (int i = 0; i < c.h; ++i) c.hq[i] = (int8_t)pos_of[s.hslot[i]], c.sh[i] = (int16_t)s.sh[i];
+    c.k = (int32_t)s.kslot.size();
+    c.h = (int32_t)s.hslot.size();
+    for (int i = 0; i < c.k; ++i) c.q[i] = pos_of[s.kslot[i]], c.sk[i] = (int16_t)s.sk[i], c.sn[i] = (int16_t)s.sn[i];
+    for (int i = 0; i < c.h; ++i) c.hq[i] = pos_of[s.hslot[i]], c.sh[i] = (int16_t)s.sh[i];
     c.soff = (int32_t)st.ws_elems;
     st.ws_elems += (int64_t(1) << (c.h + 2 * c.k));
     st.chain_small.push_back(s.small);

@@ -139,11 +139,14 @@ constexpr int kChainMaxSteps = 32;
 constexpr int kChainMinPassive = 3;   // untouched fast legs kept in the tile
 constexpr int kChainMaxS = 1024;      // elements of one staged small operand
 
+// The fields the chain kernel reads are 32-bit: it indexes steps with a uniform loop
+// counter, and dword-aligned fields load through the scalar unit (narrower ones would
+// take vector loads whose waits also drain the next tile's prefetch).
 struct ChainStepArgs {
-  int8_t k = 0, h = 0;          // contracted legs (each replaced by a new leg), batch legs
-  int8_t q[5] = {0}, hq[3] = {0};   // tile bit of each contracted / batch leg
-  int16_t sk[5] = {0}, sn[5] = {0}, sh[3] = {0};   // strides in S of contracted, new, batch legs
+  int32_t k = 0, h = 0;         // contracted legs (each replaced by a new leg), batch legs
+  int32_t q[5] = {0}, hq[3] = {0};   // tile bit of each contracted / batch leg
   int32_t soff = 0;             // offset of S packed as [h][nu][kappa] in the workspace
+  int16_t sk[5] = {0}, sn[5] = {0}, sh[3] = {0};   // strides in S of contracted, new, batch legs
 };
 
 struct ChainArgs {
@@ -151,11 +154,12 @@ struct ChainArgs {
   int grid = 256;          // workgroups (each walks tiles); tuned
   int mfma16 = 0;          // steps with 16 contracted values on matrix cores (else vector FMAs)
   int64_t win[kChainTileBits] = {0}, wout[kChainTileBits] = {0};   // tile bit -> X_0 / X_m stride
-  int8_t pout[kChainTileBits] = {0};   // store order: i-th fastest X_m tile bit -> tile bit
+  int32_t pout[kChainTileBits] = {0};  // store order: i-th fastest X_m tile bit -> tile bit
   int64_t oin[kMaxBits] = {0}, oout[kMaxBits] = {0};   // outer bits (workgroup index) -> strides
   ChainStepArgs st[kChainMaxSteps];
   const void *S[kChainMaxSteps] = {nullptr};   // small operands, set at launch
 };
+static_assert(sizeof(ChainArgs) <= 4096, "chain arguments must fit the kernel argument segment");
 
 // C[h, m, n] = sum_k A[h, m, k] B[h, k, n], every index mapped through bit
 // weights (all dims powers of two).  m bits are listed in increasing C weight,
@@ -179,7 +183,17 @@ struct Leg {
   int64_t sa, sb, sc;
 };
 
-enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows, Chain };
+// A step whose operands are plain strided-batched column-major matrices, run by the
+// vendor BLAS: C (R x S, ldc) = op(X) (R x K) . op(Y) (K x S), batch strides per
+// operand.  op: 0 = as stored, 1 = transposed.  swap: X/Y are B/A (C stored
+// N-fastest).
+struct BlasArgs {
+  int opx = 0, opy = 0, swap = 0;
+  int64_t r = 0, s = 0, k = 0, ldx = 0, ldy = 0, ldc = 0;
+  int64_t sx = 0, sy = 0, sc = 0, batch = 1;
+};
+
+enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows, Chain, Blas };
 
 struct Step {
   int node = -1;           // ncon node
@@ -196,6 +210,7 @@ struct Step {
   GemmArgs gemm;           // Kern::Gemm
   RowArgs rows;            // Kern::Rows
   RadixArgs radix;         // Kern::Radix
+  BlasArgs blas;           // Kern::Blas
   ChainArgs chain;         // Kern::Chain: in_a = X_0, node = X_m
   std::vector<int> chain_small;   // Kern::Chain: node of each S_j
   double chain_flops = 0, chain_elems = 0;   // Kern::Chain: work and compulsory elements moved
@@ -272,7 +287,8 @@ int64_t first_fit(const std::vector<int64_t> &start, const std::vector<int64_t> 
 // Kernel launch interface (kernels.hip).
 void set_kernel_args(Step &st, int precision = QX_C128);
 // Run `st` with kernel configuration `choice` (0 gather, 1 rows, 2-4 FP tile GEMMs,
-// 5 matrix-core GEMM, 6 reduce); false (and `st` unchanged) if that kernel cannot.
+// 5 / 7 matrix-core GEMM, 6 reduce, 8 vendor BLAS on plain matrix layouts); false
+// (and `st` unchanged) if that kernel cannot.
 bool force_kernel(Step &st, int precision, int choice);
 // Alternative kernel configurations of a step (the current one first) for the autotuner.
 std::vector<Step> kernel_candidates(const Step &st, int precision);

@@ -9,6 +9,7 @@
 //                     bitstring-dependent steps, add the saved tensor into FP64
 //                     Neumaier accumulators on the device.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <array>
@@ -23,6 +24,8 @@
 struct qx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;   // created by the first plan that runs a vendor BLAS step
+  void *blas_ws = nullptr;         // its workspace (the handle allocates nothing itself)
 };
 
 namespace {
@@ -207,10 +210,72 @@ void *value_ptr(qx_plan *P, const qx::Value &v, const std::vector<int64_t> &varv
   return value_ptr(P, v, varval, b);
 }
 
+constexpr size_t kBlasWorkspace = size_t(64) << 20;
+
+// The context's rocBLAS handle, bound to its stream with a workspace of its own, so
+// that its calls allocate nothing and can be captured into a graph.  Must be first
+// called outside stream capture.
+rocblas_handle blas_handle(qx_ctx *c) {
+  if (c->blas) return c->blas;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hip_check(hipStreamIsCapturing(c->stream, &cs), "hipStreamIsCapturing");
+  if (cs != hipStreamCaptureStatusNone) throw qx::Error(QX_ERR_STATE, "internal: BLAS handle created under capture");
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) throw qx::Error(QX_ERR_DEVICE, "rocblas_create_handle failed");
+  void *ws = nullptr;
+  hipError_t e = hipMalloc(&ws, kBlasWorkspace);
+  if (e != hipSuccess) {
+    rocblas_destroy_handle(h);
+    hip_check(e, "hipMalloc (BLAS workspace)");
+  }
+  if (rocblas_set_workspace(h, ws, kBlasWorkspace) != rocblas_status_success ||
+      rocblas_set_stream(h, c->stream) != rocblas_status_success) {
+    rocblas_destroy_handle(h);
+    (void)hipFree(ws);
+    throw qx::Error(QX_ERR_DEVICE, "rocblas handle set-up failed");
+  }
+  c->blas = h;
+  c->blas_ws = ws;
+  return h;
+}
+
+// Create the BLAS handle before a capture when any of `steps` needs it.
+void blas_before_capture(qx_ctx *c, const std::vector<qx::Step> &steps) {
+  for (auto &st : steps)
+    if (st.kern == qx::Kern::Blas) {
+      blas_handle(c);
+      return;
+    }
+}
+
+// C = op(X) op(Y) by rocBLAS (strided batched, no copies): the step's operands are
+// plain column-major matrices (see qx::BlasArgs).
+void launch_blas(qx_plan *P, const qx::BlasArgs &b, const void *A, const void *B, void *C) {
+  rocblas_handle h = blas_handle(P->ctx);
+  const void *X = b.swap ? B : A, *Y = b.swap ? A : B;
+  const rocblas_operation ox = b.opx ? rocblas_operation_transpose : rocblas_operation_none;
+  const rocblas_operation oy = b.opy ? rocblas_operation_transpose : rocblas_operation_none;
+  rocblas_status r;
+  if (P->prec == QX_C128) {
+    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+    r = rocblas_zgemm_strided_batched(h, ox, oy, b.r, b.s, b.k, &one, (const rocblas_double_complex *)X, b.ldx,
+                                      b.sx, (const rocblas_double_complex *)Y, b.ldy, b.sy, &zero,
+                                      (rocblas_double_complex *)C, b.ldc, b.sc, b.batch);
+  } else {
+    const rocblas_float_complex one{1.0f, 0.0f}, zero{0.0f, 0.0f};
+    r = rocblas_cgemm_strided_batched(h, ox, oy, b.r, b.s, b.k, &one, (const rocblas_float_complex *)X, b.ldx,
+                                      b.sx, (const rocblas_float_complex *)Y, b.ldy, b.sy, &zero,
+                                      (rocblas_float_complex *)C, b.ldc, b.sc, b.batch);
+  }
+  if (r != rocblas_status_success)
+    throw qx::Error(QX_ERR_DEVICE, std::string("rocBLAS gemm failed: ") + rocblas_status_to_string(r));
+}
+
 void launch_step(qx_plan *P, const qx::Lowered &L, const qx::Step &st, void *A, void *B, void *C,
                  const std::vector<int64_t> &varval, const Bases *bs = nullptr) {
   hipStream_t stream = P->ctx->stream;
   switch (st.kern) {
+    case qx::Kern::Blas: launch_blas(P, st.blas, A, B, C); break;
     case qx::Kern::Chain: {
       qx::ChainArgs a = st.chain;
       for (int j = 0; j < a.nsteps; ++j) {
@@ -601,6 +666,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     auto it = P->graphs.find(key);
     if (it == P->graphs.end()) {
       hipGraph_t g = nullptr;
+      blas_before_capture(P->ctx, L.steps);
       hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       try {
         enqueue();
@@ -714,6 +780,7 @@ const char *kern_name(const qx::Step &st) {
     case qx::Kern::Reduce: return "reduce";
     case qx::Kern::Rows: return "rows";
     case qx::Kern::Chain: return "chain";
+    case qx::Kern::Blas: return "blas";
     default: return "bits";
   }
 }
@@ -735,6 +802,9 @@ std::string kern_variant(const qx::Step &st) {
     case qx::Kern::Reduce: return "k_contract_reduce";
     case qx::Kern::Radix: return "k_contract_radix";
     case qx::Kern::Chain: return "k_contract_chain";
+    case qx::Kern::Blas:
+      return std::string("rocblas_gemm_strided_batched<") + (st.blas.opx ? "T" : "N") + (st.blas.opy ? "T" : "N") +
+             (st.blas.swap ? ",BA>" : ",AB>");
     default: return "k_contract_bits";
   }
 }
@@ -1030,6 +1100,7 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
     auto it = P->vjp_graphs.find(key);
     if (it == P->vjp_graphs.end()) {
       hipGraph_t g = nullptr;
+      blas_before_capture(P->ctx, L.steps);
       hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       try {
         enqueue();
@@ -1115,6 +1186,8 @@ qx_status qx_ctx_create(int device, qx_ctx **out) {
 void qx_ctx_destroy(qx_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
+  if (ctx->blas_ws) (void)hipFree(ctx->blas_ws);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1171,7 +1244,7 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
     } else if (n == "batch_last") {
       plan->opt_batch_last = value;
     } else if (n == "kernel") {
-      if (value < -1 || value > 7) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..7");
+      if (value < -1 || value > 8) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..8");
       plan->opt_kernel = value;
     } else if (n == "profile") {
       // may be toggled at any time; every set clears the accumulated step times

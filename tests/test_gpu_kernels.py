@@ -147,13 +147,15 @@ def test_gemm_mfma_c128(case):
 FORCE_SHAPES = [(0, 12, 5, 5), (1, 7, 5, 10), (0, 10, 2, 3), (2, 8, 2, 2), (0, 2, 2, 8), (0, 6, 6, 6)]
 
 
-@pytest.mark.parametrize("choice,prec", [(c, "c128") for c in range(8)] + [(5, "c64_4m"), (5, "c64_3m"),
-                                                                           (7, "c64_4m"), (7, "c64_3m")])
+@pytest.mark.parametrize("choice,prec", [(c, "c128") for c in range(9)] + [(5, "c64_4m"), (5, "c64_3m"),
+                                                                           (7, "c64_4m"), (7, "c64_3m"),
+                                                                           (8, "c64_4m")])
 @pytest.mark.parametrize("shape", FORCE_SHAPES)
 def test_forced_kernel_matches_oracle(shape, choice, prec):
     """Every kernel the autotuner may pick computes the same contraction: each
-    configuration (0 gather, 1 rows, 2-4 FP tiles, 5/7 matrix cores, 6 reduce) forced
-    onto the step; the gather kernel and the GEMM tiles run any power-of-two step."""
+    configuration (0 gather, 1 rows, 2-4 FP tiles, 5/7 matrix cores, 6 reduce, 8 rocBLAS)
+    forced onto the step; the gather kernel and the GEMM tiles run any power-of-two
+    step (rocBLAS only plain matrix layouts, else the step keeps its kernel)."""
     dsl, data, ref = single_step(700 + sum(shape), *shape, c_rows_first=True)
     ex = Executor(dsl, data, precision=prec, kernel=choice)
     out = np.asarray(ex.amplitudes([""])).reshape(-1)
@@ -162,3 +164,53 @@ def test_forced_kernel_matches_oracle(shape, choice, prec):
     assert err(out, ref) < (5e-5 if prec != "c128" else 1e-12)
     if choice in (0, 2, 3, 4, 5, 7):
         assert kern == ("bits" if choice == 0 else "gemm")
+
+
+def plain_step(seed, nh, nm, nn, nk, a_kfirst, b_nfirst, c_nfirst):
+    """One ncon whose operands are plain (batched) column-major matrices: each class's
+    legs consecutive and in order, the batch legs slowest."""
+    labs = list(range(1, nh + nm + nn + nk + 1))
+    H, M, N, K = labs[:nh], labs[nh:nh + nm], labs[nh + nm:nh + nm + nn], labs[nh + nm + nn:]
+    al = (K + M if a_kfirst else M + K) + H
+    bl = (N + K if b_nfirst else K + N) + H
+    cl = (N + M if c_nfirst else M + N) + H
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=[2] * len(al)) + 1j * rng.normal(size=[2] * len(al))
+    B = rng.normal(size=[2] * len(bl)) + 1j * rng.normal(size=[2] * len(bl))
+    fmt = lambda ls: ",".join(map(str, ls))
+    dsl = "\n".join([
+        "# version: 0.4.0",
+        f"load A data_1 {fmt([2] * len(al))}",
+        f"load B data_2 {fmt([2] * len(bl))}",
+        f"ncon C {fmt(cl)} A {fmt(al)} B {fmt(bl)}",
+        "save output C",
+    ]) + "\n"
+    return dsl, {"data_1": A, "data_2": B}, O.ncon(A, al, B, bl, cl).reshape(-1, order="F")
+
+
+PLAIN_SHAPES = [(0, 6, 5, 7), (1, 5, 6, 4), (2, 4, 4, 8), (0, 9, 4, 6)]
+
+
+@pytest.mark.parametrize("layout", range(8))
+@pytest.mark.parametrize("shape", PLAIN_SHAPES)
+@pytest.mark.parametrize("prec", ["c128", "c64_4m"])
+def test_rocblas_plain_layouts(shape, layout, prec):
+    """The rocBLAS candidate on every transpose / operand-order combination of plain
+    matrix layouts (A as M x K or K x M, B as K x N or N x K, C as M x N or N x M)."""
+    dsl, data, ref = plain_step(900 + layout + 10 * sum(shape), *shape, layout & 1, layout >> 1 & 1, layout >> 2)
+    ex = Executor(dsl, data, precision=prec, kernel=8)
+    out = np.asarray(ex.amplitudes([""])).reshape(-1)
+    st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][0]
+    ex.close()
+    assert st["kernel"] == "blas"
+    assert err(out, ref) < (5e-6 if prec != "c128" else 1e-12)
+
+
+def test_rocblas_step_in_graph_replays():
+    """A rocBLAS step captured into the call's graph: every replay equals the oracle."""
+    dsl, data, ref = plain_step(77, 1, 7, 6, 8, 1, 0, 1)
+    ex = Executor(dsl, data, kernel=8, graph=1)
+    for _ in range(3):
+        out = np.asarray(ex.amplitudes([""])).reshape(-1)
+        assert err(out, ref) < 1e-12
+    ex.close()

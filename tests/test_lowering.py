@@ -201,3 +201,14 @@ def test_reserved_precision_is_rejected():
     with pytest.raises(_native.QxError) as e:
         _native.inspect("load A data_1 2\nsave output A\n", precision="c64_bf16x3")
     assert e.value.status == -8
+
+
+def test_plain_matrix_steps_offer_rocblas():
+    """Steps whose operands are plain (batched) column-major matrices list the rocBLAS
+    GEMM among their candidates; shuffled leg orders do not."""
+    for layout in range(8):
+        dsl, _, _ = K.plain_step(layout, 1, 5, 6, 4, layout & 1, layout >> 1 & 1, layout >> 2)
+        c = _native.inspect(dsl)["steps"][0]["candidates"]
+        assert any(x.startswith("rocblas_gemm_strided_batched") for x in c), layout
+    dsl, _, _ = K.single_step(5, 1, 5, 6, 4)
+    assert not any(x.startswith("rocblas") for x in _native.inspect(dsl)["steps"][0]["candidates"])
