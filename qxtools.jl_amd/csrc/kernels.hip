@@ -1285,23 +1285,36 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   const E *X0e = reinterpret_cast<const E *>(X0);
   E *tile_e = reinterpret_cast<E *>(tile);
   E pf[kChainPer];
-  // The next tile's loads are issued before this tile's steps and waited for after
-  // this tile's stores, so the stores stay in flight across that wait (one in-order
-  // counter covers loads and stores); the loop header sees the same pending loads
-  // from the prologue and from the loop body.
+  // The next tile's loads are spread over this tile's steps (a quarter of them before
+  // the steps at 0, n/4, n/2 and 3n/4): issued in one burst they stall the waves at
+  // issue until most have returned, and no step work overlaps them.  They are waited
+  // for after this tile's stores, which stay in flight across that wait (one in-order
+  // counter covers loads and stores).
+  auto load_quarter = [&](int q, int64_t bn) {
+    auto ld = [&](auto r) { pf[r] = X0e[bn + rin(r)]; };
+    switch (q) {
+      case 0: static_for<4>([&](auto r) { ld(std::integral_constant<int, decltype(r)::value>{}); }); break;
+      case 1: static_for<4>([&](auto r) { ld(std::integral_constant<int, 4 + decltype(r)::value>{}); }); break;
+      case 2: static_for<4>([&](auto r) { ld(std::integral_constant<int, 8 + decltype(r)::value>{}); }); break;
+      default: static_for<4>([&](auto r) { ld(std::integral_constant<int, 12 + decltype(r)::value>{}); }); break;
+    }
+  };
+  static_assert(kChainPer == 16, "four quarters of four elements");
   int64_t w = blockIdx.x;
   if (w < ntiles) {
     const int64_t bi = in_base(w);
     static_for<kChainPer>([&](auto r) { pf[r] = X0e[bi + rin(r)]; });
     static_for<kChainPer>([&](auto r) { tile_e[t + kChainThreads * r] = pf[r]; });
-    if (w + gridDim.x < ntiles) {
-      const int64_t bn = in_base(w + gridDim.x);
-      static_for<kChainPer>([&](auto r) { pf[r] = X0e[bn + rin(r)]; });
-    }
     lds_barrier();
   }
   for (; w < ntiles; w += gridDim.x) {
-    for (int j = 0; j < a.nsteps; ++j) {
+    const bool more = w + gridDim.x < ntiles;
+    const int64_t bn = more ? in_base(w + gridDim.x) : 0;
+    const int n = a.nsteps;
+    for (int j = 0; j < n; ++j) {
+      if (more)
+        for (int q = 0; q < 4; ++q)
+          if (q * n / 4 == j) load_quarter(q, bn);
       const ChainStepArgs &s = a.st[j];
       switch (s.k) {
         case 0: chain_apply<T, 1>(tile, Sp, s); break;
@@ -1323,12 +1336,8 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
     E *Xme = reinterpret_cast<E *>(Xm);
     static_for<kChainPer>([&](auto r) { Xme[bo + rout(r)] = tile_e[rso(r)]; });
     lds_barrier();
-    if (w + gridDim.x < ntiles) {
+    if (more) {
       static_for<kChainPer>([&](auto r) { tile_e[t + kChainThreads * r] = pf[r]; });
-      if (w + 2 * gridDim.x < ntiles) {
-        const int64_t bn = in_base(w + 2 * gridDim.x);
-        static_for<kChainPer>([&](auto r) { pf[r] = X0e[bn + rin(r)]; });
-      }
       lds_barrier();
     }
   }

@@ -354,8 +354,13 @@ bool configure_kernel(Step &st, int precision, int choice) {
 //    unit stride, else the gather kernel with K split over workgroups
 //  * Bits   : everything else with power-of-two dims
 //  * Radix  : any other dims
+//  * Blas   : complex128 steps of >= 2^30 flops whose operands are plain matrices
+//             (rocBLAS' zgemm reaches about twice the FP64 rate of the tile kernels)
 int default_choice(const Step &st, int precision) {
   const StepBits sb = step_bits(st);
+  BlasArgs ba;
+  if (precision == QX_C128 && 8.0 * st.H * st.M * st.N * st.K >= double(int64_t(1) << 30) && plain_gemm(st, ba))
+    return kBlas;
   if (sb.nr >= 12 && sb.nk <= 4 && sb.nk + sb.nn >= 1 && !sb.cbits.empty() && sb.cbits[0].wa != 0 && rows_ok(sb))
     return kRows;
   if (sb.nk >= 3 && sb.nn >= 3 && sb.nm >= 5) {
@@ -488,6 +493,8 @@ Step make_step(const Value &A, const std::vector<int> &alab, bool conj_a, const 
   st.kern = (all_pow2 && nbc <= kMaxBits && nbk <= kMaxBits) ? Kern::Bits : Kern::Radix;
   if (st.kern == Kern::Radix && ((int)st.cl.size() > kMaxLegs || (int)st.kl.size() > kMaxLegs))
     throw Error(QX_ERR_UNSUPPORTED, "step with more than 48 legs of non power-of-two dims");
+  st.radix.conj_a = conj_a;   // read by the kernel choice (plain GEMMs take no conjugate)
+  st.radix.conj_b = conj_b;
   set_kernel_args(st, precision);
   st.bits.conj_a = st.gemm.conj_a = st.rows.conj_a = st.radix.conj_a = conj_a;
   st.bits.conj_b = st.gemm.conj_b = st.rows.conj_b = st.radix.conj_b = conj_b;
@@ -871,6 +878,83 @@ void level_small_steps(const Program &p, Lowered &L) {
 
 }  // namespace
 
+// Leg orders (positions in each node's own dims, fastest first) that turn large
+// contractions into plain column-major matrix products, which the vendor BLAS runs
+// without copies.  For C = A.B with legs M (A only), N (B only) and K (contracted),
+// A's legs are regrouped into a K run and an M run (each keeping its relative order,
+// the run holding A's fastest leg first), B's K legs follow A's order next to its N
+// run, and C holds the M run in A's order and the N run in B's order.  Only
+// intermediates that the contraction alone consumes are regrouped; C keeps its other
+// consumers, which read it through its strides.  The batched bitstring leg stays
+// fastest (slowest with batch_last), so its class's run is placed beside it.
+// Contractions with batch legs (in A, B and C) are left alone.  Empty = DSL order.
+static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::vector<std::vector<int64_t>> &shape,
+                                                   int64_t batch, bool batch_last) {
+  const size_t nn = p.nodes.size();
+  std::vector<std::vector<int>> order(nn);
+  std::vector<int> uses(nn, 0);
+  std::vector<char> pinned(nn, 0);   // saved or viewed: layout fixed
+  for (const Node &n : p.nodes) {
+    if (n.op == Op::Ncon) uses[n.ia]++, uses[n.ib]++;
+    if (n.op == Op::View || n.op == Op::Save) uses[n.ia]++, pinned[n.ia] = 1;
+  }
+  for (size_t i = 0; i < nn; ++i) {
+    const Node &n = p.nodes[i];
+    if (n.op != Op::Ncon || pinned[i] || !order[i].empty() || n.ia == n.ib) continue;
+    auto free_operand = [&](int x) {
+      return p.nodes[x].op == Op::Ncon && uses[x] == 1 && order[x].empty() && !pinned[x];
+    };
+    if (!free_operand(n.ia) || !free_operand(n.ib)) continue;
+    const bool da = p.nodes[n.ia].dep, db = p.nodes[n.ib].dep, dc = n.dep;
+    char beta = 0;   // class of the batched bitstring leg
+    if (!da && db && dc) beta = 'N';
+    else if (da && !db && dc) beta = 'M';
+    else if (da || db || dc) continue;
+    const std::vector<int64_t> &sa = shape[n.ia], &sb = shape[n.ib], &sc = shape[i];
+    auto find = [](const std::vector<int> &v, int l) {
+      return (int)(std::find(v.begin(), v.end(), l) - v.begin());
+    };
+    std::vector<int> aK, aM, bN;
+    double M = beta == 'M' ? (double)batch : 1.0, N = beta == 'N' ? (double)batch : 1.0, K = 1.0;
+    bool ok = true;
+    for (size_t j = 0; j < n.alab.size(); ++j) {
+      const int l = n.alab[j];
+      ok &= pow2(sa[j]);
+      const bool inb = find(n.blab, l) < (int)n.blab.size(), inc = find(n.clab, l) < (int)n.clab.size();
+      if (inb && inc) ok &= sa[j] == 1;   // batch leg of all three
+      else if (inb) aK.push_back((int)j), K *= (double)sa[j];
+      else aM.push_back((int)j), M *= (double)sa[j];
+    }
+    for (size_t j = 0; j < n.blab.size(); ++j) {
+      ok &= pow2(sb[j]);
+      if (find(n.alab, n.blab[j]) == (int)n.alab.size()) bN.push_back((int)j), N *= (double)sb[j];
+    }
+    if (!ok || M < 64 || N < 64 || K < 64 || 8.0 * M * N * K < 1e9) continue;
+    std::vector<int> bK, cM, cN;
+    for (int j : aK) bK.push_back(find(n.blab, n.alab[j]));
+    for (int j : aM) cM.push_back(find(n.clab, n.alab[j]));
+    for (int j : bN) cN.push_back(find(n.clab, n.blab[j]));
+    if ((int)(cM.size() + cN.size()) != (int)sc.size()) continue;
+    auto cat = [](std::vector<int> x, const std::vector<int> &y) {
+      x.insert(x.end(), y.begin(), y.end());
+      return x;
+    };
+    auto fastest_in = [](const std::vector<int64_t> &dims, const std::vector<int> &grp) {
+      for (size_t j = 0; j < dims.size(); ++j)
+        if (dims[j] > 1) return std::find(grp.begin(), grp.end(), (int)j) != grp.end();
+      return true;
+    };
+    // the run next to the batch leg: first when it is fastest, last when slowest
+    const bool a_m_first = beta == 'M' ? !batch_last : fastest_in(sa, aM);
+    const bool b_n_first = beta == 'N' ? !batch_last : fastest_in(sb, bN);
+    const bool c_m_first = beta == 'M' ? !batch_last : beta == 'N' ? batch_last : true;
+    order[n.ia] = a_m_first ? cat(aM, aK) : cat(aK, aM);
+    order[n.ib] = b_n_first ? cat(bN, bK) : cat(bK, bN);
+    order[i] = c_m_first ? cat(cM, cN) : cat(cN, cM);
+  }
+  return order;
+}
+
 Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uint64_t fused, bool chains,
               bool batch_last) {
   if (!pow2(batch)) throw Error(QX_ERR_ARGUMENT, "batch must be a power of two");
@@ -882,6 +966,7 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
   std::vector<std::vector<int64_t>> shape;
   std::vector<char> sdep;
   fused_shapes(p, fused, shape, sdep);
+  const std::vector<std::vector<int>> leg_order = plain_layouts(p, shape, batch, batch_last);
   std::map<std::string, int> key_index;
   for (size_t i = 0; i < p.keys.size(); ++i) key_index[p.keys[i]] = (int)i;
 
@@ -928,9 +1013,11 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
           v.beta_stride = 1;
           s = batch;
         }
-        for (int64_t d : shape[i]) {
-          v.strides.push_back(d > 1 ? s : 0);
-          if (d > 1) s *= d;
+        v.strides.assign(shape[i].size(), 0);   // legs of dim 1 (possibly left out of an order): 0
+        const std::vector<int> &ord = leg_order[i];
+        for (size_t k = 0; k < (ord.empty() ? shape[i].size() : ord.size()); ++k) {
+          const size_t j = ord.empty() ? k : (size_t)ord[k];
+          if (shape[i][j] > 1) v.strides[j] = s, s *= shape[i][j];
         }
         if (n.dep && batch_last) {
           v.beta_stride = s;
