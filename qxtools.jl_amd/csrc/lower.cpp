@@ -900,7 +900,7 @@ static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::
   }
   for (size_t i = 0; i < nn; ++i) {
     const Node &n = p.nodes[i];
-    if (n.op != Op::Ncon || pinned[i] || !order[i].empty() || n.ia == n.ib) continue;
+    if (n.op != Op::Ncon || n.ia == n.ib) continue;
     auto free_operand = [&](int x) {
       return p.nodes[x].op == Op::Ncon && uses[x] == 1 && order[x].empty() && !pinned[x];
     };
@@ -935,6 +935,30 @@ static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::
     for (int j : aM) cM.push_back(find(n.clab, n.alab[j]));
     for (int j : bN) cN.push_back(find(n.clab, n.blab[j]));
     if ((int)(cM.size() + cN.size()) != (int)sc.size()) continue;
+    const bool c_fixed = pinned[i] || !order[i].empty();
+    bool c_m_first_fixed = true;
+    if (c_fixed) {
+      // C keeps its layout: it must hold an M run and an N run (the batch leg's run on
+      // its side), and A's M legs and B's N legs take C's orders
+      std::vector<int> cs = order[i];
+      if (cs.empty())
+        for (size_t j = 0; j < sc.size(); ++j) cs.push_back((int)j);
+      std::vector<int> seq;   // classes along C's storage, legs of dim 1 skipped: 0 = M, 1 = N
+      for (int j : cs)
+        if (sc[j] > 1) seq.push_back(std::find(cM.begin(), cM.end(), j) != cM.end() ? 0 : 1);
+      int changes = 0;
+      for (size_t k = 1; k < seq.size(); ++k) changes += seq[k] != seq[k - 1];
+      if (changes > 1 || seq.empty()) continue;
+      c_m_first_fixed = seq[0] == 0;
+      if (beta && changes == 1 && (c_m_first_fixed == (beta == 'M')) == batch_last) continue;
+      auto by_c = [&](std::vector<int> &legs, const std::vector<int> &lab) {
+        std::stable_sort(legs.begin(), legs.end(), [&](int x, int y) {
+          return find(cs, find(n.clab, lab[x])) < find(cs, find(n.clab, lab[y]));
+        });
+      };
+      by_c(aM, n.alab);
+      by_c(bN, n.blab);
+    }
     auto cat = [](std::vector<int> x, const std::vector<int> &y) {
       x.insert(x.end(), y.begin(), y.end());
       return x;
@@ -950,7 +974,7 @@ static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::
     const bool c_m_first = beta == 'M' ? !batch_last : beta == 'N' ? batch_last : true;
     order[n.ia] = a_m_first ? cat(aM, aK) : cat(aK, aM);
     order[n.ib] = b_n_first ? cat(bN, bK) : cat(bK, bN);
-    order[i] = c_m_first ? cat(cM, cN) : cat(cN, cM);
+    if (!c_fixed) order[i] = c_m_first ? cat(cM, cN) : cat(cN, cM);
   }
   return order;
 }

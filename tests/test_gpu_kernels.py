@@ -214,3 +214,17 @@ def test_rocblas_step_in_graph_replays():
         out = np.asarray(ex.amplitudes([""])).reshape(-1)
         assert err(out, ref) < 1e-12
     ex.close()
+
+
+@pytest.mark.parametrize("view,prec", [(False, "c128"), (True, "c128"), (False, "c64_4m")])
+def test_regrouped_contraction_matches_oracle(view, prec):
+    """A large contraction whose operands the lowering regroups into plain matrices
+    (complex128: run by rocBLAS); with sliced operands the layouts stay as written."""
+    import qxtest
+    dsl, data, ref = qxtest.interleaved_gemm_program(view_a=view)
+    ex = Executor(dsl, data, precision=prec)
+    out = np.asarray(ex.amplitudes([""])).reshape(-1)
+    st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][-1]
+    ex.close()
+    assert (st["kernel"] == "blas") == (prec == "c128" and not view)
+    assert err(out, ref) < (5e-6 if prec != "c128" else 1e-12)

@@ -213,3 +213,25 @@ def test_plain_matrix_steps_offer_rocblas():
         assert any(x.startswith("rocblas_gemm_strided_batched") for x in c), layout
     dsl, _, _ = K.single_step(5, 1, 5, 6, 4)
     assert not any(x.startswith("rocblas") for x in _native.inspect(dsl)["steps"][0]["candidates"])
+
+
+def test_large_contraction_operands_regrouped_into_plain_matrices():
+    """The lowering regroups the interleaved legs of a large contraction's operands
+    (each produced for that contraction alone) into plain matrices, so the library
+    GEMM runs it; a viewed operand keeps its layout, and the step its tile kernel."""
+    import qxtest
+    dsl, _, _ = qxtest.interleaved_gemm_program()
+    last = _native.inspect(dsl)["steps"][-1]
+    assert last["kernel"] == "blas", last["config"]
+    # A's contracted legs form one dense run and its free legs another
+    ka = sorted(l[1] for l in last["klegs"])
+    assert ka == [2 ** i * ka[0] for i in range(len(ka))]
+    ma = sorted(l[1] for l in last["legs"] if l[1] and not l[2])
+    assert ma == [2 ** i * ma[0] for i in range(len(ma))]
+    dsl, _, _ = qxtest.interleaved_gemm_program(view_a=True)
+    last = _native.inspect(dsl)["steps"][-1]
+    assert last["kernel"] != "blas"
+    assert not any(x.startswith("rocblas") for x in last["candidates"])
+    # too small for the library GEMM: left in DSL order
+    dsl, _, _ = qxtest.interleaved_gemm_program(nk=6, nm=6, nn=6)
+    assert _native.inspect(dsl)["steps"][-1]["kernel"] != "blas"
