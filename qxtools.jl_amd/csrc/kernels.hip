@@ -1031,6 +1031,9 @@ __device__ __forceinline__ void static_for(F &&f) {
 }
 
 constexpr int kChainTile = 1 << kChainTileBits;
+// LDS beside the tile for the packed gate blocks: 32 KiB (160 KiB per workgroup)
+template <typename T>
+constexpr int kChainSLds = (32 << 10) / (int)sizeof(typename Vec<T>::type);
 constexpr int kChainThreads = 512;
 constexpr int kChainPer = kChainTile / kChainThreads;   // tile elements per lane (16)
 
@@ -1105,10 +1108,10 @@ __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const t
 // block; its 64 groups share the batch bits, so its S fragments (B operands: S at
 // nu = lane % 16, kappa = 4 kb + lane / 16) stay in registers for the whole step.
 // Operand and result layouts as in Mfma16.
-template <typename T, int K>
+template <typename T, int K, bool SL>
 __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
                                                  const typename Vec<T>::type *__restrict__ Sp,
-                                                 const ChainStepArgs &s) {
+                                                 const typename Vec<T>::type *sS, const ChainStepArgs &s) {
   using V = typename Vec<T>::type;
   using M = Mfma16<T>;
   using v4 = typename M::v4;
@@ -1165,11 +1168,12 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
   for (int rt = 0; rt < RT; ++rt) rbase[rt] = deposit((uint32_t)(g0 + 16 * rt));
   int h = 0;
   for (int i = 0; i < s.h; ++i) h |= (int)((rbase[0] >> s.hq[i]) & 1) << i;
-  const V *Sr = Sp + s.soff + h * K * K + (16 * nb + li) * K + lk;
+  const int so = s.soff + h * K * K + (16 * nb + li) * K + lk;
   T bre[KB], bim[KB];
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) {
-    const V v = Sr[4 * kb];
+    // from LDS when staged: a global load here would wait for the prefetch too
+    const V v = SL ? sS[so + 4 * kb] : Sp[so + 4 * kb];
     bre[kb] = v.x;
     bim[kb] = v.y;
   }
@@ -1236,7 +1240,11 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
                                                                   ChainArgs a) {
   using V = typename Vec<T>::type;
   __shared__ V tile[kChainTile];
+  __shared__ V sS[kChainSLds<T>];   // packed S_j of every step, when they fit
   const int t = threadIdx.x;
+  const bool s_lds = a.s_elems <= kChainSLds<T>;
+  if (s_lds)
+    for (int e = t; e < a.s_elems; e += kChainThreads) sS[e] = Sp[e];
   const int64_t ntiles = int64_t(1) << a.nouter;
   auto in_base = [&](int64_t w) {
     int64_t bi = 0;
@@ -1322,12 +1330,19 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
         case 2: chain_apply<T, 4>(tile, Sp, s); break;
         case 3: chain_apply<T, 8>(tile, Sp, s); break;
         case 4:
-          if (a.mfma16)
-            chain_apply_mfma<T, 16>(tile, Sp, s);
-          else
+          if (!a.mfma16)
             chain_apply<T, 16>(tile, Sp, s);
+          else if (s_lds)
+            chain_apply_mfma<T, 16, true>(tile, Sp, sS, s);
+          else
+            chain_apply_mfma<T, 16, false>(tile, Sp, sS, s);
           break;
-        default: chain_apply_mfma<T, 32>(tile, Sp, s); break;
+        default:
+          if (s_lds)
+            chain_apply_mfma<T, 32, true>(tile, Sp, sS, s);
+          else
+            chain_apply_mfma<T, 32, false>(tile, Sp, sS, s);
+          break;
       }
     }
     int64_t bo = 0;

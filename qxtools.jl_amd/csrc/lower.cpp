@@ -698,6 +698,7 @@ Step chain_step(const Program &p, const Lowered &L, const ChainBuild &cb) {
     flops += 8.0 * m.H * m.M * m.N * m.K;
     elems += (double)value_extent(p, L.values[s.small]);
   }
+  a.s_elems = (int)st.ws_elems;
   elems += (double)value_extent(p, L.values[cb.x0]) + (double)last.c_elems;
   st.chain_flops = flops;
   st.chain_elems = elems;

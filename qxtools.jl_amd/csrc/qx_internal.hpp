@@ -153,6 +153,7 @@ struct ChainArgs {
   int nsteps = 0, nouter = 0;
   int grid = 256;          // workgroups (each walks tiles); tuned
   int mfma16 = 0;          // steps with 16 contracted values on matrix cores (else vector FMAs)
+  int s_elems = 0;         // elements of all packed S_j (staged in LDS when they fit)
   int64_t win[kChainTileBits] = {0}, wout[kChainTileBits] = {0};   // tile bit -> X_0 / X_m stride
   int32_t pout[kChainTileBits] = {0};  // store order: i-th fastest X_m tile bit -> tile bit
   int64_t oin[kMaxBits] = {0}, oout[kMaxBits] = {0};   // outer bits (workgroup index) -> strides
