@@ -880,15 +880,16 @@ void level_small_steps(const Program &p, Lowered &L) {
 }  // namespace
 
 // Leg orders (positions in each node's own dims, fastest first) that turn large
-// contractions into plain column-major matrix products, which the vendor BLAS runs
-// without copies.  For C = A.B with legs M (A only), N (B only) and K (contracted),
-// A's legs are regrouped into a K run and an M run (each keeping its relative order,
-// the run holding A's fastest leg first), B's K legs follow A's order next to its N
-// run, and C holds the M run in A's order and the N run in B's order.  Only
-// intermediates that the contraction alone consumes are regrouped; C keeps its other
-// consumers, which read it through its strides.  The batched bitstring leg stays
-// fastest (slowest with batch_last), so its class's run is placed beside it.
-// Contractions with batch legs (in A, B and C) are left alone.  Empty = DSL order.
+// contractions into plain (strided-batched) column-major matrix products, which the
+// vendor BLAS runs without copies.  For C = A.B with legs M (A only), N (B only), K
+// (contracted) and H (all three), A's legs are regrouped into a K run and an M run
+// (each keeping its relative order, the run holding A's fastest leg first) followed by
+// the H run, B's K legs follow A's order next to its N run, and C holds the M run in
+// A's order and the N run in B's order; the H run comes last everywhere, in one order.
+// Only intermediates that the contraction alone consumes are regrouped; a result
+// that is saved, viewed or already regrouped keeps its layout and the operands follow
+// it.  The batched bitstring leg stays fastest (slowest with batch_last): its class's
+// run is placed beside it, and as an H leg it must be slowest.  Empty = DSL order.
 static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::vector<std::vector<int64_t>> &shape,
                                                    int64_t batch, bool batch_last) {
   const size_t nn = p.nodes.size();
@@ -899,6 +900,16 @@ static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::
     if (n.op == Op::Ncon) uses[n.ia]++, uses[n.ib]++;
     if (n.op == Op::View || n.op == Op::Save) uses[n.ia]++, pinned[n.ia] = 1;
   }
+  auto find = [](const std::vector<int> &v, int l) { return (int)(std::find(v.begin(), v.end(), l) - v.begin()); };
+  auto cat = [](std::vector<int> x, const std::vector<int> &y) {
+    x.insert(x.end(), y.begin(), y.end());
+    return x;
+  };
+  auto fastest_in = [](const std::vector<int64_t> &dims, const std::vector<int> &grp) {
+    for (size_t j = 0; j < dims.size(); ++j)
+      if (dims[j] > 1) return std::find(grp.begin(), grp.end(), (int)j) != grp.end();
+    return true;
+  };
   for (size_t i = 0; i < nn; ++i) {
     const Node &n = p.nodes[i];
     if (n.op != Op::Ncon || n.ia == n.ib) continue;
@@ -910,19 +921,19 @@ static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::
     char beta = 0;   // class of the batched bitstring leg
     if (!da && db && dc) beta = 'N';
     else if (da && !db && dc) beta = 'M';
+    else if (da && db && dc) beta = 'H';
     else if (da || db || dc) continue;
+    if (beta == 'H' && !batch_last) continue;   // a fastest batch leg breaks the matrices
     const std::vector<int64_t> &sa = shape[n.ia], &sb = shape[n.ib], &sc = shape[i];
-    auto find = [](const std::vector<int> &v, int l) {
-      return (int)(std::find(v.begin(), v.end(), l) - v.begin());
-    };
-    std::vector<int> aK, aM, bN;
+    std::vector<int> aK, aM, aH, bN;
     double M = beta == 'M' ? (double)batch : 1.0, N = beta == 'N' ? (double)batch : 1.0, K = 1.0;
+    double H = beta == 'H' ? (double)batch : 1.0;
     bool ok = true;
     for (size_t j = 0; j < n.alab.size(); ++j) {
       const int l = n.alab[j];
       ok &= pow2(sa[j]);
       const bool inb = find(n.blab, l) < (int)n.blab.size(), inc = find(n.clab, l) < (int)n.clab.size();
-      if (inb && inc) ok &= sa[j] == 1;   // batch leg of all three
+      if (inb && inc) aH.push_back((int)j), H *= (double)sa[j];
       else if (inb) aK.push_back((int)j), K *= (double)sa[j];
       else aM.push_back((int)j), M *= (double)sa[j];
     }
@@ -930,52 +941,57 @@ static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::
       ok &= pow2(sb[j]);
       if (find(n.alab, n.blab[j]) == (int)n.alab.size()) bN.push_back((int)j), N *= (double)sb[j];
     }
-    if (!ok || M < 64 || N < 64 || K < 64 || 8.0 * M * N * K < 1e9) continue;
-    std::vector<int> bK, cM, cN;
-    for (int j : aK) bK.push_back(find(n.blab, n.alab[j]));
-    for (int j : aM) cM.push_back(find(n.clab, n.alab[j]));
-    for (int j : bN) cN.push_back(find(n.clab, n.blab[j]));
-    if ((int)(cM.size() + cN.size()) != (int)sc.size()) continue;
+    if (!ok || M < 64 || N < 64 || K < 64 || 8.0 * H * M * N * K < 1e9) continue;
+    bool has_h = beta == 'H';
+    for (int j : aH) has_h |= sa[j] > 1;
+    if (has_h && (beta == 'M' || beta == 'N')) continue;   // the batch leg would split a run from H
+    std::vector<int> cs;   // C's storage order
     const bool c_fixed = pinned[i] || !order[i].empty();
-    bool c_m_first_fixed = true;
     if (c_fixed) {
-      // C keeps its layout: it must hold an M run and an N run (the batch leg's run on
-      // its side), and A's M legs and B's N legs take C's orders
-      std::vector<int> cs = order[i];
+      // C keeps its layout: an M run and an N run (the batch leg's run on its side),
+      // then the H run; A and B take C's orders
+      cs = order[i];
       if (cs.empty())
         for (size_t j = 0; j < sc.size(); ++j) cs.push_back((int)j);
-      std::vector<int> seq;   // classes along C's storage, legs of dim 1 skipped: 0 = M, 1 = N
-      for (int j : cs)
-        if (sc[j] > 1) seq.push_back(std::find(cM.begin(), cM.end(), j) != cM.end() ? 0 : 1);
+      std::vector<int> seq;   // classes along C's storage, legs of dim 1 skipped: 0 M, 1 N, 2 H
+      for (int j : cs) {
+        if (sc[j] <= 1) continue;
+        const int l = n.clab[j];
+        seq.push_back(find(n.alab, l) < (int)n.alab.size() ? (find(n.blab, l) < (int)n.blab.size() ? 2 : 0) : 1);
+      }
+      size_t e = seq.size();
+      while (e > 0 && seq[e - 1] == 2) --e;
       int changes = 0;
-      for (size_t k = 1; k < seq.size(); ++k) changes += seq[k] != seq[k - 1];
-      if (changes > 1 || seq.empty()) continue;
-      c_m_first_fixed = seq[0] == 0;
-      if (beta && changes == 1 && (c_m_first_fixed == (beta == 'M')) == batch_last) continue;
+      bool stray_h = false;
+      for (size_t k = 0; k < e; ++k) {
+        stray_h |= seq[k] == 2;
+        if (k) changes += seq[k] != seq[k - 1];
+      }
+      if (stray_h || changes > 1 || e == 0) continue;
+      const bool m_first = seq[0] == 0;
+      if ((beta == 'M' || beta == 'N') && changes == 1 && (m_first == (beta == 'M')) == batch_last) continue;
       auto by_c = [&](std::vector<int> &legs, const std::vector<int> &lab) {
         std::stable_sort(legs.begin(), legs.end(), [&](int x, int y) {
           return find(cs, find(n.clab, lab[x])) < find(cs, find(n.clab, lab[y]));
         });
       };
       by_c(aM, n.alab);
+      by_c(aH, n.alab);
       by_c(bN, n.blab);
     }
-    auto cat = [](std::vector<int> x, const std::vector<int> &y) {
-      x.insert(x.end(), y.begin(), y.end());
-      return x;
-    };
-    auto fastest_in = [](const std::vector<int64_t> &dims, const std::vector<int> &grp) {
-      for (size_t j = 0; j < dims.size(); ++j)
-        if (dims[j] > 1) return std::find(grp.begin(), grp.end(), (int)j) != grp.end();
-      return true;
-    };
+    std::vector<int> bK, bH, cM, cN, cH;
+    for (int j : aK) bK.push_back(find(n.blab, n.alab[j]));
+    for (int j : aH) bH.push_back(find(n.blab, n.alab[j])), cH.push_back(find(n.clab, n.alab[j]));
+    for (int j : aM) cM.push_back(find(n.clab, n.alab[j]));
+    for (int j : bN) cN.push_back(find(n.clab, n.blab[j]));
+    if (cM.size() + cN.size() + cH.size() != sc.size()) continue;
     // the run next to the batch leg: first when it is fastest, last when slowest
     const bool a_m_first = beta == 'M' ? !batch_last : fastest_in(sa, aM);
     const bool b_n_first = beta == 'N' ? !batch_last : fastest_in(sb, bN);
     const bool c_m_first = beta == 'M' ? !batch_last : beta == 'N' ? batch_last : true;
-    order[n.ia] = a_m_first ? cat(aM, aK) : cat(aK, aM);
-    order[n.ib] = b_n_first ? cat(bN, bK) : cat(bK, bN);
-    if (!c_fixed) order[i] = c_m_first ? cat(cM, cN) : cat(cN, cM);
+    order[n.ia] = cat(a_m_first ? cat(aM, aK) : cat(aK, aM), aH);
+    order[n.ib] = cat(b_n_first ? cat(bN, bK) : cat(bK, bN), bH);
+    if (!c_fixed) order[i] = cat(c_m_first ? cat(cM, cN) : cat(cN, cM), cH);
   }
   return order;
 }
