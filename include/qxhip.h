@@ -175,6 +175,12 @@ void qx_plan_destroy(qx_plan *plan);
 qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t batch, uint64_t fused,
                          char *json, size_t cap);
 
+/* qx_dsl_inspect with lowering flags: bit 0 = the "batch_last" plan option (batched
+ * bitstring leg slowest in bitstring-dependent intermediates).  No reference
+ * counterpart. */
+qx_status qx_dsl_inspect_ex(const char *dsl, size_t len, int precision, int64_t batch, uint64_t fused,
+                            int64_t flags, char *json, size_t cap);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *qx_last_error(void);
 

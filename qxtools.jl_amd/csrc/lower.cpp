@@ -882,14 +882,18 @@ void level_small_steps(const Program &p, Lowered &L) {
 // Leg orders (positions in each node's own dims, fastest first) that turn large
 // contractions into plain (strided-batched) column-major matrix products, which the
 // vendor BLAS runs without copies.  For C = A.B with legs M (A only), N (B only), K
-// (contracted) and H (all three), A's legs are regrouped into a K run and an M run
-// (each keeping its relative order, the run holding A's fastest leg first) followed by
-// the H run, B's K legs follow A's order next to its N run, and C holds the M run in
-// A's order and the N run in B's order; the H run comes last everywhere, in one order.
-// Only intermediates that the contraction alone consumes are regrouped; a result
-// that is saved, viewed or already regrouped keeps its layout and the operands follow
-// it.  The batched bitstring leg stays fastest (slowest with batch_last): its class's
-// run is placed beside it, and as an H leg it must be slowest.  Empty = DSL order.
+// (contracted) and H (all three), every operand must hold two runs of its matrix legs
+// (A: M and K, B: K and N, C: M and N, each run in one order shared by the operands
+// that hold it) followed by the H run (one order everywhere).  An operand whose layout
+// is fixed (a data tensor, a value with other consumers, a saved value, or one
+// regrouped for a larger contraction) must already have that form and sets the run
+// orders; the others (intermediates this contraction alone reads, and its own result)
+// are regrouped to match, keeping their own relative orders where nothing is set and
+// putting first the run that holds an operand's fastest leg (a result: its M run).  The batched bitstring leg
+// stays fastest (slowest with batch_last): its class's run is placed beside it, and
+// as an H leg it must be slowest.  Largest contractions (by flops) go first.  Views
+// and contractions below 2^30 flops or with M, N or K under 64 are left alone.
+// Empty order = DSL order.
 static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::vector<std::vector<int64_t>> &shape,
                                                    int64_t batch, bool batch_last) {
   const size_t nn = p.nodes.size();
@@ -900,23 +904,23 @@ static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::
     if (n.op == Op::Ncon) uses[n.ia]++, uses[n.ib]++;
     if (n.op == Op::View || n.op == Op::Save) uses[n.ia]++, pinned[n.ia] = 1;
   }
-  auto find = [](const std::vector<int> &v, int l) { return (int)(std::find(v.begin(), v.end(), l) - v.begin()); };
-  auto cat = [](std::vector<int> x, const std::vector<int> &y) {
-    x.insert(x.end(), y.begin(), y.end());
-    return x;
+  auto has = [](const std::vector<int> &v, int l) { return std::find(v.begin(), v.end(), l) != v.end(); };
+  auto pos = [](const std::vector<int> &v, int l) { return (int)(std::find(v.begin(), v.end(), l) - v.begin()); };
+  // class of a label of contraction n: M (A only), N (B only), K (A and B), H (all)
+  auto cls = [&](const Node &n, int l) {
+    const bool a = has(n.alab, l), b = has(n.blab, l), c = has(n.clab, l);
+    return a && b ? (c ? 'H' : 'K') : a ? 'M' : 'N';
   };
-  auto fastest_in = [](const std::vector<int64_t> &dims, const std::vector<int> &grp) {
-    for (size_t j = 0; j < dims.size(); ++j)
-      if (dims[j] > 1) return std::find(grp.begin(), grp.end(), (int)j) != grp.end();
-    return true;
+  struct Cand {
+    size_t node;
+    double flops;
+    char beta;
   };
+  std::vector<Cand> cands;
   for (size_t i = 0; i < nn; ++i) {
     const Node &n = p.nodes[i];
     if (n.op != Op::Ncon || n.ia == n.ib) continue;
-    auto free_operand = [&](int x) {
-      return p.nodes[x].op == Op::Ncon && uses[x] == 1 && order[x].empty() && !pinned[x];
-    };
-    if (!free_operand(n.ia) || !free_operand(n.ib)) continue;
+    if (p.nodes[n.ia].op == Op::View || p.nodes[n.ib].op == Op::View) continue;
     const bool da = p.nodes[n.ia].dep, db = p.nodes[n.ib].dep, dc = n.dep;
     char beta = 0;   // class of the batched bitstring leg
     if (!da && db && dc) beta = 'N';
@@ -924,74 +928,116 @@ static std::vector<std::vector<int>> plain_layouts(const Program &p, const std::
     else if (da && db && dc) beta = 'H';
     else if (da || db || dc) continue;
     if (beta == 'H' && !batch_last) continue;   // a fastest batch leg breaks the matrices
-    const std::vector<int64_t> &sa = shape[n.ia], &sb = shape[n.ib], &sc = shape[i];
-    std::vector<int> aK, aM, aH, bN;
-    double M = beta == 'M' ? (double)batch : 1.0, N = beta == 'N' ? (double)batch : 1.0, K = 1.0;
-    double H = beta == 'H' ? (double)batch : 1.0;
-    bool ok = true;
+    double prod[4] = {1, 1, 1, 1};               // M N K H
+    auto slot = [](char c) { return c == 'M' ? 0 : c == 'N' ? 1 : c == 'K' ? 2 : 3; };
+    if (beta) prod[slot(beta)] *= (double)batch;
+    bool ok = true, has_h = beta == 'H';
     for (size_t j = 0; j < n.alab.size(); ++j) {
-      const int l = n.alab[j];
-      ok &= pow2(sa[j]);
-      const bool inb = find(n.blab, l) < (int)n.blab.size(), inc = find(n.clab, l) < (int)n.clab.size();
-      if (inb && inc) aH.push_back((int)j), H *= (double)sa[j];
-      else if (inb) aK.push_back((int)j), K *= (double)sa[j];
-      else aM.push_back((int)j), M *= (double)sa[j];
+      const int64_t d = shape[n.ia][j];
+      ok &= pow2(d);
+      const char c = cls(n, n.alab[j]);
+      prod[slot(c)] *= (double)d;
+      has_h |= c == 'H' && d > 1;
     }
     for (size_t j = 0; j < n.blab.size(); ++j) {
-      ok &= pow2(sb[j]);
-      if (find(n.alab, n.blab[j]) == (int)n.alab.size()) bN.push_back((int)j), N *= (double)sb[j];
+      ok &= pow2(shape[n.ib][j]);
+      if (cls(n, n.blab[j]) == 'N') prod[1] *= (double)shape[n.ib][j];
     }
-    if (!ok || M < 64 || N < 64 || K < 64 || 8.0 * H * M * N * K < 1e9) continue;
-    bool has_h = beta == 'H';
-    for (int j : aH) has_h |= sa[j] > 1;
+    if (!ok || prod[0] < 64 || prod[1] < 64 || prod[2] < 64) continue;
     if (has_h && (beta == 'M' || beta == 'N')) continue;   // the batch leg would split a run from H
-    std::vector<int> cs;   // C's storage order
-    const bool c_fixed = pinned[i] || !order[i].empty();
-    if (c_fixed) {
-      // C keeps its layout: an M run and an N run (the batch leg's run on its side),
-      // then the H run; A and B take C's orders
-      cs = order[i];
-      if (cs.empty())
-        for (size_t j = 0; j < sc.size(); ++j) cs.push_back((int)j);
-      std::vector<int> seq;   // classes along C's storage, legs of dim 1 skipped: 0 M, 1 N, 2 H
-      for (int j : cs) {
-        if (sc[j] <= 1) continue;
-        const int l = n.clab[j];
-        seq.push_back(find(n.alab, l) < (int)n.alab.size() ? (find(n.blab, l) < (int)n.blab.size() ? 2 : 0) : 1);
-      }
-      size_t e = seq.size();
-      while (e > 0 && seq[e - 1] == 2) --e;
+    const double fl = 8.0 * prod[0] * prod[1] * prod[2] * prod[3];
+    if (fl >= 1e9) cands.push_back({i, fl, beta});
+  }
+  std::stable_sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.flops > y.flops; });
+
+  for (const Cand &cd : cands) {
+    const size_t i = cd.node;
+    const Node &n = p.nodes[i];
+    const char beta = cd.beta;
+    struct Op3 {
+      int node;
+      const std::vector<int> *lab;
+      bool fixed;
+      char r1, r2;            // the two matrix runs of this operand (A: M K, B: N K, C: M N)
+      std::vector<int> store;  // labels of legs with dim > 1, fastest first
+    };
+    Op3 ops[3] = {{n.ia, &n.alab, false, 'M', 'K', {}}, {n.ib, &n.blab, false, 'N', 'K', {}},
+                  {(int)i, &n.clab, false, 'M', 'N', {}}};
+    bool ok = true;
+    for (int o = 0; o < 3; ++o) {
+      Op3 &x = ops[o];
+      const Node &xn = p.nodes[x.node];
+      x.fixed = o == 2 ? (pinned[i] || !order[i].empty())
+                       : !(xn.op == Op::Ncon && uses[x.node] == 1 && order[x.node].empty() && !pinned[x.node]);
+      const std::vector<int64_t> &dims = shape[x.node];
+      std::vector<int> st = order[x.node];
+      if (st.empty())
+        for (size_t j = 0; j < dims.size(); ++j) st.push_back((int)j);
+      for (int j : st)
+        if (dims[j] > 1) x.store.push_back((*x.lab)[j]);
+      if (!x.fixed) continue;
+      // fixed: matrix runs (at most one change of class) then the H run
+      size_t e = x.store.size();
+      while (e > 0 && cls(n, x.store[e - 1]) == 'H') --e;
       int changes = 0;
-      bool stray_h = false;
       for (size_t k = 0; k < e; ++k) {
-        stray_h |= seq[k] == 2;
-        if (k) changes += seq[k] != seq[k - 1];
+        if (cls(n, x.store[k]) == 'H') ok = false;
+        if (k && cls(n, x.store[k]) != cls(n, x.store[k - 1])) ++changes;
       }
-      if (stray_h || changes > 1 || e == 0) continue;
-      const bool m_first = seq[0] == 0;
-      if ((beta == 'M' || beta == 'N') && changes == 1 && (m_first == (beta == 'M')) == batch_last) continue;
-      auto by_c = [&](std::vector<int> &legs, const std::vector<int> &lab) {
-        std::stable_sort(legs.begin(), legs.end(), [&](int x, int y) {
-          return find(cs, find(n.clab, lab[x])) < find(cs, find(n.clab, lab[y]));
-        });
-      };
-      by_c(aM, n.alab);
-      by_c(aH, n.alab);
-      by_c(bN, n.blab);
+      if (changes > 1) ok = false;
+      // the batch leg sits beside its class's run
+      const bool beta_here = (beta == x.r1 || beta == x.r2) && beta != 'H';
+      if (beta_here && e > 0 && changes == 1) {
+        const char edge = batch_last ? cls(n, x.store[e - 1]) : cls(n, x.store[0]);
+        if (edge != beta) ok = false;
+      }
     }
-    std::vector<int> bK, bH, cM, cN, cH;
-    for (int j : aK) bK.push_back(find(n.blab, n.alab[j]));
-    for (int j : aH) bH.push_back(find(n.blab, n.alab[j])), cH.push_back(find(n.clab, n.alab[j]));
-    for (int j : aM) cM.push_back(find(n.clab, n.alab[j]));
-    for (int j : bN) cN.push_back(find(n.clab, n.blab[j]));
-    if (cM.size() + cN.size() + cH.size() != sc.size()) continue;
-    // the run next to the batch leg: first when it is fastest, last when slowest
-    const bool a_m_first = beta == 'M' ? !batch_last : fastest_in(sa, aM);
-    const bool b_n_first = beta == 'N' ? !batch_last : fastest_in(sb, bN);
-    const bool c_m_first = beta == 'M' ? !batch_last : beta == 'N' ? batch_last : true;
-    order[n.ia] = cat(a_m_first ? cat(aM, aK) : cat(aK, aM), aH);
-    order[n.ib] = cat(b_n_first ? cat(bN, bK) : cat(bK, bN), bH);
-    if (!c_fixed) order[i] = cat(c_m_first ? cat(cM, cN) : cat(cN, cM), cH);
+    if (!ok) continue;
+    // run orders: from fixed operands (which must agree), else the first holder's order
+    std::map<char, std::vector<int>> run;
+    for (char c : {'K', 'M', 'N', 'H'}) {
+      bool set = false;
+      for (int o = 0; o < 3 && ok; ++o) {
+        Op3 &x = ops[o];
+        if (!x.fixed || (c != x.r1 && c != x.r2 && c != 'H')) continue;
+        std::vector<int> v;
+        for (int l : x.store)
+          if (cls(n, l) == c) v.push_back(l);
+        if (set && v != run[c]) ok = false;
+        run[c] = v, set = true;
+      }
+      if (set) continue;
+      for (int o = 0; o < 3 && !set; ++o) {
+        Op3 &x = ops[o];
+        if (c != x.r1 && c != x.r2 && c != 'H') continue;
+        std::vector<int> v;
+        for (int l : x.store)
+          if (cls(n, l) == c) v.push_back(l);
+        run[c] = v, set = true;
+      }
+    }
+    if (!ok) continue;
+    for (int o = 0; o < 3; ++o) {
+      Op3 &x = ops[o];
+      if (x.fixed) continue;
+      // the run next to the batch leg goes on its side, else the run of an operand's
+      // fastest leg; results put M first (measured better for their consumers)
+      bool r1_first = o == 2 || x.store.empty() || cls(n, x.store[0]) == x.r1;
+      if (beta == x.r1) r1_first = !batch_last;
+      else if (beta == x.r2) r1_first = batch_last;
+      std::vector<int> labs = r1_first ? run[x.r1] : run[x.r2];
+      const std::vector<int> &second = r1_first ? run[x.r2] : run[x.r1];
+      labs.insert(labs.end(), second.begin(), second.end());
+      labs.insert(labs.end(), run['H'].begin(), run['H'].end());
+      std::vector<int> ord;
+      for (int l : labs) ord.push_back(pos(*x.lab, l));
+      if (ord.size() != x.store.size()) {
+        ok = false;
+        break;
+      }
+      order[x.node] = ord;
+    }
+    if (!ok) throw Error(QX_ERR_STATE, "internal: layout regrouping lost a leg");
   }
   return order;
 }

@@ -1375,12 +1375,19 @@ const char *qx_last_error(void) { return g_last_error.c_str(); }
 
 qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t batch, uint64_t fused, char *json,
                          size_t cap) {
+  return qx_dsl_inspect_ex(dsl, len, precision, batch, fused, 0, json, cap);
+}
+
+qx_status qx_dsl_inspect_ex(const char *dsl, size_t len, int precision, int64_t batch, uint64_t fused,
+                            int64_t flags, char *json, size_t cap) {
   return guard([&] {
+    if (flags & ~int64_t(1)) throw qx::Error(QX_ERR_ARGUMENT, "unknown inspect flags");
+    const bool batch_last = flags & 1;
     if (!json || cap == 0) throw qx::Error(QX_ERR_ARGUMENT, "null output buffer");
     if (precision < QX_C128 || precision > QX_C64_BF16X3) throw qx::Error(QX_ERR_ARGUMENT, "unknown precision");
     if (precision == QX_C64_BF16X3) throw qx::Error(QX_ERR_UNSUPPORTED, "precision QX_C64_BF16X3 is not implemented");
     qx::Program prog = qx::parse_program(dsl, len);
-    qx::Lowered L = qx::lower(prog, batch, false, precision, fused);
+    qx::Lowered L = qx::lower(prog, batch, false, precision, fused, true, batch_last);
     std::ostringstream os;
     os.precision(17);
     lowered_json(os, prog, L, precision == QX_C128 ? 16 : 8, true);
@@ -1388,7 +1395,7 @@ qx_status qx_dsl_inspect(const char *dsl, size_t len, int precision, int64_t bat
     // reverse-mode lowering (its own host-side bounds proof runs here too)
     std::string v;
     try {
-      qx::VjpLowered V = qx::lower_vjp(prog, batch, precision, fused);
+      qx::VjpLowered V = qx::lower_vjp(prog, batch, precision, fused, batch_last);
       size_t nb = V.bsteps[0].size() + V.bsteps[1].size() + V.bsteps[2].size();
       v = ",\"vjp\":{\"backward_steps\":" + std::to_string(nb) + ",\"forward_elems\":" +
           std::to_string(V.fwd.arena_elems) + ",\"cot_elems\":" + std::to_string(V.cot_elems) +

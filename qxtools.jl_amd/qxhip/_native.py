@@ -19,7 +19,7 @@ SYMBOLS = [
     "qx_ctx_create", "qx_ctx_destroy", "qx_plan_from_dsl", "qx_plan_set_tensor",
     "qx_plan_set_option", "qx_plan_finalize", "qx_plan_info", "qx_plan_key",
     "qx_amplitudes", "qx_amplitudes_vjp", "qx_plan_stats", "qx_plan_destroy", "qx_last_error",
-    "qx_dsl_inspect", "qx_amplitudes_uniform", "qx_sample_rejection",
+    "qx_dsl_inspect", "qx_dsl_inspect_ex", "qx_amplitudes_uniform", "qx_sample_rejection",
 ]
 
 
@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
         L.qx_plan_destroy.argtypes = [p]
         L.qx_plan_destroy.restype = None
         L.qx_dsl_inspect.argtypes = [C.c_char_p, sz, C.c_int, i64, C.c_uint64, C.c_char_p, sz]
+        L.qx_dsl_inspect_ex.argtypes = [C.c_char_p, sz, C.c_int, i64, C.c_uint64, i64, C.c_char_p, sz]
         L.qx_amplitudes_uniform.argtypes = [p, i64, C.c_uint64, C.c_char_p, dp]
         L.qx_sample_rejection.argtypes = [p, i64, C.c_uint64, C.c_double, C.c_int, i64, C.c_char_p, dp, dp,
                                           C.POINTER(i64)]
@@ -76,14 +77,15 @@ def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def inspect(dsl: str, precision: str = "c128", batch: int = 1, fused: int = 0) -> dict:
+def inspect(dsl: str, precision: str = "c128", batch: int = 1, fused: int = 0, batch_last: bool = False) -> dict:
     """Lower a program on the host (no device): steps, kernels, arena schedule.
-    `fused`: bit mask of slice variables contracted inline."""
+    `fused`: bit mask of slice variables contracted inline; `batch_last`: the plan
+    option of that name."""
     b = dsl.encode()
     cap = 1 << 16
     while True:
         buf = C.create_string_buffer(cap)
-        st = lib().qx_dsl_inspect(b, len(b), PRECISIONS[precision], batch, fused, buf, cap)
+        st = lib().qx_dsl_inspect_ex(b, len(b), PRECISIONS[precision], batch, fused, int(bool(batch_last)), buf, cap)
         if st == 0:
             return json.loads(buf.value.decode())
         if st != -1 or cap > (1 << 28):
