@@ -1048,55 +1048,74 @@ __device__ __forceinline__ void lds_barrier() {
 // Step j's S, packed densely as [h][nu][kappa] at Sp + a.st[j].soff, is read with
 // wave-uniform indices (the batch bits h are placed on wave-uniform group bits), so
 // its values come through the scalar cache rather than LDS.
-template <typename T, int K>
+//
+// LDS addresses: tile element i lives at P(i), the XOR of swz[b] over the set bits b
+// of i (swz[b] = 2^b plus, for some bits, a pattern in the low bits that spreads a
+// step's lanes over the bank groups; P is a bijection of the tile).  An element's
+// address is therefore the XOR of the images of its bit groups.
+// LDS image of tile bit b and the combination of images of disjoint bit sets: with
+// SWZ the host's swz table and XOR, else the plain bit and OR (cheaper addressing for
+// tiles whose accesses have no bank conflicts anyway).
+template <bool SWZ>
+__device__ __forceinline__ uint32_t tile_img(const int32_t *swz, int b) {
+  return SWZ ? (uint32_t)swz[b] : 1u << b;
+}
+template <bool SWZ>
+__device__ __forceinline__ uint32_t tile_cmb(uint32_t x, uint32_t y) {
+  return SWZ ? x ^ y : x | y;
+}
+
+template <typename T, int K, bool SWZ>
 __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const typename Vec<T>::type *__restrict__ Sp,
-                                            const ChainStepArgs &s) {
+                                            const ChainStepArgs &s, const int32_t *swz) {
   using V = typename Vec<T>::type;
   constexpr int LK = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : K == 16 ? 4 : 5;
   const int t = threadIdx.x;
   uint32_t kmask = 0, hmask = 0;
   for (int i = 0; i < LK; ++i) kmask |= 1u << s.q[i];
   for (int i = 0; i < s.h; ++i) hmask |= 1u << s.hq[i];
+  const int nfree = kChainTileBits - LK - s.h;
   auto koff = [&](int kap) {
     uint32_t o = 0;
 #pragma unroll
     for (int i = 0; i < LK; ++i)
-      if ((kap >> i) & 1) o |= 1u << s.q[i];
+      if ((kap >> i) & 1) o = tile_cmb<SWZ>(o, tile_img<SWZ>(swz, s.q[i]));
     return o;
   };
   // group index bits -> tile bits outside q: other positions first, batch bits last
+  // (the LDS address of the group's element kappa = 0)
   auto deposit = [&](uint32_t g) {
     uint32_t base = 0;
     int j = 0;
     for (int b = 0; b < kChainTileBits; ++b)
       if (!(((kmask | hmask) >> b) & 1)) {
-        if ((g >> j) & 1) base |= 1u << b;
+        if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, b));
         ++j;
       }
     for (int i = 0; i < s.h; ++i, ++j)
-      if ((g >> j) & 1) base |= 1u << s.hq[i];
+      if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, s.hq[i]));
     return base;
   };
-  auto srow = [&](uint32_t base) {
-    int h = 0;
-    for (int i = 0; i < s.h; ++i) h |= (int)((base >> s.hq[i]) & 1) << i;
+  auto srow = [&](uint32_t g) {
+    int h = (int)(g >> nfree) & ((1 << s.h) - 1);
     h = __builtin_amdgcn_readfirstlane(h);
     return Sp + s.soff + h * K * K;
   };
   static_assert(K <= kChainPer, "larger steps run on matrix cores");
   // whole groups per lane, updated in place
   for (int i = 0; i < kChainPer / K; ++i) {
-    const uint32_t base = deposit((uint32_t)(t + kChainThreads * i));
-    const V *Sr = srow(base);
+    const uint32_t g = (uint32_t)(t + kChainThreads * i);
+    const uint32_t base = deposit(g);
+    const V *Sr = srow(g);
     V in[K];
 #pragma unroll
-    for (int kap = 0; kap < K; ++kap) in[kap] = tile[base | koff(kap)];
+    for (int kap = 0; kap < K; ++kap) in[kap] = tile[tile_cmb<SWZ>(base, koff(kap))];
 #pragma unroll
     for (int nu = 0; nu < K; ++nu) {
       V acc{0, 0};
 #pragma unroll
       for (int kap = 0; kap < K; ++kap) cmac<T>(acc, Sr[nu * K + kap], in[kap], 0, 0);
-      tile[base | koff(nu)] = acc;
+      tile[tile_cmb<SWZ>(base, koff(nu))] = acc;
     }
   }
   lds_barrier();
@@ -1108,10 +1127,11 @@ __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const t
 // block; its 64 groups share the batch bits, so its S fragments (B operands: S at
 // nu = lane % 16, kappa = 4 kb + lane / 16) stay in registers for the whole step.
 // Operand and result layouts as in Mfma16.
-template <typename T, int K, bool SL>
+template <typename T, int K, bool SL, bool SWZ>
 __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
                                                  const typename Vec<T>::type *__restrict__ Sp,
-                                                 const typename Vec<T>::type *sS, const ChainStepArgs &s) {
+                                                 const typename Vec<T>::type *sS, const ChainStepArgs &s,
+                                                 const int32_t *swz) {
   using V = typename Vec<T>::type;
   using M = Mfma16<T>;
   using v4 = typename M::v4;
@@ -1126,11 +1146,12 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
   uint32_t kmask = 0, hmask = 0;
   for (int i = 0; i < LK; ++i) kmask |= 1u << s.q[i];
   for (int i = 0; i < s.h; ++i) hmask |= 1u << s.hq[i];
+  const int nfree = kChainTileBits - LK - s.h;
   auto koff = [&](int kap) {
     uint32_t o = 0;
 #pragma unroll
     for (int i = 0; i < LK; ++i)
-      if ((kap >> i) & 1) o |= 1u << s.q[i];
+      if ((kap >> i) & 1) o = tile_cmb<SWZ>(o, tile_img<SWZ>(swz, s.q[i]));
     return o;
   };
   auto deposit = [&](uint32_t g) {
@@ -1138,11 +1159,11 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
     int j = 0;
     for (int b = 0; b < kChainTileBits; ++b)
       if (!(((kmask | hmask) >> b) & 1)) {
-        if ((g >> j) & 1) base |= 1u << b;
+        if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, b));
         ++j;
       }
     for (int i = 0; i < s.h; ++i, ++j)
-      if ((g >> j) & 1) base |= 1u << s.hq[i];
+      if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, s.hq[i]));
     return base;
   };
   // the low four group bits (a row inside a 16-row block) go to the first four
@@ -1152,7 +1173,7 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
     int j = 0;
     for (int b = 0; b < kChainTileBits && j < 4; ++b)
       if (!(((kmask | hmask) >> b) & 1)) {
-        const uint32_t bit = 1u << b;
+        const uint32_t bit = tile_img<SWZ>(swz, b);
         if (j == 0) p0 = bit;
         else if (j == 1) p1 = bit;
         else if (j == 2) p2 = bit;
@@ -1161,13 +1182,13 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
       }
   }
   auto scatter4 = [&](int row) {
-    return ((row & 1) ? p0 : 0u) | ((row & 2) ? p1 : 0u) | ((row & 4) ? p2 : 0u) | ((row & 8) ? p3 : 0u);
+    return tile_cmb<SWZ>(tile_cmb<SWZ>((row & 1) ? p0 : 0u, (row & 2) ? p1 : 0u),
+                         tile_cmb<SWZ>((row & 4) ? p2 : 0u, (row & 8) ? p3 : 0u));
   };
   uint32_t rbase[RT];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) rbase[rt] = deposit((uint32_t)(g0 + 16 * rt));
-  int h = 0;
-  for (int i = 0; i < s.h; ++i) h |= (int)((rbase[0] >> s.hq[i]) & 1) << i;
+  const int h = (g0 >> nfree) & ((1 << s.h) - 1);   // the wave's groups share their batch bits
   const int so = s.soff + h * K * K + (16 * nb + li) * K + lk;
   T bre[KB], bim[KB];
 #pragma unroll
@@ -1181,11 +1202,11 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
   v4 cre[RT], cim[RT];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) {
-    const uint32_t base = rbase[rt] | arow;
+    const uint32_t base = tile_cmb<SWZ>(rbase[rt], arow);
     v4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
-      const V a = tile[base | koff(4 * kb + lk)];
+      const V a = tile[tile_cmb<SWZ>(base, koff(4 * kb + lk))];
       xr = M::mma(a.x, bre[kb], xr);
       xr = M::mma(-a.y, bim[kb], xr);
       xi = M::mma(a.x, bim[kb], xi);
@@ -1205,17 +1226,18 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
       V o;
       o.x = cre[rt][r];
       o.y = cim[rt][r];
-      tile[rbase[rt] | scatter4(M::row(lk, r)) | ocol] = o;
+      tile[tile_cmb<SWZ>(tile_cmb<SWZ>(rbase[rt], scatter4(M::row(lk, r))), ocol)] = o;
     }
   lds_barrier();
 }
 
 // Gather every S_j of a chain into the dense [h][nu][kappa] blocks the chain
-// kernel reads.
+// kernel reads: step j = blockIdx.y, elements over blockIdx.x (at most kChainMaxS).
 template <typename T>
 __global__ void k_chain_pack(typename Vec<T>::type *__restrict__ Sp, ChainArgs a) {
   using V = typename Vec<T>::type;
-  for (int j = 0; j < a.nsteps; ++j) {
+  {
+    const int j = blockIdx.y;
     const ChainStepArgs &s = a.st[j];
     const V *S = (const V *)a.S[j];
     const int kk = 1 << s.k, tot = (1 << s.h) * kk * kk;
@@ -1233,7 +1255,9 @@ __global__ void k_chain_pack(typename Vec<T>::type *__restrict__ Sp, ChainArgs a
   }
 }
 
-template <typename T>
+// SWZ: LDS addresses through the host's swizzle table (a separate instance: the
+// plain one keeps the cheaper addressing).
+template <typename T, bool SWZ>
 __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename Vec<T>::type *__restrict__ X0,
                                                                   typename Vec<T>::type *__restrict__ Xm,
                                                                   const typename Vec<T>::type *__restrict__ Sp,
@@ -1257,12 +1281,13 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   constexpr int RB = 4;
   static_assert(kChainPer == 1 << RB && kChainThreads == 1 << (kChainTileBits - RB), "tile split");
   int64_t lin = 0, lout = 0;
-  int sout = 0;
+  uint32_t sin = 0, sout = 0;   // LDS addresses of the lane's element r = 0 (load / store order)
   for (int b = 0; b < kChainTileBits - RB; ++b)
     if ((t >> b) & 1) {
       lin += a.win[b];
       lout += a.wout[a.pout[b]];
-      sout |= 1 << a.pout[b];
+      sin = tile_cmb<SWZ>(sin, tile_img<SWZ>(a.swz, b));
+      sout = tile_cmb<SWZ>(sout, tile_img<SWZ>(a.swz, a.pout[b]));
     }
   // offsets of element r of this lane: the top RB tile bits come from r (a
   // compile-time constant, so these fold to sums of kernel arguments)
@@ -1280,11 +1305,15 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
       if ((r >> i) & 1) o += a.wout[a.pout[kChainTileBits - RB + i]];
     return o;
   };
+  // the r bits are never swizzled (the host keeps their images plain powers of two):
+  // in load order they are the top tile bits, above every bit of sin, so their offset
+  // is a constant that folds into the LDS instructions
+  auto rsi = [&](auto r) { return sin + (uint32_t)(kChainThreads * r); };
   auto rso = [&](auto r) {
-    int o = sout;
+    uint32_t o = sout;
 #pragma unroll
     for (int i = 0; i < RB; ++i)
-      if ((r >> i) & 1) o |= 1 << a.pout[kChainTileBits - RB + i];
+      if ((r >> i) & 1) o = tile_cmb<SWZ>(o, 1u << a.pout[kChainTileBits - RB + i]);
     return o;
   };
   // prefetch registers as native two-element vectors (a struct copy would keep them
@@ -1312,7 +1341,7 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   if (w < ntiles) {
     const int64_t bi = in_base(w);
     static_for<kChainPer>([&](auto r) { pf[r] = X0e[bi + rin(r)]; });
-    static_for<kChainPer>([&](auto r) { tile_e[t + kChainThreads * r] = pf[r]; });
+    static_for<kChainPer>([&](auto r) { tile_e[rsi(r)] = pf[r]; });
     lds_barrier();
   }
   for (; w < ntiles; w += gridDim.x) {
@@ -1325,23 +1354,23 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
           if (q * n / 4 == j) load_quarter(q, bn);
       const ChainStepArgs &s = a.st[j];
       switch (s.k) {
-        case 0: chain_apply<T, 1>(tile, Sp, s); break;
-        case 1: chain_apply<T, 2>(tile, Sp, s); break;
-        case 2: chain_apply<T, 4>(tile, Sp, s); break;
-        case 3: chain_apply<T, 8>(tile, Sp, s); break;
+        case 0: chain_apply<T, 1, SWZ>(tile, Sp, s, a.swz); break;
+        case 1: chain_apply<T, 2, SWZ>(tile, Sp, s, a.swz); break;
+        case 2: chain_apply<T, 4, SWZ>(tile, Sp, s, a.swz); break;
+        case 3: chain_apply<T, 8, SWZ>(tile, Sp, s, a.swz); break;
         case 4:
           if (!a.mfma16)
-            chain_apply<T, 16>(tile, Sp, s);
+            chain_apply<T, 16, SWZ>(tile, Sp, s, a.swz);
           else if (s_lds)
-            chain_apply_mfma<T, 16, true>(tile, Sp, sS, s);
+            chain_apply_mfma<T, 16, true, SWZ>(tile, Sp, sS, s, a.swz);
           else
-            chain_apply_mfma<T, 16, false>(tile, Sp, sS, s);
+            chain_apply_mfma<T, 16, false, SWZ>(tile, Sp, sS, s, a.swz);
           break;
         default:
           if (s_lds)
-            chain_apply_mfma<T, 32, true>(tile, Sp, sS, s);
+            chain_apply_mfma<T, 32, true, SWZ>(tile, Sp, sS, s, a.swz);
           else
-            chain_apply_mfma<T, 32, false>(tile, Sp, sS, s);
+            chain_apply_mfma<T, 32, false, SWZ>(tile, Sp, sS, s, a.swz);
           break;
       }
     }
@@ -1352,7 +1381,7 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
     static_for<kChainPer>([&](auto r) { Xme[bo + rout(r)] = tile_e[rso(r)]; });
     lds_barrier();
     if (more) {
-      static_for<kChainPer>([&](auto r) { tile_e[t + kChainThreads * r] = pf[r]; });
+      static_for<kChainPer>([&](auto r) { tile_e[rsi(r)] = pf[r]; });
       lds_barrier();
     }
   }
@@ -1610,14 +1639,23 @@ void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const C
   const int64_t ntiles = int64_t(1) << args.nouter;
   const int g = (int)(ntiles < args.grid ? ntiles : args.grid);
   hipStream_t s = (hipStream_t)stream;
+  const dim3 pack_grid(kChainMaxS / 256, args.nsteps);
   if (prec == QX_C128) {
-    hipLaunchKernelGGL(k_chain_pack<double>, dim3(8), dim3(256), 0, s, (double2 *)ws, args);
-    hipLaunchKernelGGL(k_contract_chain<double>, dim3(g), dim3(kChainThreads), 0, s, (const double2 *)X0,
-                       (double2 *)Xm, (const double2 *)ws, args);
+    hipLaunchKernelGGL(k_chain_pack<double>, pack_grid, dim3(256), 0, s, (double2 *)ws, args);
+    if (args.swizzled)
+      hipLaunchKernelGGL((k_contract_chain<double, true>), dim3(g), dim3(kChainThreads), 0, s, (const double2 *)X0,
+                         (double2 *)Xm, (const double2 *)ws, args);
+    else
+      hipLaunchKernelGGL((k_contract_chain<double, false>), dim3(g), dim3(kChainThreads), 0, s, (const double2 *)X0,
+                         (double2 *)Xm, (const double2 *)ws, args);
   } else {
-    hipLaunchKernelGGL(k_chain_pack<float>, dim3(8), dim3(256), 0, s, (float2 *)ws, args);
-    hipLaunchKernelGGL(k_contract_chain<float>, dim3(g), dim3(kChainThreads), 0, s, (const float2 *)X0,
-                       (float2 *)Xm, (const float2 *)ws, args);
+    hipLaunchKernelGGL(k_chain_pack<float>, pack_grid, dim3(256), 0, s, (float2 *)ws, args);
+    if (args.swizzled)
+      hipLaunchKernelGGL((k_contract_chain<float, true>), dim3(g), dim3(kChainThreads), 0, s, (const float2 *)X0,
+                         (float2 *)Xm, (const float2 *)ws, args);
+    else
+      hipLaunchKernelGGL((k_contract_chain<float, false>), dim3(g), dim3(kChainThreads), 0, s, (const float2 *)X0,
+                         (float2 *)Xm, (const float2 *)ws, args);
   }
   check_launch("contract_chain");
 }

@@ -79,6 +79,22 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
       if (ex.first + mi >= ex.second || ec.first + mo >= ec.second || mo + 1 != st.c_elems ||
           (int64_t(1) << (kChainTileBits + a.nouter)) != st.c_elems)
         throw Error(QX_ERR_STATE, "internal: chain '" + p.nodes[st.node].name + "' addresses outside its operands");
+      // LDS addresses are XORs of swz entries: each below the tile size (in bounds) and
+      // together a basis over GF(2) (every element has its own address); pout a permutation
+      uint32_t basis[kChainTileBits] = {0}, seen = 0;
+      for (int b = 0; b < kChainTileBits; ++b) {
+        if (a.swz[b] <= 0 || a.swz[b] >= (1 << kChainTileBits) || a.pout[b] < 0 || a.pout[b] >= kChainTileBits)
+          throw Error(QX_ERR_STATE, "internal: chain tile layout");
+        seen |= 1u << a.pout[b];
+        uint32_t x = (uint32_t)a.swz[b];
+        for (int k = kChainTileBits - 1; k >= 0 && x; --k)
+          if ((x >> k) & 1) {
+            if (basis[k]) x ^= basis[k];
+            else basis[k] = x, x = 0;
+          }
+        if (x) throw Error(QX_ERR_STATE, "internal: chain tile layout is not a bijection");
+      }
+      if (seen != (1u << kChainTileBits) - 1) throw Error(QX_ERR_STATE, "internal: chain store order");
       for (int j = 0; j < a.nsteps; ++j) {
         const ChainStepArgs &c = a.st[j];
         int64_t ms = 0;
@@ -406,14 +422,19 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
   if (st.kern == Kern::Chain) {
     bool k16 = false;
     for (int j = 0; j < st.chain.nsteps; ++j) k16 |= st.chain.st[j].k == 4;
+    bool swz = false;   // a non-trivial swizzle exists
+    for (int b = 0; b < kChainTileBits; ++b) swz |= st.chain.swz[b] != (1 << b);
     for (int grid : {256, 512, 1024})
-      for (int m : {0, 1}) {
-        if ((m && !k16) || (grid == st.chain.grid && m == st.chain.mfma16)) continue;
-        Step v = st;
-        v.chain.grid = grid;
-        v.chain.mfma16 = m;
-        out.push_back(std::move(v));
-      }
+      for (int m : {0, 1})
+        for (int z : {0, 1}) {
+          if ((m && !k16) || (z && !swz)) continue;
+          if (grid == st.chain.grid && m == st.chain.mfma16 && z == st.chain.swizzled) continue;
+          Step v = st;
+          v.chain.grid = grid;
+          v.chain.mfma16 = m;
+          v.chain.swizzled = z;
+          out.push_back(std::move(v));
+        }
     return out;
   }
   if (st.kern == Kern::Radix) return out;
@@ -674,6 +695,7 @@ Step chain_step(const Program &p, const Lowered &L, const ChainBuild &cb) {
     pos_of[tile[i]] = i;
     a.win[i] = cb.x0_stride[tile[i]];
     a.wout[i] = xm_stride[tile[i]];
+    a.swz[i] = 1 << i;
   }
   std::vector<int> order(kChainTileBits);
   for (int i = 0; i < kChainTileBits; ++i) order[i] = i;
@@ -793,6 +815,93 @@ std::vector<int> form_chains(const Program &p, Lowered &L) {
 }
 
 }  // namespace
+
+// LDS bank groups: one access cycle serves kLdsCycleBytes, i.e. 2^nb lanes of
+// elem_bytes each, and those lanes collide unless their addresses differ in the low nb
+// element bits.  Element i sits at P(i) = i ^ sum_{b >= nb} G[b] i_b (G[b] < 2^nb): the
+// bank group of i is the linear map L(i) = i_low ^ sum G[b] i_b over GF(2), and a
+// cycle's lanes, which differ in nb tile bits, are conflict-free when L maps those bits
+// to independent vectors.  A cycle's lanes differ in: the low tile bits (loads), the
+// fastest X_m tile bits (stores), each step's lowest free tile bits (its groups) and,
+// for steps with 16 or 32 contracted values, their first contracted bits (matrix-core
+// result columns).  G is 0 on the bits of a lane's element index r (tile bits 9-12 for
+// loads, the last four X_m tile bits for stores) so those offsets stay additive; it
+// minimises the summed conflict degree (weights: 2 per step, 1 per load / store pass)
+// by hill climbing from fixed seeds.
+void chain_swizzle(ChainArgs &a, int elem_bytes) {
+  int nb = 0;
+  while ((elem_bytes << (nb + 1)) <= kLdsCycleBytes) ++nb;
+  std::vector<std::pair<std::vector<int>, int>> sets;
+  {
+    std::vector<int> v, w;
+    for (int i = 0; i < nb; ++i) v.push_back(i), w.push_back(a.pout[i]);
+    sets.push_back({v, 1});
+    sets.push_back({w, 1});
+  }
+  for (int j = 0; j < a.nsteps; ++j) {
+    const ChainStepArgs &c = a.st[j];
+    uint32_t used = 0;
+    for (int i = 0; i < c.k; ++i) used |= 1u << c.q[i];
+    for (int i = 0; i < c.h; ++i) used |= 1u << c.hq[i];
+    std::vector<int> v;
+    for (int b = 0; b < kChainTileBits && (int)v.size() < nb; ++b)
+      if (!((used >> b) & 1)) v.push_back(b);
+    sets.push_back({v, 2});
+    if (c.k >= 4) sets.push_back({std::vector<int>(c.q, c.q + std::min(nb, c.k)), 2});
+  }
+  std::vector<char> fixed(kChainTileBits, 0);   // G must stay 0
+  for (int b = 0; b < kChainTileBits; ++b) fixed[b] = b < nb || b >= kChainTileBits - 4;
+  for (int i = kChainTileBits - 4; i < kChainTileBits; ++i) fixed[a.pout[i]] = 1;
+  std::vector<uint32_t> G(kChainTileBits, 0);
+  auto image = [&](int b) { return b < nb ? 1u << b : G[b]; };
+  auto cost = [&]() {
+    int total = 0;
+    for (auto &st : sets) {
+      uint32_t basis[8] = {0};   // GF(2) elimination, basis[k] has leading bit k
+      int rank = 0;
+      for (int b : st.first) {
+        uint32_t x = image(b);
+        for (int k = nb - 1; k >= 0 && x; --k)
+          if ((x >> k) & 1) {
+            if (basis[k]) x ^= basis[k];
+            else basis[k] = x, x = 0, ++rank;
+          }
+      }
+      total += st.second << ((int)st.first.size() - rank);
+    }
+    return total;
+  };
+  std::vector<uint32_t> best = G;
+  int best_cost = cost();
+  uint64_t rng = 0x9E3779B97F4A7C15ull;
+  for (int restart = 0; restart < 32; ++restart) {
+    for (int b = 0; b < kChainTileBits; ++b) {
+      rng ^= rng << 13, rng ^= rng >> 7, rng ^= rng << 17;
+      G[b] = (restart == 0 || fixed[b]) ? 0u : (uint32_t)(rng % (1u << nb));
+    }
+    int cur = cost();
+    for (bool improved = true; improved;) {
+      improved = false;
+      for (int b = 0; b < kChainTileBits; ++b) {
+        if (fixed[b]) continue;
+        uint32_t pick = G[b];
+        int pick_cost = cur;
+        for (uint32_t v = 0; v < (1u << nb); ++v) {
+          G[b] = v;
+          const int c = cost();
+          if (c < pick_cost) pick_cost = c, pick = v;
+        }
+        G[b] = pick;
+        if (pick_cost < cur) cur = pick_cost, improved = true;
+      }
+    }
+    if (cur < best_cost) best_cost = cur, best = G;
+  }
+  for (int b = 0; b < kChainTileBits; ++b) a.swz[b] = (int32_t)((1u << b) | (fixed[b] ? 0u : best[b]));
+  G.assign(kChainTileBits, 0);
+  a.swizzled = best_cost < cost() ? 1 : 0;   // by default only where it removes conflicts
+}
+
 
 namespace {
 
@@ -1136,7 +1245,10 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
   std::vector<char> absorbed(nn, 0);
   if (chains && !keep_all) {
     for (int a : form_chains(p, L)) absorbed[a] = 1;
-    for (const Step &st : L.steps) L.ws_elems = std::max(L.ws_elems, st.ws_elems);
+    for (Step &st : L.steps) {
+      if (st.kern == Kern::Chain) chain_swizzle(st.chain, precision == QX_C128 ? 16 : 8);
+      L.ws_elems = std::max(L.ws_elems, st.ws_elems);
+    }
   }
   if (chains || keep_all) level_small_steps(p, L);
 

@@ -156,6 +156,10 @@ struct ChainArgs {
   int s_elems = 0;         // elements of all packed S_j (staged in LDS when they fit)
   int64_t win[kChainTileBits] = {0}, wout[kChainTileBits] = {0};   // tile bit -> X_0 / X_m stride
   int32_t pout[kChainTileBits] = {0};  // store order: i-th fastest X_m tile bit -> tile bit
+  // LDS address of tile bit b: element i sits at the XOR of swz[b] over its set bits.
+  // swz[b] = 2^b plus, for some b above the bank-group bits, a pattern in those bits.
+  int32_t swz[kChainTileBits] = {0};
+  int swizzled = 0;        // 1: LDS addresses through swz (else plain; swz unused); tuned
   int64_t oin[kMaxBits] = {0}, oout[kMaxBits] = {0};   // outer bits (workgroup index) -> strides
   ChainStepArgs st[kChainMaxSteps];
   const void *S[kChainMaxSteps] = {nullptr};   // small operands, set at launch
@@ -287,6 +291,15 @@ int64_t first_fit(const std::vector<int64_t> &start, const std::vector<int64_t> 
 // ----------------------------------------------------------------------------
 // Kernel launch interface (kernels.hip).
 void set_kernel_args(Step &st, int precision = QX_C128);
+// Bytes that one LDS access cycle serves for 16-byte accesses: lanes whose addresses
+// differ only above this many bytes collide on the same banks (measured on gfx950 with
+// SQ_LDS_BANK_CONFLICT: a swizzle built for 256 bytes leaves no conflicts where one
+// for 128 bytes leaves some).
+constexpr int kLdsCycleBytes = 256;
+
+// Choose the chain tile's LDS swizzle (ChainArgs::swz) for elements of `elem_bytes`.
+void chain_swizzle(ChainArgs &a, int elem_bytes);
+
 // Run `st` with kernel configuration `choice` (0 gather, 1 rows, 2-4 FP tile GEMMs,
 // 5 / 7 matrix-core GEMM, 6 reduce, 8 vendor BLAS on plain matrix layouts); false
 // (and `st` unchanged) if that kernel cannot.

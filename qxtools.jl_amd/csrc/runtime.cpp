@@ -387,7 +387,7 @@ std::string kern_variant(const qx::Step &st);
 std::string cand_label(const qx::Step &st) {
   std::string v = kern_variant(st);
   if (st.kern == qx::Kern::Chain)
-    v += "/g" + std::to_string(st.chain.grid) + (st.chain.mfma16 ? "/mfma" : "/fma");
+    v += "/g" + std::to_string(st.chain.grid) + (st.chain.mfma16 ? "/mfma" : "/fma") + (st.chain.swizzled ? "/swz" : "");
   return v;
 }
 
@@ -853,6 +853,26 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
       for (size_t j = 0; j < st.kl.size(); ++j)
         os << (j ? "," : "") << "[" << st.kl[j].dim << "," << st.kl[j].sa << "," << st.kl[j].sb << "]";
       os << "]";
+      if (st.kern == qx::Kern::Chain) {   // tile layout: store order, LDS swizzle, each step's tile bits
+        const qx::ChainArgs &a = st.chain;
+        os << ",\"chain_tile\":{\"swizzled\":" << a.swizzled << ",\"pout\":[";
+        for (int i = 0; i < qx::kChainTileBits; ++i) os << (i ? "," : "") << a.pout[i];
+        os << "],\"swz\":[";
+        for (int i = 0; i < qx::kChainTileBits; ++i) os << (i ? "," : "") << a.swz[i];
+        os << "],\"q\":[";
+        for (int j = 0; j < a.nsteps; ++j) {
+          os << (j ? "," : "") << "[";
+          for (int i = 0; i < a.st[j].k; ++i) os << (i ? "," : "") << a.st[j].q[i];
+          os << "]";
+        }
+        os << "],\"hq\":[";
+        for (int j = 0; j < a.nsteps; ++j) {
+          os << (j ? "," : "") << "[";
+          for (int i = 0; i < a.st[j].h; ++i) os << (i ? "," : "") << a.st[j].hq[i];
+          os << "]";
+        }
+        os << "]}";
+      }
     }
     os << ",\"candidates\":[";
     auto cands = st.level >= 0 ? std::vector<qx::Step>{st} : qx::kernel_candidates(st, es == 16 ? QX_C128 : QX_C64_4M);

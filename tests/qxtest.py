@@ -97,18 +97,21 @@ def gate_sequence_program(n: int, gates: list, seed: int = 0):
     return "\n".join(lines) + "\n", data
 
 
-def interleaved_gemm_program(nk=11, nm=8, nn=8, seed=0, view_a=False):
+def interleaved_gemm_program(nk=11, nm=8, nn=8, seed=0, view_a=False, a_data=False):
     """Two gate contractions that leave their results with the legs of one large
     contraction interleaved, then that contraction C = A.B over nk contracted legs:
     A carries contracted and A-only legs alternately, B contracted and B-only legs
     alternately (all dims 2).  view_a slices contracted label 1 through views of A and
-    B (their layouts are then pinned).
+    B (their layouts are then pinned); a_data makes A a data tensor holding its M legs
+    then its K legs in a shuffled order (B must follow that order).
     Returns (dsl, data, reference C flattened column-major)."""
     rng = np.random.default_rng(seed)
     K = list(range(1, nk + 1))
     M = list(range(100, 100 + nm))
     N = list(range(200, 200 + nn))
     xa = [l for pair in zip(K, M) for l in pair] + K[len(M):] + M[len(K):]
+    if a_data:   # A is the data tensor itself: its M run, then its K run in a shuffled order
+        xa = M + list(rng.permutation(K))
     xb = [l for pair in zip(N, K) for l in pair] + N[len(K):] + K[len(N):]
     # A = X . G on A's first contracted leg (label 1 takes the gate's input label 900)
     xa_in = [900 if l == 1 else l for l in xa]
@@ -124,13 +127,19 @@ def interleaved_gemm_program(nk=11, nm=8, nn=8, seed=0, view_a=False):
              "load G1 data_g1 2,2", "load G2 data_g2 2,2",
              f"ncon A {fmt(xa)} X {fmt(xa_in)} G1 900,1",
              f"ncon B {fmt(xb)} Y {fmt(xb_in)} G2 901,2"]
+    if a_data:
+        xa = [int(l) for l in xa]
+        X = rng.normal(size=[2] * len(xa)) + 1j * rng.normal(size=[2] * len(xa))
+        lines = [lines[0], f"load A data_x {fmt([2] * len(xa))}", lines[2], lines[4], lines[6]]
     if view_a:   # slice contracted label 1 in both operands
         lines.append(f"view Av A v1 {xa.index(1) + 1} 2")
         lines.append(f"view Bv B v1 {xb.index(1) + 1} 2")
     lines += [f"ncon C {fmt(cl)} {'Av' if view_a else 'A'} {fmt(xa)} {'Bv' if view_a else 'B'} {fmt(xb)}",
               "save output C"]
     data = {"data_x": X, "data_y": Y, "data_g1": G1, "data_g2": G2}
-    A = O.ncon(X, xa_in, G1, [900, 1], xa)
+    if a_data:
+        del data["data_g1"]
+    A = X if a_data else O.ncon(X, xa_in, G1, [900, 1], xa)
     B = O.ncon(Y, xb_in, G2, [901, 2], xb)
     ref = O.ncon(A, xa, B, xb, cl).reshape(-1, order="F")
     return "\n".join(lines) + "\n", data, ref

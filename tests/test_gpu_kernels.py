@@ -216,12 +216,14 @@ def test_rocblas_step_in_graph_replays():
     ex.close()
 
 
-@pytest.mark.parametrize("view,prec", [(False, "c128"), (True, "c128"), (False, "c64_4m")])
-def test_regrouped_contraction_matches_oracle(view, prec):
+@pytest.mark.parametrize("view,prec,a_data", [(False, "c128", False), (True, "c128", False), (False, "c64_4m", False),
+                                              (False, "c128", True)])
+def test_regrouped_contraction_matches_oracle(view, prec, a_data):
     """A large contraction whose operands the lowering regroups into plain matrices
-    (complex128: run by rocBLAS); with sliced operands the layouts stay as written."""
+    (complex128: run by rocBLAS), also around a data operand's fixed layout; with
+    sliced operands the layouts stay as written."""
     import qxtest
-    dsl, data, ref = qxtest.interleaved_gemm_program(view_a=view)
+    dsl, data, ref = qxtest.interleaved_gemm_program(view_a=view, a_data=a_data)
     ex = Executor(dsl, data, precision=prec)
     out = np.asarray(ex.amplitudes([""])).reshape(-1)
     st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][-1]
