@@ -1201,19 +1201,23 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
   const uint32_t arow = scatter4(li);
   v4 cre[RT], cim[RT];
 #pragma unroll
-  for (int rt = 0; rt < RT; ++rt) {
-    const uint32_t base = tile_cmb<SWZ>(rbase[rt], arow);
-    v4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+  for (int rt = 0; rt < RT; ++rt) cre[rt] = v4{0, 0, 0, 0}, cim[rt] = v4{0, 0, 0, 0};
+  // k blocks outer, row blocks inner: the 2 RT accumulators are independent, so
+  // consecutive matrix-core ops never wait for each other's results
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      const V a = tile[tile_cmb<SWZ>(base, koff(4 * kb + lk))];
-      xr = M::mma(a.x, bre[kb], xr);
-      xr = M::mma(-a.y, bim[kb], xr);
-      xi = M::mma(a.x, bim[kb], xi);
-      xi = M::mma(a.y, bre[kb], xi);
-    }
-    cre[rt] = xr;
-    cim[rt] = xi;
+  for (int kb = 0; kb < KB; ++kb) {
+    V a[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+      a[rt] = tile[tile_cmb<SWZ>(tile_cmb<SWZ>(rbase[rt], arow), koff(4 * kb + lk))];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) cre[rt] = M::mma(a[rt].x, bre[kb], cre[rt]);
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) cim[rt] = M::mma(a[rt].x, bim[kb], cim[rt]);
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) cre[rt] = M::mma(-a[rt].y, bim[kb], cre[rt]);
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) cim[rt] = M::mma(a[rt].y, bre[kb], cim[rt]);
   }
   // waves of other nu blocks read these groups: all reads before any write (with
   // one nu block every wave owns its groups alone)
