@@ -827,6 +827,13 @@ __global__ void k_zero(uint4 *p, int64_t n16) {
     p[i] = uint4{0, 0, 0, 0};
 }
 
+// Batched zero fill: workgroup i clears range i of `base` (offset and length in
+// 16-byte units, pairs in `ranges`).
+__global__ void k_zero_batch(uint4 *base, const int64_t *__restrict__ ranges) {
+  const int64_t off = ranges[2 * blockIdx.x], n = ranges[2 * blockIdx.x + 1];
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) base[off + i] = uint4{0, 0, 0, 0};
+}
+
 int grid_for(int64_t work, int per_block) {
   int64_t g = (work + per_block - 1) / per_block;
   if (g < 1) g = 1;
@@ -1393,15 +1400,45 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
 
 // ---------------------------------------------------------------------------
 // Small-step batch: workgroup i contracts the step of descriptor i (lanes along C,
-// the K loop in registers).  The steps of one launch are independent.
+// the K loop in registers; a C smaller than the workgroup splits each element's K
+// range over 256 / |C| lanes, summed in LDS).  The steps of one launch are
+// independent.
+constexpr int kSmallThreads = 256;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_small_batch(const SmallDesc *__restrict__ d) {
+__global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *__restrict__ d) {
   using V = typename Vec<T>::type;
   const SmallDesc &s = d[blockIdx.x];
   const V *A = (const V *)s.A;
   const V *B = (const V *)s.B;
   V *C = (V *)s.C;
   const int nc = 1 << s.nbc, nk = 1 << s.nbk;
+  if (nc < kSmallThreads && nk > 1) {
+    __shared__ V part[kSmallThreads];
+    const int lt = 8 - s.nbc;   // log2 of the lanes per element (kSmallThreads = 2^8)
+    const int c = threadIdx.x >> lt, sub = threadIdx.x & ((1 << lt) - 1);
+    int64_t oa = 0, ob = 0;
+    for (int b = 0; b < s.nbc; ++b)
+      if ((c >> b) & 1) oa += s.wa[b], ob += s.wb[b];
+    V acc{0, 0};
+    for (int k = sub; k < nk; k += 1 << lt) {
+      int64_t xa = oa, xb = ob;
+      for (int b = 0; b < s.nbk; ++b)
+        if ((k >> b) & 1) xa += s.ka[b], xb += s.kb[b];
+      cmac<T>(acc, A[xa], B[xb], s.conj_a, s.conj_b);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (sub == 0) {
+      V sum = part[threadIdx.x];
+      for (int j = 1; j < (1 << lt); ++j) {
+        sum.x += part[threadIdx.x + j].x;
+        sum.y += part[threadIdx.x + j].y;
+      }
+      C[c] = sum;
+    }
+    return;
+  }
   for (int c = threadIdx.x; c < nc; c += blockDim.x) {
     int64_t oa = 0, ob = 0;
     for (int b = 0; b < s.nbc; ++b)
@@ -1425,7 +1462,6 @@ __global__ __launch_bounds__(256) void k_add_batch(const AddDesc *__restrict__ d
   using VS = typename Vec<TS>::type;
   const AddDesc &a = d[blockIdx.x];
   const VS *src = (const VS *)a.src;
-  double2 *dst = (double2 *)a.dst;
   for (int64_t e = threadIdx.x; e < a.n; e += blockDim.x) {
     int64_t r = e, o = 0;
     for (int j = 0; j < a.nd; ++j) {
@@ -1434,8 +1470,15 @@ __global__ __launch_bounds__(256) void k_add_batch(const AddDesc *__restrict__ d
       o += q * a.st[j];
     }
     const VS v = src[e];
-    dst[o].x += (double)v.x;
-    dst[o].y += (double)v.y;
+    if (a.dst_f64) {
+      double2 *dst = (double2 *)a.dst;
+      dst[o].x += (double)v.x;
+      dst[o].y += (double)v.y;
+    } else {
+      VS *dst = (VS *)a.dst;
+      dst[o].x += v.x;
+      dst[o].y += v.y;
+    }
   }
 }
 
@@ -1617,6 +1660,12 @@ void launch_finalize(const double *acc, double *out, int64_t n, void *stream) {
   check_launch("finalize");
 }
 
+void launch_zero_batch(void *base, const int64_t *d_ranges, int n, void *stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_batch, dim3(n), dim3(256), 0, (hipStream_t)stream, (uint4 *)base, d_ranges);
+  check_launch("zero_batch");
+}
+
 void launch_zero(void *p, int64_t bytes, void *stream) {
   if (bytes <= 0) return;
   if (bytes % 16) throw Error(QX_ERR_ARGUMENT, "zero fill of a size not a multiple of 16 bytes");
@@ -1668,9 +1717,9 @@ void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream) 
   if (n <= 0) return;
   hipStream_t s = (hipStream_t)stream;
   if (prec == QX_C128)
-    hipLaunchKernelGGL(k_small_batch<double>, dim3(n), dim3(256), 0, s, d_desc);
+    hipLaunchKernelGGL(k_small_batch<double>, dim3(n), dim3(kSmallThreads), 0, s, d_desc);
   else
-    hipLaunchKernelGGL(k_small_batch<float>, dim3(n), dim3(256), 0, s, d_desc);
+    hipLaunchKernelGGL(k_small_batch<float>, dim3(n), dim3(kSmallThreads), 0, s, d_desc);
   check_launch("small_batch");
 }
 

@@ -905,12 +905,17 @@ void chain_swizzle(ChainArgs &a, int elem_bytes) {
 
 namespace {
 
-// Whether a step may run inside a small-step batch: slice-free, power-of-two legs
-// with at most kSmallMaxBits C and K bits each, a few thousand MACs, and operand
-// offsets that fit 32 bits.
-bool small_step(const Step &st) {
+// Whether a step may run inside a small-step batch (one workgroup per step):
+// slice-free, power-of-two legs with at most kSmallMaxBits C and K bits each, at most
+// 2^18 MACs, and operand offsets that fit 32 bits.  C and K are each at most 4096,
+// except with `wide` (the pullback, whose gradient steps sum long bitstring-batch
+// ranges into a few elements): then up to 2^16 MACs with any split.  (The forward
+// keeps the limit: its larger steps run faster as multi-workgroup launches.)
+bool small_step(const Step &st, bool wide = false) {
   if (st.section == 1 || st.kern == Kern::Chain || st.kern == Kern::Radix) return false;
-  if (st.c_elems > 4096 || st.K > 4096 || st.c_elems * st.K > (int64_t(1) << 18)) return false;
+  const int64_t macs = st.c_elems * st.K;
+  if (macs > (int64_t(1) << 18)) return false;
+  if ((st.c_elems > 4096 || st.K > 4096) && (!wide || macs > (int64_t(1) << 16))) return false;
   int nbc = 0, nbk = 0;
   int64_t wmax = 0;
   for (auto &g : st.cl) {
@@ -1512,11 +1517,16 @@ VjpLowered lower_vjp(const Program &p, int64_t batch, int precision, uint64_t fu
     std::vector<BwdStep> sorted;
     std::map<int, int64_t> level_temp;
     std::map<int, int> level_count;
+    std::set<std::pair<int, int>> level_cot;   // (level, cotangent buffer) added into by the level
     for (size_t i : order) {
       BwdStep b = std::move(bv[i]);
-      const bool ok = small_step(b.step) && no_slice_terms(b) &&
-                      (b.direct || (b.dest == Buf::Grad && key_uses[b.dest_key] == 1 &&
-                                    b.add_dims.size() <= (size_t)kSmallMaxBits));
+      // adds of one level go out in one launch: each destination at most once (a key's
+      // gradient has one adding step; a cotangent buffer one per level)
+      bool ok = small_step(b.step, true) && no_slice_terms(b) &&
+                (b.direct || (b.add_dims.size() <= (size_t)kSmallMaxBits &&
+                              ((b.dest == Buf::Grad && key_uses[b.dest_key] == 1) ||
+                               (b.dest == Buf::Cot && !level_cot.count({level_base + lev[i], b.dest_root})))));
+      if (ok && !b.direct && b.dest == Buf::Cot) level_cot.insert({level_base + lev[i], b.dest_root});
       if (ok) {
         b.level = level_base + lev[i];
         level_count[b.level]++;

@@ -110,7 +110,7 @@ struct RadixArgs {
 
 // Small slice-free steps (gate-block building) run many per launch: one workgroup
 // per step, its bit weights read from a device table of these descriptors.
-constexpr int kSmallMaxBits = 12;
+constexpr int kSmallMaxBits = 18;
 struct SmallDesc {
   const void *A = nullptr, *B = nullptr;
   void *C = nullptr;
@@ -119,13 +119,15 @@ struct SmallDesc {
   int32_t ka[kSmallMaxBits] = {0}, kb[kSmallMaxBits] = {0};   // K bits
 };
 
-// Batched scatter-add of small temps into FP64 gradients: dst[map(e)] += src[e],
-// map(e) = sum of digit_j(e) * st[j] over the dims (first fastest).
+// Batched scatter-add of small temps into FP64 gradients or cotangent buffers:
+// dst[map(e)] += src[e], map(e) = sum of digit_j(e) * st[j] over the dims (first
+// fastest).
 struct AddDesc {
   void *dst = nullptr;
   const void *src = nullptr;
   int64_t n = 0;
-  int32_t nd = 0, pad = 0;
+  int32_t nd = 0;
+  int32_t dst_f64 = 1;   // destination FP64 (a gradient); else the plan's element type
   int64_t dim[kSmallMaxBits] = {0}, st[kSmallMaxBits] = {0};
 };
 
@@ -335,6 +337,8 @@ void launch_root_cot(int prec, void *dst, int64_t dep, int64_t batch, int64_t nv
                      const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream);
 void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream);
 void launch_add_batch(int prec, const AddDesc *d_desc, int n, void *stream);
+// d_ranges: n (offset, length) pairs in 16-byte units from base; one workgroup each.
+void launch_zero_batch(void *base, const int64_t *d_ranges, int n, void *stream);
 void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args, void *stream);
 // Output sampling: candidate bitstrings from Philox4x64-10 and the sequential
 // empirical-supremum rejection test over one round of candidates.

@@ -117,7 +117,8 @@ struct qx_plan {
   std::map<const qx::Lowered *, SmallTable> small;   // per lowering (forward or pullback)
   // batched slice-free backward levels of one pullback lowering
   struct BwdTable {
-    DevBuf desc, adds;
+    DevBuf desc, adds, zeros;
+    int zero0[3] = {0, 0, 0}, nzero[3] = {0, 0, 0};   // per section: its zero-fill ranges
     std::vector<char *> bases;              // arena, cot, temp, grad the tables point into
     std::vector<int> first[3];              // per section and backward step: its level's group, or -1
     struct Group { int desc0, ndesc, add0, nadd; };
@@ -973,13 +974,6 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
   bs.temp = (char *)P->temp.p;
   const size_t nv = P->prog.vars.size();
   std::vector<int64_t> varval(nv, 0);
-  auto zero_cots = [&](int sec) {
-    for (int node : V.zero_at[sec]) {
-      const qx::Value &c = V.cot[node];
-      int64_t elems = (c.elems + 31) / 32 * 32;
-      qx::launch_zero(bs.cot + c.slot * es, elems * es, stream);
-    }
-  };
   prepare_small(P, L, bs);
   auto run_fwd = [&](int sec) {
     for (size_t s = 0; s < L.steps.size(); ++s) {
@@ -1042,7 +1036,13 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
           if (!b.direct) {
             qx::AddDesc a;
             a.src = C;
-            a.dst = (char *)P->grad.p + (key_off[b.dest_key] + value_offset(L.values[b.x], varval)) * 16;
+            const int64_t xoff = value_offset(L.values[b.x], varval);
+            if (b.dest == qx::Buf::Grad) {
+              a.dst = (char *)P->grad.p + (key_off[b.dest_key] + xoff) * 16;
+            } else {
+              a.dst = bs.cot + (V.cot[b.dest_root].slot + xoff) * es;
+              a.dst_f64 = P->prec == QX_C128;
+            }
             a.n = b.add_elems;
             a.nd = (int32_t)b.add_dims.size();
             for (int j = 0; j < a.nd; ++j) a.dim[j] = b.add_dims[j], a.st[j] = b.add_strides[j];
@@ -1059,10 +1059,29 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
         BT.adds.reserve(ad.size() * sizeof(qx::AddDesc));
         hip_check(hipMemcpy(BT.adds.p, ad.data(), ad.size() * sizeof(qx::AddDesc), hipMemcpyHostToDevice), "copy");
       }
+      // zero fills of the accumulating cotangents: one launch per section, ranges of
+      // at most 64 KiB per workgroup
+      std::vector<int64_t> zr;
+      for (int sec = 0; sec < 3; ++sec) {
+        BT.zero0[sec] = (int)(zr.size() / 2);
+        for (int node : V.zero_at[sec]) {
+          const qx::Value &c = V.cot[node];
+          const int64_t n16 = (c.elems + 31) / 32 * 32 * es / 16, o16 = c.slot * es / 16;
+          for (int64_t o = 0; o < n16; o += 4096) zr.push_back(o16 + o), zr.push_back(std::min<int64_t>(4096, n16 - o));
+        }
+        BT.nzero[sec] = (int)(zr.size() / 2) - BT.zero0[sec];
+      }
+      if (!zr.empty()) {
+        BT.zeros.reserve(zr.size() * 8);
+        hip_check(hipMemcpy(BT.zeros.p, zr.data(), zr.size() * 8, hipMemcpyHostToDevice), "copy");
+      }
       BT.bases = bases;
       P->drop_graphs();
     }
   }
+  auto zero_cots = [&](int sec) {
+    qx::launch_zero_batch(bs.cot, (const int64_t *)BT.zeros.p + 2 * BT.zero0[sec], BT.nzero[sec], stream);
+  };
   auto run_bwd = [&](int sec) {
     for (size_t bi = 0; bi < V.bsteps[sec].size(); ++bi) {
       const qx::BwdStep &b = V.bsteps[sec][bi];
@@ -1417,9 +1436,14 @@ qx_status qx_dsl_inspect_ex(const char *dsl, size_t len, int precision, int64_t 
     try {
       qx::VjpLowered V = qx::lower_vjp(prog, batch, precision, fused, batch_last);
       size_t nb = V.bsteps[0].size() + V.bsteps[1].size() + V.bsteps[2].size();
+      std::ostringstream bs;   // backward steps: section, level, kernel, H, M, N, K, destination
+      for (int sec = 0; sec < 3; ++sec)
+        for (const qx::BwdStep &b : V.bsteps[sec])
+          bs << (bs.tellp() > 0 ? "," : "") << "[" << sec << "," << b.level << ",\"" << kern_name(b.step) << "\","
+             << b.step.H << "," << b.step.M << "," << b.step.N << "," << b.step.K << "," << (b.direct ? 0 : 1) << "]";
       v = ",\"vjp\":{\"backward_steps\":" + std::to_string(nb) + ",\"forward_elems\":" +
           std::to_string(V.fwd.arena_elems) + ",\"cot_elems\":" + std::to_string(V.cot_elems) +
-          ",\"temp_elems\":" + std::to_string(V.temp_elems) + "}";
+          ",\"temp_elems\":" + std::to_string(V.temp_elems) + ",\"steps\":[" + bs.str() + "]}";
     } catch (const qx::Error &e) {
       if (e.code != QX_ERR_UNSUPPORTED) throw;
       v = ",\"vjp_error\":\"" + std::string(e.what()) + "\"";
