@@ -1399,10 +1399,10 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
 }
 
 // ---------------------------------------------------------------------------
-// Small-step batch: workgroup i contracts the step of descriptor i (lanes along C,
-// the K loop in registers; a C smaller than the workgroup splits each element's K
-// range over 256 / |C| lanes, summed in LDS).  The steps of one launch are
-// independent.
+// Small-step batch: workgroup i contracts C elements [c0, c1) of the step of
+// descriptor i (lanes along C, the K loop in registers; a C smaller than the
+// workgroup splits each element's K range over 256 / |C| lanes, summed in LDS).  The
+// steps of one launch are independent.
 constexpr int kSmallThreads = 256;
 
 template <typename T>
@@ -1413,7 +1413,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *
   const V *B = (const V *)s.B;
   V *C = (V *)s.C;
   const int nc = 1 << s.nbc, nk = 1 << s.nbk;
-  if (nc < kSmallThreads && nk > 1) {
+  if (nc < kSmallThreads && nk > 1) {   // (one workgroup holds the whole C here)
     __shared__ V part[kSmallThreads];
     const int lt = 8 - s.nbc;   // log2 of the lanes per element (kSmallThreads = 2^8)
     const int c = threadIdx.x >> lt, sub = threadIdx.x & ((1 << lt) - 1);
@@ -1439,7 +1439,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *
     }
     return;
   }
-  for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+  for (int c = s.c0 + threadIdx.x; c < s.c1; c += blockDim.x) {
     int64_t oa = 0, ob = 0;
     for (int b = 0; b < s.nbc; ++b)
       if ((c >> b) & 1) oa += s.wa[b], ob += s.wb[b];

@@ -111,10 +111,12 @@ struct RadixArgs {
 // Small slice-free steps (gate-block building) run many per launch: one workgroup
 // per step, its bit weights read from a device table of these descriptors.
 constexpr int kSmallMaxBits = 18;
+constexpr int kSmallChunk = 1024;   // C elements per workgroup of a small-step batch
 struct SmallDesc {
   const void *A = nullptr, *B = nullptr;
   void *C = nullptr;
   int32_t nbc = 0, nbk = 0, conj_a = 0, conj_b = 0;
+  int32_t c0 = 0, c1 = 0;      // C elements [c0, c1) of this workgroup (a large C is split)
   int32_t wa[kSmallMaxBits] = {0}, wb[kSmallMaxBits] = {0};   // C bits
   int32_t ka[kSmallMaxBits] = {0}, kb[kSmallMaxBits] = {0};   // K bits
 };

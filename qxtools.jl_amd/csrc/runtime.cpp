@@ -294,6 +294,19 @@ void launch_step(qx_plan *P, const qx::Lowered &L, const qx::Step &st, void *A, 
   }
 }
 
+// Append x to a small-step descriptor table, its C split into workgroup ranges of
+// kSmallChunk elements; returns the number of descriptors added.
+int push_small(std::vector<qx::SmallDesc> &d, qx::SmallDesc x) {
+  const int nc = 1 << x.nbc;
+  int n = 0;
+  for (int c0 = 0; c0 < nc; c0 += qx::kSmallChunk, ++n) {
+    x.c0 = c0;
+    x.c1 = std::min(nc, c0 + qx::kSmallChunk);
+    d.push_back(x);
+  }
+  return n;
+}
+
 // Build and upload the descriptor table of L's batched small steps (slice-free, so
 // their operand addresses depend only on the arena and data buffers).
 void prepare_small(qx_plan *P, const qx::Lowered &L, const Bases &bs) {
@@ -308,7 +321,6 @@ void prepare_small(qx_plan *P, const qx::Lowered &L, const Bases &bs) {
     const qx::Step &st = L.steps[s];
     if (st.level < 0) continue;
     if (first < 0 || L.steps[first].level != st.level) first = (int)s;
-    T.count[first] += 1;
     T.index[s] = (int)d.size();
     qx::SmallDesc x;
     x.A = value_ptr(P, L.values[st.in_a], zero, bs);
@@ -330,7 +342,7 @@ void prepare_small(qx_plan *P, const qx::Lowered &L, const Bases &bs) {
         x.kb[x.nbk] = (int32_t)(g.sb * m);
         ++x.nbk;
       }
-    d.push_back(x);
+    T.count[first] += push_small(d, x);
   }
   if (!d.empty()) {
     T.desc.reserve(d.size() * sizeof(qx::SmallDesc));
@@ -1031,8 +1043,7 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
           for (auto &l : b.step.kl)
             for (int64_t m = 1; m < l.dim; m <<= 1)
               x.ka[x.nbk] = (int32_t)(l.sa * m), x.kb[x.nbk] = (int32_t)(l.sb * m), ++x.nbk;
-          d.push_back(x);
-          g.ndesc++;
+          g.ndesc += push_small(d, x);
           if (!b.direct) {
             qx::AddDesc a;
             a.src = C;
