@@ -46,8 +46,9 @@ typedef int32_t qx_status;
 /* Arithmetic of a plan.  QX_C128: complex128 storage and arithmetic.  The C64
  * variants store complex64 and accumulate in FP32 (the cross-slice sum in FP64);
  * large GEMM steps run on FP32 matrix cores with 4 real products (4M) or 3 (3M,
- * Gauss).  QX_C64_BF16X3 (three bf16 products of a hi/lo split) is reserved:
- * qx_plan_from_dsl returns QX_ERR_UNSUPPORTED for it. */
+ * Gauss).  QX_C64_BF16X3 is 4M except on steps whose operands are plain matrices
+ * with M, N, K >= 1024: each operand is split into bf16 hi + lo parts and every real
+ * product is hi.hi + hi.lo + lo.hi on bf16 matrix cores, accumulated in FP32. */
 enum qx_precision {
   QX_C128 = 0,
   QX_C64_4M = 1,

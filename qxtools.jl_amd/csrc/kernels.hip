@@ -834,6 +834,30 @@ __global__ void k_zero_batch(uint4 *base, const int64_t *__restrict__ ranges) {
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) base[off + i] = uint4{0, 0, 0, 0};
 }
 
+// float -> bf16 bits, rounded to nearest with ties to even (finite inputs).
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_value(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__global__ void k_bf16_split(const float2 *__restrict__ src, int64_t n, uint16_t *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 v = src[i];
+    const uint16_t rh = bf16_bits(v.x), ih = bf16_bits(v.y);
+    out[i] = rh;
+    out[n + i] = bf16_bits(v.x - bf16_value(rh));
+    out[2 * n + i] = ih;
+    out[3 * n + i] = bf16_bits(v.y - bf16_value(ih));
+  }
+}
+
+__global__ void k_planar_merge(const float *__restrict__ re, const float *__restrict__ im, int64_t n,
+                               float2 *__restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = float2{re[i], im[i]};
+}
+
 int grid_for(int64_t work, int per_block) {
   int64_t g = (work + per_block - 1) / per_block;
   if (g < 1) g = 1;
@@ -1658,6 +1682,20 @@ void launch_root_cot(int prec, void *dst, int64_t dep, int64_t batch, int64_t nv
 void launch_finalize(const double *acc, double *out, int64_t n, void *stream) {
   hipLaunchKernelGGL(k_finalize, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, acc, out, n);
   check_launch("finalize");
+}
+
+void launch_bf16_split(const void *src, int64_t n, void *out, void *stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_bf16_split, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, (const float2 *)src, n,
+                     (uint16_t *)out);
+  check_launch("bf16_split");
+}
+
+void launch_planar_merge(const float *re, const float *im, int64_t n, void *dst, void *stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_planar_merge, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, re, im, n,
+                     (float2 *)dst);
+  check_launch("planar_merge");
 }
 
 void launch_zero_batch(void *base, const int64_t *d_ranges, int n, void *stream) {

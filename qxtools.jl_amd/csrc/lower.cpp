@@ -263,6 +263,18 @@ bool configure_kernel(Step &st, int precision, int choice) {
     case kBlas: {
       BlasArgs ba;
       if (!plain_gemm(st, ba)) return false;
+      if (precision == QX_C64_BF16X3 && ba.r >= 1024 && ba.s >= 1024 && ba.k >= 1024) {
+        // operand extents (elements) in their own layouts; workspace: 4 bf16 planes per
+        // operand and 2 FP32 planes for C, each 256-byte aligned (complex64 units)
+        auto ext = [&](int op, int64_t rows, int64_t cols, int64_t ld, int64_t bs) {
+          return (op ? (cols - 1) + (rows - 1) * ld : (rows - 1) + (cols - 1) * ld) + 1 + (ba.batch - 1) * bs;
+        };
+        ba.split = 1;
+        ba.ex = ext(ba.opx, ba.r, ba.k, ba.ldx, ba.sx);
+        ba.ey = ext(ba.opy, ba.k, ba.s, ba.ldy, ba.sy);
+        ba.ec = ba.r + (ba.s - 1) * ba.ldc + (ba.batch - 1) * ba.sc;
+        st.ws_elems = ba.ex + ba.ey + ba.ec + 128;
+      }
       st.kern = Kern::Blas;
       st.blas = ba;
       return true;
@@ -371,12 +383,15 @@ bool configure_kernel(Step &st, int precision, int choice) {
 //  * Bits   : everything else with power-of-two dims
 //  * Radix  : any other dims
 //  * Blas   : complex128 steps of >= 2^30 flops whose operands are plain matrices
-//             (rocBLAS' zgemm reaches about twice the FP64 rate of the tile kernels)
+//             (rocBLAS' zgemm reaches about twice the FP64 rate of the tile kernels);
+//             with QX_C64_BF16X3, plain steps with M, N, K >= 1024 (split bf16 product)
 int default_choice(const Step &st, int precision) {
   const StepBits sb = step_bits(st);
   BlasArgs ba;
   if (precision == QX_C128 && 8.0 * st.H * st.M * st.N * st.K >= double(int64_t(1) << 30) && plain_gemm(st, ba))
     return kBlas;
+  if (precision == QX_C64_BF16X3 && plain_gemm(st, ba) && ba.r >= 1024 && ba.s >= 1024 && ba.k >= 1024)
+    return kBlas;   // the split bf16 product
   if (sb.nr >= 12 && sb.nk <= 4 && sb.nk + sb.nn >= 1 && !sb.cbits.empty() && sb.cbits[0].wa != 0 && rows_ok(sb))
     return kRows;
   if (sb.nk >= 3 && sb.nn >= 3 && sb.nm >= 5) {

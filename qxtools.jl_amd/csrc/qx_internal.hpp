@@ -200,6 +200,12 @@ struct BlasArgs {
   int opx = 0, opy = 0, swap = 0;
   int64_t r = 0, s = 0, k = 0, ldx = 0, ldy = 0, ldc = 0;
   int64_t sx = 0, sy = 0, sc = 0, batch = 1;
+  // QX_C64_BF16X3 on steps with r, s, k >= 1024: each complex64 operand is split into
+  // bf16 hi + lo parts (planar re / im, in the operand's own layout, over its extent
+  // ex / ey) and C = Xh Yh + Xh Yl + Xl Yh by bf16 GEMMs accumulating in FP32 (planar,
+  // extent ec), then interleaved into C.  0: one complex64 GEMM.
+  int split = 0;
+  int64_t ex = 0, ey = 0, ec = 0;
 };
 
 enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows, Chain, Blas };
@@ -339,6 +345,12 @@ void launch_root_cot(int prec, void *dst, int64_t dep, int64_t batch, int64_t nv
                      const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream);
 void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream);
 void launch_add_batch(int prec, const AddDesc *d_desc, int n, void *stream);
+// bf16 split of n complex64 values: out holds four planar n-element bf16 arrays
+// (re hi, re lo, im hi, im lo); hi = x rounded to bf16 (nearest, ties to even),
+// lo = (x - hi) rounded likewise.
+void launch_bf16_split(const void *src, int64_t n, void *out, void *stream);
+// dst[i] = (re[i], im[i]) as complex64, i < n.
+void launch_planar_merge(const float *re, const float *im, int64_t n, void *dst, void *stream);
 // d_ranges: n (offset, length) pairs in 16-byte units from base; one workgroup each.
 void launch_zero_batch(void *base, const int64_t *d_ranges, int n, void *stream);
 void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args, void *stream);

@@ -257,6 +257,43 @@ void launch_blas(qx_plan *P, const qx::BlasArgs &b, const void *A, const void *B
   const rocblas_operation ox = b.opx ? rocblas_operation_transpose : rocblas_operation_none;
   const rocblas_operation oy = b.opy ? rocblas_operation_transpose : rocblas_operation_none;
   rocblas_status r;
+  if (b.split) {
+    // complex64 operands split into bf16 hi / lo planes (in their own layouts), three
+    // bf16 products per real product accumulated in FP32 planes, then interleaved
+    hipStream_t stream = P->ctx->stream;
+    auto up = [](int64_t bytes) { return (bytes + 255) / 256 * 256; };
+    char *w = (char *)P->ws.p;
+    uint16_t *xs = (uint16_t *)w;
+    uint16_t *ys = (uint16_t *)(w + up(8 * b.ex));
+    float *cre = (float *)(w + up(8 * b.ex) + up(8 * b.ey));
+    float *cim = (float *)((char *)cre + up(4 * b.ec));
+    qx::launch_bf16_split(X, b.ex, xs, stream);
+    qx::launch_bf16_split(Y, b.ey, ys, stream);
+    auto xp = [&](int part) { return xs + part * b.ex; };   // 0 re hi, 1 re lo, 2 im hi, 3 im lo
+    auto yp = [&](int part) { return ys + part * b.ey; };
+    const float one = 1.0f, minus = -1.0f, zero = 0.0f;
+    auto gemm = [&](const uint16_t *x, const uint16_t *y, float *c, bool neg, bool first) {
+      rocblas_status st = rocblas_gemm_strided_batched_ex(
+          h, ox, oy, (rocblas_int)b.r, (rocblas_int)b.s, (rocblas_int)b.k, neg ? &minus : &one, x,
+          rocblas_datatype_bf16_r, (rocblas_int)b.ldx, b.sx, y, rocblas_datatype_bf16_r, (rocblas_int)b.ldy, b.sy,
+          first ? &zero : &one, c, rocblas_datatype_f32_r, (rocblas_int)b.ldc, b.sc, c, rocblas_datatype_f32_r,
+          (rocblas_int)b.ldc, b.sc, (rocblas_int)b.batch, rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
+      if (st != rocblas_status_success)
+        throw qx::Error(QX_ERR_DEVICE, std::string("rocBLAS bf16 gemm failed: ") + rocblas_status_to_string(st));
+    };
+    // one real product u . v as hi.hi + hi.lo + lo.hi (parts: hi = base, lo = base + 1)
+    auto product = [&](int u, int v, float *c, bool neg, bool first) {
+      gemm(xp(u), yp(v), c, neg, first);
+      gemm(xp(u), yp(v + 1), c, neg, false);
+      gemm(xp(u + 1), yp(v), c, neg, false);
+    };
+    product(0, 0, cre, false, true);    // re = Xr Yr - Xi Yi
+    product(2, 2, cre, true, false);
+    product(0, 2, cim, false, true);    // im = Xr Yi + Xi Yr
+    product(2, 0, cim, false, false);
+    qx::launch_planar_merge(cre, cim, b.ec, C, stream);
+    return;
+  }
   if (P->prec == QX_C128) {
     const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
     r = rocblas_zgemm_strided_batched(h, ox, oy, b.r, b.s, b.k, &one, (const rocblas_double_complex *)X, b.ldx,
@@ -822,8 +859,9 @@ std::string kern_variant(const qx::Step &st) {
   }
 }
 
-void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Lowered &L, size_t es,
+void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Lowered &L, int prec,
                   bool with_slots) {
+  const size_t es = prec == QX_C128 ? 16 : 8;
   os << "{\"batch\":" << L.batch << ",\"arena_bytes\":" << L.arena_elems * (int64_t)es
      << ",\"result_elems\":" << L.result_elems;
   if (with_slots) {
@@ -888,7 +926,7 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
       }
     }
     os << ",\"candidates\":[";
-    auto cands = st.level >= 0 ? std::vector<qx::Step>{st} : qx::kernel_candidates(st, es == 16 ? QX_C128 : QX_C64_4M);
+    auto cands = st.level >= 0 ? std::vector<qx::Step>{st} : qx::kernel_candidates(st, prec);
     for (size_t c = 0; c < cands.size(); ++c)
       os << (c ? "," : "") << "\"" << cand_label(cands[c]) << "\"";
     os << "]}";
@@ -1247,7 +1285,6 @@ qx_status qx_plan_from_dsl(qx_ctx *ctx, const char *dsl, size_t len, int precisi
     if (!ctx || !out) throw qx::Error(QX_ERR_ARGUMENT, "null context or output pointer");
     *out = nullptr;
     if (precision < QX_C128 || precision > QX_C64_BF16X3) throw qx::Error(QX_ERR_ARGUMENT, "unknown precision");
-    if (precision == QX_C64_BF16X3) throw qx::Error(QX_ERR_UNSUPPORTED, "precision QX_C64_BF16X3 is not implemented");
     auto P = std::make_unique<qx_plan>();
     P->ctx = ctx;
     P->prec = precision;
@@ -1435,12 +1472,11 @@ qx_status qx_dsl_inspect_ex(const char *dsl, size_t len, int precision, int64_t 
     const bool batch_last = flags & 1;
     if (!json || cap == 0) throw qx::Error(QX_ERR_ARGUMENT, "null output buffer");
     if (precision < QX_C128 || precision > QX_C64_BF16X3) throw qx::Error(QX_ERR_ARGUMENT, "unknown precision");
-    if (precision == QX_C64_BF16X3) throw qx::Error(QX_ERR_UNSUPPORTED, "precision QX_C64_BF16X3 is not implemented");
     qx::Program prog = qx::parse_program(dsl, len);
     qx::Lowered L = qx::lower(prog, batch, false, precision, fused, true, batch_last);
     std::ostringstream os;
     os.precision(17);
-    lowered_json(os, prog, L, precision == QX_C128 ? 16 : 8, true);
+    lowered_json(os, prog, L, precision, true);
     std::string s = os.str();
     // reverse-mode lowering (its own host-side bounds proof runs here too)
     std::string v;

@@ -230,3 +230,23 @@ def test_regrouped_contraction_matches_oracle(view, prec, a_data):
     ex.close()
     assert (st["kernel"] == "blas") == (prec == "c128" and not view)
     assert err(out, ref) < (5e-6 if prec != "c128" else 1e-12)
+
+
+def test_split_bf16_product_matches_oracle():
+    """c64_bf16x3 on a 1024^3 contraction: three bf16 products of the hi / lo split,
+    accumulated in FP32.  The dropped lo.lo term and the split residual are ~2^-16 of
+    each product, so the normwise error stays near 1e-5 (bar: 1e-4), against 1e-6-ish
+    for the FP32 4M product of the same step."""
+    import qxtest
+    dsl, data, ref = qxtest.interleaved_gemm_program(nk=10, nm=10, nn=10, seed=5)
+    ex = Executor(dsl, data, precision="c64_bf16x3")
+    out = np.asarray(ex.amplitudes([""])).reshape(-1)
+    st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][-1]
+    ex.close()
+    assert st["kernel"] == "blas"
+    e_split = err(out, ref)
+    assert e_split < 1e-4
+    ex = Executor(dsl, data, precision="c64_4m")
+    e_4m = err(np.asarray(ex.amplitudes([""])).reshape(-1), ref)
+    ex.close()
+    assert e_4m < e_split < 200 * max(e_4m, 1e-7)

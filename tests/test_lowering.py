@@ -198,10 +198,16 @@ def test_batched_small_levels_do_not_share_memory():
         assert all(s["variant"] == "k_small_batch" for s in sts)
 
 
-def test_reserved_precision_is_rejected():
-    with pytest.raises(_native.QxError) as e:
-        _native.inspect("load A data_1 2\nsave output A\n", precision="c64_bf16x3")
-    assert e.value.status == -8
+def test_split_bf16_precision_takes_large_plain_steps():
+    """c64_bf16x3 runs plain steps with M, N, K >= 1024 as split bf16 products (the
+    library GEMM with hi/lo planes); smaller or non-plain steps keep the complex64
+    kernels, and c64_4m never takes the library GEMM by default."""
+    import qxtest
+    dsl, _, _ = qxtest.interleaved_gemm_program(nk=10, nm=10, nn=10)
+    assert _native.inspect(dsl, precision="c64_bf16x3")["steps"][-1]["kernel"] == "blas"
+    assert _native.inspect(dsl, precision="c64_4m")["steps"][-1]["kernel"] != "blas"
+    dsl, _, _ = qxtest.interleaved_gemm_program(nk=11, nm=8, nn=8)   # M = N = 256
+    assert _native.inspect(dsl, precision="c64_bf16x3")["steps"][-1]["kernel"] != "blas"
 
 
 def test_plain_matrix_steps_offer_rocblas():
