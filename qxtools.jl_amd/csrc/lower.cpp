@@ -421,12 +421,35 @@ void set_kernel_args(Step &st, int precision) {
     throw Error(QX_ERR_STATE, "internal: default kernel cannot run its step");
 }
 
+// The same contraction with the operands exchanged (C = B.A): M and N trade places,
+// so the row kernels can walk B's free legs (a skinny A with a long B).
+static Step swap_operands(const Step &st) {
+  Step v = st;
+  std::swap(v.in_a, v.in_b);
+  for (auto &g : v.cl) std::swap(g.sa, g.sb);
+  for (auto &g : v.kl) std::swap(g.sa, g.sb);
+  std::swap(v.M, v.N);
+  std::swap(v.radix.conj_a, v.radix.conj_b);
+  std::swap(v.bits.conj_a, v.bits.conj_b);
+  v.swapped = !st.swapped;
+  return v;
+}
+
 bool force_kernel(Step &st, int precision, int choice) {
   if (st.kern == Kern::Radix || st.kern == Kern::Chain) return false;
-  Step v = st;
-  if (!configure_kernel(v, precision, choice)) return false;
-  v.bits.conj_a = v.gemm.conj_a = v.rows.conj_a = st.bits.conj_a;
-  v.bits.conj_b = v.gemm.conj_b = v.rows.conj_b = st.bits.conj_b;
+  Step src = st;
+  Step v = src;
+  if (!configure_kernel(v, precision, choice)) {
+    // the row kernel over B's side when A's side cannot hold it
+    if (choice != kRows || st.swapped || st.level >= 0) return false;
+    src = swap_operands(st);
+    const StepBits sw = step_bits(src);
+    if (!(sw.nr >= 8 && rows_ok(sw))) return false;
+    v = src;
+    if (!configure_kernel(v, precision, choice)) return false;
+  }
+  v.bits.conj_a = v.gemm.conj_a = v.rows.conj_a = src.bits.conj_a;
+  v.bits.conj_b = v.gemm.conj_b = v.rows.conj_b = src.bits.conj_b;
   st = std::move(v);
   return true;
 }
@@ -466,6 +489,16 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
     if (c == def) continue;
     Step v = st;
     if (force_kernel(v, precision, c)) out.push_back(std::move(v));
+  }
+  // a skinny A against a long B: the row and gather kernels over B's side
+  if (sb.nn >= 12 && sb.nm <= 6 && !st.swapped && st.level < 0) {
+    const Step w = swap_operands(st);
+    const StepBits sw = step_bits(w);
+    for (int c : {kRows, kBits}) {
+      if (c == kRows && !(sw.nr >= 8 && rows_ok(sw))) continue;
+      Step v = w;
+      if (force_kernel(v, precision, c)) out.push_back(std::move(v));
+    }
   }
   return out;
 }

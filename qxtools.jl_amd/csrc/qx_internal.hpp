@@ -221,6 +221,7 @@ struct Step {
   Kern kern = Kern::Bits;
   int section = 0;         // 0 shared slice-free, 1 shared per slice, 2 per bitstring batch
   int level = -1;          // >= 0: small slice-free step run with the others of its level in one launch
+  bool swapped = false;    // operands exchanged (in_a is the DSL's B): C = B.A, M and N traded
   BitArgs bits;            // Kern::Bits / Kern::Reduce
   GemmArgs gemm;           // Kern::Gemm
   RowArgs rows;            // Kern::Rows

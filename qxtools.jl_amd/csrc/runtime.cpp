@@ -436,6 +436,7 @@ std::string kern_variant(const qx::Step &st);
 // Distinct name of a candidate configuration (kernel plus launch choices).
 std::string cand_label(const qx::Step &st) {
   std::string v = kern_variant(st);
+  if (st.swapped) v += "/swap";
   if (st.kern == qx::Kern::Chain)
     v += "/g" + std::to_string(st.chain.grid) + (st.chain.mfma16 ? "/mfma" : "/fma") + (st.chain.swizzled ? "/swz" : "");
   return v;

@@ -250,3 +250,20 @@ def test_split_bf16_product_matches_oracle():
     e_4m = err(np.asarray(ex.amplitudes([""])).reshape(-1), ref)
     ex.close()
     assert e_4m < e_split < 200 * max(e_4m, 1e-7)
+
+
+SWAP_SHAPES = [(1, 2, 12, 2), (0, 3, 13, 1), (2, 1, 10, 3), (0, 2, 14, 4)]
+
+
+@pytest.mark.parametrize("shape", SWAP_SHAPES)
+@pytest.mark.parametrize("prec", ["c128", "c64_4m"])
+def test_rows_kernel_over_swapped_operands(shape, prec):
+    """A skinny A against a long B: the row kernel forced onto the step runs over B's
+    free legs with the operands exchanged (C = B.A), and matches the oracle."""
+    dsl, data, ref = single_step(300 + sum(shape), *shape)
+    ex = Executor(dsl, data, precision=prec, kernel=1)
+    out = np.asarray(ex.amplitudes([""])).reshape(-1)
+    st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][0]
+    ex.close()
+    assert st["kernel"] == "rows" and st["config"].endswith("/swap")
+    assert err(out, ref) < (5e-6 if prec != "c128" else 1e-12)
