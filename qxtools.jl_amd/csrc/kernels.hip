@@ -980,17 +980,134 @@ void launch_contract_gemm(int prec, const void *A, const void *B, void *C, void 
   }
 }
 
+// Skinny contraction: C[h, m, n] = sum_k A[h, m, k] B[h, k, n] for at most 32 (m, n)
+// pairs (N <= 2) and a long K.  Workgroup (h, chunk): each lane walks K indices of the
+// chunk (low K bits, A's smallest strides), loads its A value for every m and B value
+// for every n once, and keeps the MB x NB sums; the workgroup reduces them (wave
+// shuffles, then LDS) into the chunk's partial copy of C.  Every pass reads A and B
+// once (the one-wave-per-element reduce reads A N times and B M times).
+template <typename T, int MB, int NB>
+__global__ __launch_bounds__(256) void k_contract_skinny(const typename Vec<T>::type *__restrict__ A,
+                                                         const typename Vec<T>::type *__restrict__ B,
+                                                         typename Vec<T>::type *__restrict__ P, int64_t c_elems,
+                                                         BitArgs a) {
+  using V = typename Vec<T>::type;
+  const int chunk = (int)(blockIdx.x % a.ksplit);
+  const int64_t h = blockIdx.x / a.ksplit;
+  int64_t oa = 0, ob = 0, ch = 0;
+  {
+    int hi = 0;
+    for (int i = 0; i < a.nbc; ++i)
+      if (a.wa[i] && a.wb[i]) {
+        if ((h >> hi) & 1) oa += a.wa[i], ob += a.wb[i], ch += int64_t(1) << i;
+        ++hi;
+      }
+  }
+  int64_t am[MB], cm[MB], bn[NB], cn[NB];
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    int64_t o = 0, c = 0;
+    int j = 0;
+    for (int i = 0; i < a.nbc; ++i)
+      if (a.wa[i] && !a.wb[i]) {
+        if ((m >> j) & 1) o += a.wa[i], c += int64_t(1) << i;
+        ++j;
+      }
+    am[m] = o, cm[m] = c;
+  }
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+    int64_t o = 0, c = 0;
+    int j = 0;
+    for (int i = 0; i < a.nbc; ++i)
+      if (!a.wa[i] && a.wb[i]) {
+        if ((n >> j) & 1) o += a.wb[i], c += int64_t(1) << i;
+        ++j;
+      }
+    bn[n] = o, cn[n] = c;
+  }
+  for (int b = a.kcb; b < a.nbk; ++b)
+    if ((chunk >> (b - a.kcb)) & 1) oa += a.ka[b], ob += a.kb[b];
+  V acc[MB][NB];
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int n = 0; n < NB; ++n) acc[m][n] = V{0, 0};
+  const int64_t kn = int64_t(1) << a.kcb;
+  for (int64_t k = threadIdx.x; k < kn; k += 256) {
+    int64_t xa = oa, xb = ob;
+    for (int b = 0; b < a.kcb; ++b)
+      if ((k >> b) & 1) xa += a.ka[b], xb += a.kb[b];
+    V av[MB], bv[NB];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) av[m] = A[xa + am[m]];
+#pragma unroll
+    for (int n = 0; n < NB; ++n) bv[n] = B[xb + bn[n]];
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < NB; ++n) cmac<T>(acc[m][n], av[m], bv[n], a.conj_a, a.conj_b);
+  }
+  __shared__ V red[4][MB * NB];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      V v = acc[m][n];
+      for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_xor(v.x, off, 64);
+        v.y += __shfl_xor(v.y, off, 64);
+      }
+      if (lane == 0) red[wave][m * NB + n] = v;
+    }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+      if (threadIdx.x == m * NB + n) {
+        const int j = m * NB + n;
+        V sum = red[0][j];
+        for (int w = 1; w < 4; ++w) sum.x += red[w][j].x, sum.y += red[w][j].y;
+        P[(int64_t)chunk * c_elems + ch + cm[m] + cn[n]] = sum;
+      }
+}
+
+template <typename T>
+void launch_skinny(const void *A, const void *B, void *P, int64_t c_elems, const BitArgs &args, hipStream_t s) {
+  using V = typename Vec<T>::type;
+  const int64_t groups = (c_elems >> (args.nbm + args.nbn)) * args.ksplit;
+  const dim3 g((unsigned)groups), t(256);
+#define QX_SKINNY(MBITS, NBITS)                                                                             \
+  if (args.nbm == MBITS && args.nbn == NBITS) {                                                             \
+    hipLaunchKernelGGL((k_contract_skinny<T, 1 << MBITS, 1 << NBITS>), g, t, 0, s, (const V *)A, (const V *)B, \
+                       (V *)P, c_elems, args);                                                              \
+    return;                                                                                                 \
+  }
+  QX_SKINNY(0, 0) QX_SKINNY(1, 0) QX_SKINNY(2, 0) QX_SKINNY(3, 0) QX_SKINNY(4, 0) QX_SKINNY(5, 0)
+  QX_SKINNY(0, 1) QX_SKINNY(1, 1) QX_SKINNY(2, 1) QX_SKINNY(3, 1) QX_SKINNY(4, 1)
+#undef QX_SKINNY
+  throw Error(QX_ERR_STATE, "internal: skinny contraction shape");
+}
+
 void launch_contract_reduce(int prec, const void *A, const void *B, void *C, void *ws, int64_t c_elems,
                             const BitArgs &args, void *stream) {
   int g = grid_for(c_elems * args.ksplit, 4);
   hipStream_t s = (hipStream_t)stream;
   void *dst = args.ksplit > 1 ? ws : C;
-  if (prec == QX_C128)
+  if (args.skinny) {
+    if (prec == QX_C128)
+      launch_skinny<double>(A, B, dst, c_elems, args, s);
+    else
+      launch_skinny<float>(A, B, dst, c_elems, args, s);
+  } else if (prec == QX_C128) {
     hipLaunchKernelGGL(k_contract_reduce<double>, dim3(g), dim3(256), 0, s, (const double2 *)A, (const double2 *)B,
                        (double2 *)dst, c_elems, args);
-  else
+  } else {
     hipLaunchKernelGGL(k_contract_reduce<float>, dim3(g), dim3(256), 0, s, (const float2 *)A, (const float2 *)B,
                        (float2 *)dst, c_elems, args);
+  }
   check_launch("contract_reduce");
   if (args.ksplit > 1) {
     int gs = grid_for(c_elems, 256);

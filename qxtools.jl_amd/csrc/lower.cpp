@@ -131,7 +131,7 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
 
 // Kernel configurations a power-of-two step can run with.
 enum Choice { kBits = 0, kRows = 1, kGemm0 = 2, kGemm1 = 3, kGemm2 = 4, kMfma = 5, kReduce = 6, kMfma32 = 7,
-              kBlas = 8, kNumChoices = 9 };
+              kBlas = 8, kSkinny = 9, kNumChoices = 10 };
 
 namespace {
 struct CB {
@@ -293,6 +293,25 @@ bool configure_kernel(Step &st, int precision, int choice) {
       if (nk < 6) return false;
       st.kern = Kern::Reduce;
       while (st.c_elems * b.ksplit < 8192 && (int64_t(1) << nk) / b.ksplit >= 128) b.ksplit *= 2;
+      break;
+    }
+    case kSkinny: {
+      // C = H x M x N with few M N values against a long K: chunks of K so that the
+      // H elements times the chunks give about 1024 workgroups, each at least 2^8 long
+      if (nk < 12 || nm + nn > 5 || nn > 1) return false;
+      st.kern = Kern::Reduce;
+      b.skinny = 1;
+      b.nbm = nm;
+      b.nbn = nn;
+      const int nh = (int)sb.hb.size();
+      int split_bits = 0;
+      while (nh + split_bits < 10 && nk - split_bits > 8) ++split_bits;
+      b.kcb = nk - split_bits;
+      b.ksplit = 1 << split_bits;
+      // lanes walk the low K bits: give them A's smallest strides
+      std::vector<std::pair<int64_t, int64_t>> kb(sb.kbits);
+      std::sort(kb.begin(), kb.end());
+      for (int i = 0; i < nk; ++i) b.ka[i] = kb[i].first, b.kb[i] = kb[i].second;
       break;
     }
     case kRows: {
@@ -482,6 +501,7 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
   if (sb.nk >= 5 && sb.nn >= 4 && sb.nm >= 4) alts.push_back(kMfma32);
   if (sb.nr >= 8 && rows_ok(sb)) alts.push_back(kRows);
   if (st.c_elems <= (int64_t(1) << 16) && sb.nk >= 6) alts.push_back(kReduce);
+  if (sb.nk >= 12 && sb.nm + sb.nn <= 5 && sb.nn <= 1) alts.push_back(kSkinny);
   if (sb.nk >= 4 && sb.nn >= 4 && sb.nm >= 4) alts.push_back(kBlas);
   if (sb.nk <= 6 || st.c_elems <= (int64_t(1) << 16)) alts.push_back(kBits);
   const int def = default_choice(st, precision);

@@ -88,6 +88,9 @@ struct BitArgs {
   int nbc = 0, nbk = 0;
   int conj_a = 0, conj_b = 0;
   int ksplit = 1;          // K ranges whose partials are summed after (Reduce and Bits)
+  // Reduce only: skinny form (C = H x M x N with M N <= 32, N <= 2, K long): one
+  // workgroup per (H element, K chunk of 2^kcb) keeps all M N sums per lane
+  int skinny = 0, kcb = 0, nbm = 0, nbn = 0;
 };
 
 // Rows kernel: C[r, n] = sum_k A[r, k] B[h(r), k, n] with r over the H and M bits

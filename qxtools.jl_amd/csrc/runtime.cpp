@@ -850,7 +850,10 @@ std::string kern_variant(const qx::Step &st) {
       if (st.gemm.tile == 4) return "k_contract_gemm_mfma<32>";
       return st.gemm.tile == 0 ? "k_contract_gemm<128,16,16>" : st.gemm.tile == 1 ? "k_contract_gemm<128,32,16>"
                                                                                    : "k_contract_gemm<64,64,16>";
-    case qx::Kern::Reduce: return "k_contract_reduce";
+    case qx::Kern::Reduce:
+      if (st.bits.skinny)
+        return "k_contract_skinny<" + std::to_string(1 << st.bits.nbm) + "," + std::to_string(1 << st.bits.nbn) + ">";
+      return "k_contract_reduce";
     case qx::Kern::Radix: return "k_contract_radix";
     case qx::Kern::Chain: return "k_contract_chain";
     case qx::Kern::Blas:
@@ -1332,7 +1335,7 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
     } else if (n == "batch_last") {
       plan->opt_batch_last = value;
     } else if (n == "kernel") {
-      if (value < -1 || value > 8) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..8");
+      if (value < -1 || value > 9) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..9");
       plan->opt_kernel = value;
     } else if (n == "profile") {
       // may be toggled at any time; every set clears the accumulated step times

@@ -267,3 +267,20 @@ def test_rows_kernel_over_swapped_operands(shape, prec):
     ex.close()
     assert st["kernel"] == "rows" and st["config"].endswith("/swap")
     assert err(out, ref) < (5e-6 if prec != "c128" else 1e-12)
+
+
+SKINNY_SHAPES = [(1, 3, 1, 12), (0, 4, 0, 13), (2, 2, 1, 12), (1, 5, 0, 12), (0, 0, 1, 14)]
+
+
+@pytest.mark.parametrize("shape", SKINNY_SHAPES)
+@pytest.mark.parametrize("prec", ["c128", "c64_4m"])
+def test_skinny_contraction_matches_oracle(shape, prec):
+    """Long K into at most 32 outputs per batch element (kernel 9): every lane keeps all
+    (m, n) sums of its K slice, split K partials summed after."""
+    dsl, data, ref = single_step(500 + sum(shape), *shape)
+    ex = Executor(dsl, data, precision=prec, kernel=9)
+    out = np.asarray(ex.amplitudes([""])).reshape(-1)
+    st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][0]
+    ex.close()
+    assert st["variant"].startswith("k_contract_skinny")
+    assert err(out, ref) < (5e-6 if prec != "c128" else 1e-12)
