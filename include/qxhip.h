@@ -99,8 +99,10 @@ qx_status qx_plan_set_tensor(qx_plan *plan, const char *key, const double *z, in
  *                 FP tile GEMMs, 5 / 7 matrix-core GEMM (K chunks of 16 / 32), 6 reduce,
  *                 8 rocBLAS strided-batched GEMM (steps whose operands are plain
  *                 column-major matrices), 9 skinny contraction (K >= 2^12, at most 32
- *                 output values per batch element, N <= 2) = use that kernel on every
- *                 step it can run (testing; the autotuner also considers 8 and 9) */
+ *                 output values per batch element, N <= 2), 10 rocBLAS on gathered
+ *                 plain copies (steps of >= 2^34 flops whose operands are not plain) =
+ *                 use that kernel on every step it can run (testing; the autotuner
+ *                 also considers 8-10) */
 qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value);
 
 /* Check that every data key is present, lower the program (label classes,

@@ -852,6 +852,22 @@ __global__ void k_bf16_split(const float2 *__restrict__ src, int64_t n, uint16_t
   }
 }
 
+struct GatherArgs {
+  int nb;
+  int64_t w[kMaxBits];
+};
+
+template <typename T>
+__global__ void k_gather(const typename Vec<T>::type *__restrict__ src, typename Vec<T>::type *__restrict__ dst,
+                         int64_t n, GatherArgs g) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t o = 0;
+    for (int b = 0; b < g.nb; ++b)
+      if ((i >> b) & 1) o += g.w[b];
+    dst[i] = src[o];
+  }
+}
+
 __global__ void k_planar_merge(const float *__restrict__ re, const float *__restrict__ im, int64_t n,
                                float2 *__restrict__ dst) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1806,6 +1822,22 @@ void launch_bf16_split(const void *src, int64_t n, void *out, void *stream) {
   hipLaunchKernelGGL(k_bf16_split, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, (const float2 *)src, n,
                      (uint16_t *)out);
   check_launch("bf16_split");
+}
+
+void launch_gather(int prec, const void *src, void *dst, int nb, const int64_t *w, void *stream) {
+  if (nb < 0 || nb > kMaxBits) throw Error(QX_ERR_STATE, "internal: gather bits");
+  GatherArgs g;
+  g.nb = nb;
+  for (int b = 0; b < nb; ++b) g.w[b] = w[b];
+  const int64_t n = int64_t(1) << nb;
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == QX_C128)
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const double2 *)src,
+                       (double2 *)dst, n, g);
+  else
+    hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const float2 *)src,
+                       (float2 *)dst, n, g);
+  check_launch("gather");
 }
 
 void launch_planar_merge(const float *re, const float *im, int64_t n, void *dst, void *stream) {

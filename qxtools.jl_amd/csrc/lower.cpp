@@ -131,7 +131,7 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
 
 // Kernel configurations a power-of-two step can run with.
 enum Choice { kBits = 0, kRows = 1, kGemm0 = 2, kGemm1 = 3, kGemm2 = 4, kMfma = 5, kReduce = 6, kMfma32 = 7,
-              kBlas = 8, kSkinny = 9, kNumChoices = 10 };
+              kBlas = 8, kSkinny = 9, kTtgt = 10, kNumChoices = 11 };
 
 namespace {
 struct CB {
@@ -294,6 +294,45 @@ bool configure_kernel(Step &st, int precision, int choice) {
       st.kern = Kern::Reduce;
       while (st.c_elems * b.ksplit < 8192 && (int64_t(1) << nk) / b.ksplit >= 128) b.ksplit *= 2;
       break;
+    }
+    case kTtgt: {
+      // a very large step whose operands are not plain matrices: A and B gathered into
+      // plain copies, the library GEMM, the result gathered into C (the copies cost a
+      // pass over each operand; the autotuner keeps it only where that pays)
+      if (nm < 4 || nn < 4 || nk < 4 || 8.0 * st.H * st.M * st.N * st.K < double(int64_t(1) << 34)) return false;
+      BlasArgs pa;
+      if (plain_gemm(st, pa) || st.radix.conj_a || st.radix.conj_b) return false;
+      const int nh = (int)sb.hb.size();
+      if (nm + nk + nh > kMaxBits || nk + nn + nh > kMaxBits) return false;
+      BlasArgs ba;
+      const int64_t M = int64_t(1) << nm, N = int64_t(1) << nn, K = int64_t(1) << nk, H = int64_t(1) << nh;
+      if (M >= (int64_t(1) << 31) || N >= (int64_t(1) << 31) || K >= (int64_t(1) << 31) || H >= (int64_t(1) << 31))
+        return false;
+      for (auto &c : sb.mb) ba.ga[ba.nga++] = c.wa;
+      for (auto &k : sb.kbits) ba.ga[ba.nga++] = k.first;
+      for (auto &c : sb.hb) ba.ga[ba.nga++] = c.wa;
+      for (auto &k : sb.kbits) ba.gb[ba.ngb++] = k.second;
+      for (auto &c : sb.nbits) ba.gb[ba.ngb++] = c.wb;
+      for (auto &c : sb.hb) ba.gb[ba.ngb++] = c.wb;
+      int jm = 0, jn = 0, jh = 0;   // C bits (C order) -> their weight in the plain result
+      for (auto &c : sb.cbits) {
+        if (c.wa && c.wb) ba.gc[ba.ngc++] = (M * N) << jh++;
+        else if (c.wa) ba.gc[ba.ngc++] = int64_t(1) << jm++;
+        else ba.gc[ba.ngc++] = M << jn++;
+      }
+      ba.ttgt = 1;
+      ba.r = M, ba.s = N, ba.k = K, ba.ldx = M, ba.ldy = K, ba.ldc = M;
+      ba.sx = M * K, ba.sy = K * N, ba.sc = M * N, ba.batch = H;
+      ba.ta = M * K * H, ba.tb = K * N * H;
+      const int64_t tc = M * N * H;
+      st.ws_elems = ba.ta + ba.tb + tc + 512;
+      if (precision == QX_C64_BF16X3 && M >= 1024 && N >= 1024 && K >= 1024) {
+        ba.split = 1, ba.ex = ba.ta, ba.ey = ba.tb, ba.ec = tc;
+        st.ws_elems += ba.ex + ba.ey + ba.ec;
+      }
+      st.kern = Kern::Blas;
+      st.blas = ba;
+      return true;
     }
     case kSkinny: {
       // C = H x M x N with few M N values against a long K: chunks of K so that the
@@ -502,6 +541,8 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
   if (sb.nr >= 8 && rows_ok(sb)) alts.push_back(kRows);
   if (st.c_elems <= (int64_t(1) << 16) && sb.nk >= 6) alts.push_back(kReduce);
   if (sb.nk >= 12 && sb.nm + sb.nn <= 5 && sb.nn <= 1) alts.push_back(kSkinny);
+  if (sb.nk >= 4 && sb.nn >= 4 && sb.nm >= 4 && 8.0 * st.H * st.M * st.N * st.K >= double(int64_t(1) << 34))
+    alts.push_back(kTtgt);
   if (sb.nk >= 4 && sb.nn >= 4 && sb.nm >= 4) alts.push_back(kBlas);
   if (sb.nk <= 6 || st.c_elems <= (int64_t(1) << 16)) alts.push_back(kBits);
   const int def = default_choice(st, precision);

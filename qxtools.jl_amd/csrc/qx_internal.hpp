@@ -209,6 +209,12 @@ struct BlasArgs {
   // extent ec), then interleaved into C.  0: one complex64 GEMM.
   int split = 0;
   int64_t ex = 0, ey = 0, ec = 0;
+  // ttgt: the step's operands are not plain; A and B are first gathered into plain
+  // [M][K][H] / [K][N][H] copies and C from a plain [M][N][H] result (weights of the
+  // copies' bits in the source, dense order), after the split planes in the workspace
+  int ttgt = 0, nga = 0, ngb = 0, ngc = 0;
+  int64_t ga[kMaxBits] = {0}, gb[kMaxBits] = {0}, gc[kMaxBits] = {0};
+  int64_t ta = 0, tb = 0;   // elements of the A and B copies (the result copy: r s batch)
 };
 
 enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows, Chain, Blas };
@@ -353,6 +359,8 @@ void launch_add_batch(int prec, const AddDesc *d_desc, int n, void *stream);
 // (re hi, re lo, im hi, im lo); hi = x rounded to bf16 (nearest, ties to even),
 // lo = (x - hi) rounded likewise.
 void launch_bf16_split(const void *src, int64_t n, void *out, void *stream);
+// dst[i] = src[sum of bit_j(i) * w[j]] for i < 2^nb (a permuted dense copy).
+void launch_gather(int prec, const void *src, void *dst, int nb, const int64_t *w, void *stream);
 // dst[i] = (re[i], im[i]) as complex64, i < n.
 void launch_planar_merge(const float *re, const float *im, int64_t n, void *dst, void *stream);
 // d_ranges: n (offset, length) pairs in 16-byte units from base; one workgroup each.

@@ -252,6 +252,21 @@ void blas_before_capture(qx_ctx *c, const std::vector<qx::Step> &steps) {
 // C = op(X) op(Y) by rocBLAS (strided batched, no copies): the step's operands are
 // plain column-major matrices (see qx::BlasArgs).
 void launch_blas(qx_plan *P, const qx::BlasArgs &b, const void *A, const void *B, void *C) {
+  if (b.ttgt) {
+    // plain copies after the split planes: gather A and B, multiply, gather C
+    auto up = [](int64_t bytes) { return (bytes + 255) / 256 * 256; };
+    const int64_t es = (int64_t)P->esize();
+    char *w = (char *)P->ws.p + (b.split ? up(8 * b.ex) + up(8 * b.ey) + 2 * up(4 * b.ec) : 0);
+    char *pa = w, *pb = pa + up(b.ta * es), *pc = pb + up(b.tb * es);
+    hipStream_t stream = P->ctx->stream;
+    qx::launch_gather(P->prec, A, pa, b.nga, b.ga, stream);
+    qx::launch_gather(P->prec, B, pb, b.ngb, b.gb, stream);
+    qx::BlasArgs inner = b;
+    inner.ttgt = 0;
+    launch_blas(P, inner, pa, pb, pc);
+    qx::launch_gather(P->prec, pc, C, b.ngc, b.gc, stream);
+    return;
+  }
   rocblas_handle h = blas_handle(P->ctx);
   const void *X = b.swap ? B : A, *Y = b.swap ? A : B;
   const rocblas_operation ox = b.opx ? rocblas_operation_transpose : rocblas_operation_none;
@@ -858,7 +873,7 @@ std::string kern_variant(const qx::Step &st) {
     case qx::Kern::Chain: return "k_contract_chain";
     case qx::Kern::Blas:
       return std::string("rocblas_gemm_strided_batched<") + (st.blas.opx ? "T" : "N") + (st.blas.opy ? "T" : "N") +
-             (st.blas.swap ? ",BA>" : ",AB>");
+             (st.blas.swap ? ",BA>" : ",AB>") + (st.blas.ttgt ? "+gather" : "");
     default: return "k_contract_bits";
   }
 }
@@ -1335,7 +1350,7 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
     } else if (n == "batch_last") {
       plan->opt_batch_last = value;
     } else if (n == "kernel") {
-      if (value < -1 || value > 9) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..9");
+      if (value < -1 || value > 10) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..10");
       plan->opt_kernel = value;
     } else if (n == "profile") {
       // may be toggled at any time; every set clears the accumulated step times

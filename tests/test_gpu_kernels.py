@@ -284,3 +284,16 @@ def test_skinny_contraction_matches_oracle(shape, prec):
     ex.close()
     assert st["variant"].startswith("k_contract_skinny")
     assert err(out, ref) < (5e-6 if prec != "c128" else 1e-12)
+
+
+@pytest.mark.parametrize("prec,tol", [("c128", 1e-12), ("c64_4m", 5e-6), ("c64_bf16x3", 1e-4)])
+def test_gathered_library_gemm_matches_oracle(prec, tol):
+    """A 2^34-flop step with shuffled leg orders (kernel 10): A and B gathered into plain
+    copies, the library GEMM (the split bf16 product for c64_bf16x3), C gathered back."""
+    dsl, data, ref = single_step(811, 0, 10, 10, 11)
+    ex = Executor(dsl, data, precision=prec, kernel=10)
+    out = np.asarray(ex.amplitudes([""])).reshape(-1)
+    st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][0]
+    ex.close()
+    assert st["kernel"] == "blas" and st["variant"].endswith("+gather")
+    assert err(out, ref) < tol
