@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <set>
 #include <sstream>
 #include <tuple>
 
@@ -87,6 +88,7 @@ struct qx_plan {
   int64_t opt_batch = 256;
   int64_t opt_arena_mb = -1;
   int64_t opt_graph = 0;
+  int64_t last_launches = 0;   // launches of the last amplitudes / vjp call (estimate)
   int64_t opt_profile = 0;
   int64_t opt_autotune = 0;
   int64_t opt_kernel = -1;                  // >= 0: force that kernel wherever it can run
@@ -595,6 +597,24 @@ void slice_values(const qx_plan *P, uint64_t fused, int64_t sl, std::vector<int6
   }
 }
 
+// Launches of one section of a lowered program per pass: unbatched steps, plus one
+// per level of batched small steps.
+int64_t section_launches(const qx::Lowered &L, int sec) {
+  int64_t n = 0;
+  std::set<int> levels;
+  for (auto &st : L.steps)
+    if (st.section == sec) {
+      if (st.level >= 0) levels.insert(st.level);
+      else ++n;
+    }
+  return n + (int64_t)levels.size();
+}
+
+// A call of more launches than this runs without graph capture: capturing and
+// instantiating a graph of that many nodes costs host time and memory that its replay
+// cannot win back, since such a call runs for a long time anyway.
+constexpr int64_t kMaxGraphLaunches = int64_t(1) << 15;
+
 void check_bits(qx_plan *P, const char *bits, int64_t nb) {
   const int64_t nch = P->prog.num_chars;
   for (int64_t i = 0; i < nb * nch; ++i) {
@@ -727,7 +747,11 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     }
     qx::launch_finalize((const double *)P->acc.p, (double *)P->outbuf.p, nb * R, stream);
   };
-  if (P->opt_graph && !P->opt_profile && !tuner) {
+  const int64_t nrows = (nb + batch - 1) / batch;
+  const int64_t launches = 2 + section_launches(L, 0) +
+                           (s1 - s0) * (section_launches(L, 1) + nrows * (section_launches(L, 2) + 2));
+  P->last_launches = launches;
+  if (P->opt_graph && !P->opt_profile && !tuner && launches <= kMaxGraphLaunches) {
     auto key = std::make_tuple(batch, nb, s0, s1 + (fused ? (int64_t(1) << 40) : 0));
     auto it = P->graphs.find(key);
     if (it == P->graphs.end()) {
@@ -1202,7 +1226,15 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
   }
   run_bwd(0);
   };
-  if (P->opt_graph && !P->opt_profile) {
+  // forward launches plus at most two per backward step (a contraction and its add)
+  int64_t bwd[3] = {0, 0, 0};
+  for (int sec = 0; sec < 3; ++sec) bwd[sec] = 2 * (int64_t)V.bsteps[sec].size();
+  const int64_t nrows = (nb + batch - 1) / batch;
+  const int64_t launches = 1 + section_launches(L, 0) + bwd[0] + 1 +
+                           nslices * (section_launches(L, 1) + bwd[1] + 1 +
+                                      nrows * (section_launches(L, 2) + bwd[2] + 3));
+  P->last_launches = launches;
+  if (P->opt_graph && !P->opt_profile && launches <= kMaxGraphLaunches) {
     auto key = std::make_tuple(batch, nb, int64_t(-1), (int64_t)fused);
     auto it = P->vjp_graphs.find(key);
     if (it == P->vjp_graphs.end()) {
@@ -1234,7 +1266,8 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
 std::string stats_json(const qx_plan *P) {
   std::ostringstream os;
   os.precision(17);
-  os << "{\"precision\":" << P->prec << ",\"batches\":[";
+  os << "{\"precision\":" << P->prec << ",\"last_call_launches\":" << P->last_launches
+     << ",\"max_graph_launches\":" << kMaxGraphLaunches << ",\"batches\":[";
   bool first = true;
   for (auto &kv : P->lowered) {
     const qx::Lowered &L = kv.second;

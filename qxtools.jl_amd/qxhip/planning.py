@@ -418,6 +418,7 @@ class PlanCost:
             for i in s:
                 count[i] = count.get(i, 0) + 1
         self.steps = []   # (all indices of A u B, indices of C)
+        self.operands = []   # (indices of A, indices of B)
         for a, b, c in plan:
             sa, sb = cur.pop(a), cur.pop(b)
             shared = sa & sb
@@ -427,6 +428,7 @@ class PlanCost:
             sc = (sa | sb) - gone
             cur[c] = sc
             self.steps.append((frozenset(sa | sb), frozenset(sc)))
+            self.operands.append((frozenset(sa), frozenset(sb)))
 
     def cost(self, sliced=frozenset()) -> tuple:
         """(log2 total complex flops of one slice incl. 8x, log2 largest intermediate)"""
@@ -439,6 +441,24 @@ class PlanCost:
             s = sum(lg[i] for i in cidx if i not in sliced)
             mx = max(mx, s)
         return math.log2(8.0 * tot) if tot > 0 else 0.0, mx
+
+    def roofline_seconds(self, sliced=frozenset(), element_bytes: int = 16,
+                         flops_peak: float = 78.6e12, hbm_bw: float = 8.0e12) -> float:
+        """Roofline time of the whole contraction over all slices: per step of one
+        slice max(8 B M N K / flops_peak, e (|A| + |B| + |C|) / hbm_bw), summed over the
+        steps and multiplied by the number of slices (the product of the sliced
+        dimensions)."""
+        lg = self.lg
+
+        def size(idx):
+            return 2.0 ** sum(lg[i] for i in idx if i not in sliced)
+
+        t = 0.0
+        for (allidx, cidx), (aidx, bidx) in zip(self.steps, self.operands):
+            flops = 8.0 * size(allidx)
+            nbytes = element_bytes * (size(aidx) + size(bidx) + size(cidx))
+            t += max(flops / flops_peak, nbytes / hbm_bw)
+        return t * 2.0 ** sum(lg[i] for i in sliced)
 
 
 def contraction_plan(tn, trials: int = 32, seed: int = 42, max_log2_size: float | None = None,
@@ -499,6 +519,40 @@ def greedy_slicing(tn, plan: list, num: int, open_indices=()) -> list:
     return sliced
 
 
+def roofline_slicing(tn, plan: list, num: int, open_indices=(), element_bytes: int = 16,
+                     flops_peak: float = 78.6e12, hbm_bw: float = 8.0e12,
+                     max_log2_size: float | None = None) -> list:
+    """Pick `num` indices to slice with the plan order fixed, scoring each choice by
+    the roofline time of all slices (`PlanCost.roofline_seconds`) instead of flops.
+
+    Without `max_log2_size` each pick first minimises the largest intermediate, like
+    `greedy_slicing`, and breaks ties by roofline time; with it, each pick minimises
+    the excess over that size first and then the roofline time, so once the plan fits
+    the picks add the least device time.  Replaces the treewidth cost of
+    `contraction_scheme` (`src/contraction_planning.jl:219-299`)."""
+    open_indices = _open_of(tn, open_indices)
+    tn = _as_tn(tn)
+    pc = PlanCost(tn, plan, open_indices)
+    open_set = set(open_indices)
+    candidates = sorted(i for i in tn.dims if i not in open_set and tn.dims[i] > 1)
+    sliced: list = []
+    for _ in range(num):
+        best, best_key = None, None
+        for i in candidates:
+            if i in sliced:
+                continue
+            s = frozenset(sliced + [i])
+            mx = pc.cost(s)[1]
+            width = mx if max_log2_size is None else max(0.0, mx - max_log2_size)
+            key = (round(width, 6), pc.roofline_seconds(s, element_bytes, flops_peak, hbm_bw))
+            if best_key is None or key < best_key:
+                best, best_key = i, key
+        if best is None:
+            break
+        sliced.append(best)
+    return sliced
+
+
 def expand_slice_bonds_to_hyperindices(tn, bonds: list) -> list:
     """Group sliced indices by hyperedge (`src/contraction_planning.jl:190-203`).
     An index here already names its whole hyperedge, so each group has one index."""
@@ -510,16 +564,30 @@ def expand_slice_bonds_to_hyperindices(tn, bonds: list) -> list:
 
 
 def contraction_scheme(tn, num: int, trials: int = 32, seed: int = 42,
-                       max_log2_size: float | None = None, min_fill_trials: int = 0) -> tuple:
+                       max_log2_size: float | None = None, min_fill_trials: int = 0,
+                       slicer: str = "width", element_bytes: int = 16,
+                       max_log2_slice_size: float | None = None) -> tuple:
     """(bond_groups, plan, metadata) as `contraction_scheme` returns
-    (`src/contraction_planning.jl:219-299`)."""
+    (`src/contraction_planning.jl:219-299`).
+
+    slicer: "width" (`greedy_slicing`, flops as the tie-break) or "roofline"
+    (`roofline_slicing` with `element_bytes` per element and an optional cap on the
+    largest intermediate of one slice)."""
+    if slicer not in ("width", "roofline"):
+        raise ValueError(f"unknown slicer {slicer!r}")
     open_indices = _open_of(tn, ())
     tn = _as_tn(tn)
     plan = contraction_plan(tn, trials=trials, seed=seed, max_log2_size=max_log2_size,
                             open_indices=open_indices, min_fill_trials=min_fill_trials)
     pc = PlanCost(tn, plan, open_indices)
     flops0, mx0 = pc.cost()
-    sliced = greedy_slicing(tn, plan, num, open_indices) if num > 0 else []
+    if num <= 0:
+        sliced = []
+    elif slicer == "roofline":
+        sliced = roofline_slicing(tn, plan, num, open_indices, element_bytes=element_bytes,
+                                  max_log2_size=max_log2_slice_size)
+    else:
+        sliced = greedy_slicing(tn, plan, num, open_indices)
     widths = [pc.cost(frozenset(sliced[:k + 1]))[1] for k in range(len(sliced))]
     groups = expand_slice_bonds_to_hyperindices(tn, sliced)
     meta = OrderedDict()
@@ -530,8 +598,10 @@ def contraction_scheme(tn, num: int, trials: int = 32, seed: int = 42,
         ("Hypergraph used", True),
     ])
     meta["Slicing"] = OrderedDict([
-        ("Method used", "greedy largest-intermediate reduction, order fixed"),
+        ("Method used", "greedy largest-intermediate reduction, order fixed" if slicer == "width"
+         else "greedy roofline-time reduction, order fixed"),
         ("Edges sliced", num),
         ("log2 largest intermediate after slicing consecutive edges", [round(w, 3) for w in widths]),
+        ("Roofline seconds, all slices", float(f"{pc.roofline_seconds(frozenset(sliced), element_bytes):.6g}")),
     ])
     return groups, plan, meta

@@ -32,6 +32,27 @@ def test_graph_replays_match_eager(planner, nslice, nb, batch):
     assert np.array_equal(part, eager.amplitudes(bits, 0, 1))
 
 
+def test_call_beyond_graph_launch_cap_runs_uncaptured():
+    """A call of more launches than the capture limit (256 looped slices x 8 batches
+    of bitstrings, about 50K launches) runs without capture under the graph option,
+    with the eager results bit for bit."""
+    circ = circuits.create_rqc_circuit(3, 3, 10, seed=21)
+    t, plan, g = qxtest.graph_for(circ, planner="greedy", nslice=8)
+    dsl = g.to_dsl()
+    bits = simulation.amplitudes_uniform(9, 7, 32)
+    eager = Executor(dsl, g.data, batch=4, fuse_slices=0)
+    graphed = Executor(dsl, g.data, batch=4, graph=1, fuse_slices=0)
+    want = eager.amplitudes(bits)
+    for rep in range(2):
+        assert np.array_equal(graphed.amplitudes(bits), want), f"call {rep}"
+    st = graphed.stats()
+    assert st["last_call_launches"] > st["max_graph_launches"]
+    fused = Executor(dsl, g.data, batch=4)
+    assert np.max(qxtest.rel_err(want, fused.amplitudes(bits))) < 1e-12
+    ref = O.amplitudes(dsl, g.data, bits[:2])
+    assert np.max(qxtest.rel_err(want[:2], ref)) < 1e-12
+
+
 @pytest.mark.parametrize("planner,nslice,nb,batch", [("sweep", 0, 8, 8), ("greedy", 2, 6, 4)])
 def test_vjp_graph_replays_match_eager(planner, nslice, nb, batch):
     """qx_amplitudes_vjp replayed from its captured graph equals the eager pullback,

@@ -104,6 +104,51 @@ def test_slicing_scheme_groups_and_metadata():
     assert len(plan) == len(tnc) - 1
 
 
+@pytest.mark.parametrize("nslice,e", [(0, 16), (2, 16), (3, 8)])
+def test_roofline_cost_matches_lowered_step_shapes(nslice, e):
+    # the planner's roofline time from its own index sets against the one computed from
+    # the written DSL's ncon steps (the oracle's per-step B M N K, sliced legs 1):
+    # sum over steps of max(8 BMNK / P, e B (MK + KN + MN) / BW), times the slice count
+    tnc = convert_to_tnc(circuits.create_rqc_circuit(3, 3, 8, seed=3))
+    groups, plan, meta = planning.contraction_scheme(tnc, nslice, trials=4, slicer="roofline",
+                                                     element_bytes=e)
+    dsl = build_compute_graph(tnc, plan, groups, metadata=meta).to_dsl()
+    P, BW = 78.6e12, 8.0e12
+    shapes = O.step_shapes(dsl)
+    assert len(shapes) == len(plan)
+    t = sum(max(8.0 * B * M * N * K / P, e * B * (M * K + K * N + M * N) / BW) for B, M, N, K in shapes)
+    nsl = len(list(O.slice_assignments(O.parse(dsl))))
+    assert nsl == 2 ** nslice
+    sliced = frozenset(g[0] for g in groups)
+    got = planning.PlanCost(tnc, plan).roofline_seconds(sliced, e)
+    assert got == pytest.approx(t * nsl, rel=1e-12)
+    assert meta["Slicing"]["Roofline seconds, all slices"] == pytest.approx(got, rel=1e-5)
+
+
+def test_roofline_slicer_picks_and_cap():
+    tnc = convert_to_tnc(circuits.create_rqc_circuit(4, 4, 10, seed=5))
+    plan = planning.contraction_plan(tnc, trials=4, seed=42)
+    pc = planning.PlanCost(tnc, plan)
+    # one pick: the narrowest largest intermediate, then the least roofline time, over
+    # every candidate index by exhaustion
+    cands = [i for i in tnc.tn.dims if tnc.tn.dims[i] > 1]
+    best = min(cands, key=lambda i: (round(pc.cost(frozenset([i]))[1], 6), pc.roofline_seconds(frozenset([i]))))
+    assert planning.roofline_slicing(tnc, plan, 1) == [best]
+    # a cap the plan already meets: each pick adds the least roofline time
+    w0 = pc.cost()[1]
+    got = planning.roofline_slicing(tnc, plan, 1, max_log2_size=w0)
+    assert got == [min(cands, key=lambda i: pc.roofline_seconds(frozenset([i])))]
+    # slicing more never lowers the total time below the unsliced plan, and the
+    # roofline picks cost no more time than the width picks at the same width
+    s_w = planning.greedy_slicing(tnc, plan, 3)
+    s_r = planning.roofline_slicing(tnc, plan, 3)
+    assert pc.cost(frozenset(s_r))[1] <= pc.cost(frozenset(s_w))[1] + 1e-9
+    assert pc.roofline_seconds(frozenset()) <= pc.roofline_seconds(frozenset(s_r)) <= \
+        pc.roofline_seconds(frozenset(s_w)) * (1 + 1e-12)
+    with pytest.raises(ValueError):
+        planning.contraction_scheme(tnc, 1, trials=1, slicer="treewidth")
+
+
 # ------------------------------------------------------------------ DSL
 def test_dsl_round_trip_and_labels():
     tnc = convert_to_tnc(circuits.create_rqc_circuit(2, 3, 6, seed=2))
