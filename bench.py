@@ -211,7 +211,7 @@ def main() -> int:
     import torch
 
     from qxhip import workloads
-    from qxhip.simulation import Executor
+    from qxhip.simulation import Executor, context
 
     if a.grad:
         print(json.dumps(grad_bench(a)), flush=True)
@@ -284,22 +284,30 @@ def main() -> int:
         "ms_per_step_eager_with_events": 1e3 * el_ev / a.steps,
         "kernels": kernels,
     }
-    if a.copy_mib:
-        # a plain copy of known size: HBM counters of this kernel calibrate the
-        # counters of the contraction kernels (bytes read = bytes written = size)
-        src = torch.ones(a.copy_mib << 18, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
+    # the device's own ceilings beside the published peaks: a plain copy of known size
+    # (its HBM counters also calibrate those of the contraction kernels: bytes read =
+    # bytes written = size) and register-only matrix-core loops
+    mib = a.copy_mib or 1024
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
         dst.copy_(src)
-        torch.cuda.synchronize()
+    e1.record()
+    torch.cuda.synchronize()
+    # read + write bytes per copy / time: the practical HBM ceiling of a stream
+    copy_gbps = 2 * (mib << 20) * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    ctx = context(0)
+    out["measured_peaks"] = {"copy_gbps": copy_gbps, "copy_mib": mib,
+                             "fp64_mfma_tflops": ctx.measure_mfma("c128"),
+                             "fp32_mfma_tflops": ctx.measure_mfma("c64_4m")}
+    if a.copy_mib:
         out["copy_calibration_bytes"] = a.copy_mib << 20
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(5):
-            dst.copy_(src)
-        e1.record()
-        torch.cuda.synchronize()
-        # read + write bytes per copy / time: the practical HBM ceiling of a stream
-        out["copy_gbps"] = 2 * (a.copy_mib << 20) * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+        out["copy_gbps"] = copy_gbps
     if not a.no_c64 and prec == "c128":
         out["c64"] = c64_comparison(dsl, data, bits, np.asarray(amps), a)
     elif not a.no_c64:

@@ -64,6 +64,13 @@ typedef struct qx_plan qx_plan; /* lowered, device-resident plan */
 qx_status qx_ctx_create(int device, qx_ctx **out);
 void qx_ctx_destroy(qx_ctx *ctx);
 
+/* Measured matrix-core rate of the context's device in TFLOP/s: a register-only loop
+ * of 16x16x4 FP64 products for QX_C128, FP32 products for the complex64 precisions
+ * (the pipes the contractions use).  No reference counterpart: a measurement of the
+ * peak that roofline fractions are quoted against (allocates and frees a small
+ * output buffer; not for use under graph capture). */
+qx_status qx_measure_mfma(qx_ctx *ctx, int precision, double *tflops);
+
 /* Parse and validate a .qx program (docs/src/users_guide.md:95-165): load, view,
  * ncon, output, save.  Replaces QXContexts' DSL parse ("Parse input files",
  * docs/src/distributed.md:97). */

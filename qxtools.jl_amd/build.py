@@ -16,8 +16,11 @@ OUT = os.path.join(HERE, "qxhip", "libqxhip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HOST_SRC = ["dsl.cpp", "lower.cpp", "runtime.cpp"]
 HIP_SRC = ["kernels.hip"]
+# -amdgpu-mfma-vgpr-form: matrix-core products accumulate in ordinary VGPRs; with
+# accumulation registers, the K loops of the matrix-core GEMM copied every loop-carried
+# accumulator between the two register files on each trip
 HIPFLAGS = ["--offload-arch=gfx950", "-std=c++17", "-fPIC", "-Xclang", "-target-feature",
-            "-Xclang", "-packed-fp32-ops"]
+            "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def _run(cmd):

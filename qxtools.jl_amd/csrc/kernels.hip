@@ -358,6 +358,30 @@ template <> struct Mfma16<float> {
   static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
+// Matrix-core rate probe: each wave runs `iters` rounds (a multiple of 16) of four
+// independent 16x16x4 products on register operands (no memory traffic in the loop;
+// 2 * 16 * 16 * 4 flops each), and each lane stores one sum at the end so that the
+// work stays live.  The loop body holds 16 rounds, so that the loop's own instructions
+// are few beside the products.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mfma_probe(T *out, int iters, T seed) {
+  typedef typename Mfma16<T>::v4 v4;
+  v4 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = v4{T(0), T(0), T(0), T(0)};
+  const T a = seed + T(threadIdx.x) * T(1e-7), b = seed - T(threadIdx.x) * T(1e-7);
+  for (int i = 0; i < iters; i += 16) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] = Mfma16<T>::mma(a, b, acc[j]);
+  }
+  T s = T(0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  out[(int64_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 // ---------------------------------------------------------------------------
 // Complex GEMM on 16x16x4 matrix cores (FP32 for complex64, FP64 for complex128),
 // same bit-weight tile gathers as k_contract_gemm.  A 64x64 complex C tile per
@@ -1838,6 +1862,15 @@ void launch_gather(int prec, const void *src, void *dst, int nb, const int64_t *
     hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const float2 *)src,
                        (float2 *)dst, n, g);
   check_launch("gather");
+}
+
+void launch_mfma_probe(int prec, void *out, int blocks, int iters, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == QX_C128)
+    hipLaunchKernelGGL(k_mfma_probe<double>, dim3(blocks), dim3(256), 0, s, (double *)out, iters, 0.5);
+  else
+    hipLaunchKernelGGL(k_mfma_probe<float>, dim3(blocks), dim3(256), 0, s, (float *)out, iters, 0.5f);
+  check_launch("mfma_probe");
 }
 
 void launch_planar_merge(const float *re, const float *im, int64_t n, void *dst, void *stream) {

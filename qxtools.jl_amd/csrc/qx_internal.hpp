@@ -369,6 +369,9 @@ void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const C
 // Output sampling: candidate bitstrings from Philox4x64-10 and the sequential
 // empirical-supremum rejection test over one round of candidates.
 void launch_gen_bits(char *bits, int64_t nb, int nch, uint64_t seed, int64_t j0, void *stream);
+// `blocks` workgroups of 256 lanes, each lane `iters` x 4 matrix-core products; writes
+// blocks * 256 values of the precision's real type to `out`
+void launch_mfma_probe(int prec, void *out, int blocks, int iters, void *stream);
 void launch_reject(const double *amps, const char *cand, int64_t nb, int nch, uint64_t seed, int64_t j0, double scale,
                    int fix_m, int64_t num_samples, double *state, int64_t *cnt, char *out_bits, double *out_amps,
                    void *stream);

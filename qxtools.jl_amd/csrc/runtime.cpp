@@ -1323,6 +1323,41 @@ qx_status qx_ctx_create(int device, qx_ctx **out) {
   });
 }
 
+qx_status qx_measure_mfma(qx_ctx *ctx, int precision, double *tflops) {
+  return guard([&] {
+    if (!ctx || !tflops) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
+    if (precision < QX_C128 || precision > QX_C64_BF16X3) throw qx::Error(QX_ERR_ARGUMENT, "unknown precision");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const int blocks = 8192, iters = 1024;   // 32 waves per CU on 256 CUs
+    const size_t bytes = size_t(blocks) * 256 * (precision == QX_C128 ? 8 : 4);
+    void *out = nullptr;
+    hip_check(hipMalloc(&out, bytes), "hipMalloc (probe)");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0;
+    hipError_t err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    try {
+      hip_check(err, "hipEventCreate");
+      qx::launch_mfma_probe(precision, out, blocks, iters, ctx->stream);   // untimed: code load
+      hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
+      qx::launch_mfma_probe(precision, out, blocks, iters, ctx->stream);
+      hip_check(hipEventRecord(e1, ctx->stream), "hipEventRecord");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    } catch (...) {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      (void)hipFree(out);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    const double flops = double(blocks) * 4 /* waves */ * iters * 4 /* products */ * 2.0 * 16 * 16 * 4;
+    *tflops = flops / (double(ms) * 1e-3) / 1e12;
+  });
+}
+
 void qx_ctx_destroy(qx_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);

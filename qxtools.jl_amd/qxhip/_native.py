@@ -20,6 +20,7 @@ SYMBOLS = [
     "qx_plan_set_option", "qx_plan_finalize", "qx_plan_info", "qx_plan_key",
     "qx_amplitudes", "qx_amplitudes_vjp", "qx_plan_stats", "qx_plan_destroy", "qx_last_error",
     "qx_dsl_inspect", "qx_dsl_inspect_ex", "qx_amplitudes_uniform", "qx_sample_rejection",
+    "qx_measure_mfma",
 ]
 
 
@@ -59,6 +60,7 @@ def lib() -> C.CDLL:
         L.qx_amplitudes_uniform.argtypes = [p, i64, C.c_uint64, C.c_char_p, dp]
         L.qx_sample_rejection.argtypes = [p, i64, C.c_uint64, C.c_double, C.c_int, i64, C.c_char_p, dp, dp,
                                           C.POINTER(i64)]
+        L.qx_measure_mfma.argtypes = [p, C.c_int, dp]
         L.qx_last_error.argtypes = []
         L.qx_last_error.restype = C.c_char_p
         for f in SYMBOLS:
@@ -99,6 +101,13 @@ class Context:
     def __init__(self, device: int = 0):
         self._h = C.c_void_p()
         _check(lib().qx_ctx_create(device, C.byref(self._h)))
+
+    def measure_mfma(self, precision: str = "c128") -> float:
+        """Measured matrix-core TFLOP/s of the device: FP64 products for "c128",
+        FP32 for the complex64 precisions (`qx_measure_mfma`)."""
+        t = C.c_double()
+        _check(lib().qx_measure_mfma(self._h, PRECISIONS[precision], C.byref(t)))
+        return t.value
 
     def close(self):
         if self._h:
