@@ -6,7 +6,7 @@ A step is one `qx_amplitudes` call over all 256 bitstrings, the plan's tensors
 already resident in HBM.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--precision c128|c64_4m|c64_3m] [--cpu-seconds S | --no-cpu]
+                    [--precision c128|c64_4m|c64_3m] [--cpu-seconds S [--cpu-threads T] | --no-cpu]
 """
 
 from __future__ import annotations
@@ -58,14 +58,15 @@ def plan_roofline(stats: dict, prec: str, ms_per_step: float, steps: int) -> flo
     return t / (ms_per_step * 1e-3)
 
 
-def cpu_baseline(dsl: str, data: dict, bits: list, seconds: float) -> dict:
+def cpu_baseline(dsl: str, data: dict, bits: list, seconds: float, threads: int = 0) -> dict:
     """The oracle (numpy TTGT, OpenBLAS zgemm) contracting whole amplitudes one
-    bitstring at a time, as the reference's per-bitstring loop does."""
+    bitstring at a time, as the reference's per-bitstring loop does, on `threads`
+    BLAS threads (0: the host's cores, at most 16)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import qxoracle
     from threadpoolctl import threadpool_limits
 
-    cores = min(os.cpu_count() or 1, 16)
+    cores = threads if threads > 0 else min(os.cpu_count() or 1, 16)
     done = 0
     t0 = time.perf_counter()
     with threadpool_limits(limits=cores, user_api="blas"):
@@ -192,6 +193,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="BLAS threads of the CPU baseline (default: the host's cores, at most 16)")
     ap.add_argument("--graph", type=int, default=1, help="replay each call from a captured hipGraph")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64 comparison")
     ap.add_argument("--grad", action="store_true", help="gradient / forward time of the parameterised workload")
@@ -315,7 +318,7 @@ def main() -> int:
         out["error_vs_c128"] = c64_comparison(dsl, data, bits, ref, a, precs=(prec,))[prec]
         out["error_vs_c128"].pop("value"), out["error_vs_c128"].pop("ms_per_step")
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(dsl, data, bits, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(dsl, data, bits, a.cpu_seconds, a.cpu_threads)
     _ = np.asarray(amps)
     print(json.dumps(out), flush=True)
     return 0

@@ -892,6 +892,75 @@ __global__ void k_gather(const typename Vec<T>::type *__restrict__ src, typename
   }
 }
 
+// Bit-permutation copy through LDS: tile bits are the destination's lowest bits and
+// the bits that are the source's lowest (source weights 1, 2, 4, ...).  A tile is
+// read with lanes along the source's low bits and written with lanes along the
+// destination's low bits; the other bits number the tiles.
+struct GatherTileArgs {
+  int nt, nr;                  // tile bits, other bits
+  int tdst[12];                // destination bit of tile bit q (ascending)
+  int lbit[12];                // load-order bit j -> tile bit
+  int64_t tw[12];              // source weight of tile bit q
+  int rdst[kMaxBits];          // destination bit of other bit r
+  int64_t rw[kMaxBits];        // source weight of other bit r
+};
+
+template <typename T, int NT>
+__global__ __launch_bounds__(256) void k_gather_tiled(const typename Vec<T>::type *__restrict__ src,
+                                                      typename Vec<T>::type *__restrict__ dst, int64_t ntiles,
+                                                      GatherTileArgs g) {
+  using V2 = typename Vec<T>::type;
+  constexpr int E = (1 << NT) / 256;   // elements per lane per tile
+  __shared__ V2 tile[(1 << NT) + (1 << NT) / 32];
+  // tile element k = threadIdx.x + 256 e: the offsets of the lane's bits (low 8) and of
+  // e's bits (high NT - 8), the same for every tile
+  const int tid = threadIdx.x;
+  int64_t so_lo = 0, d_lo = 0;
+  int loc_lo = 0;
+  for (int j = 0; j < 8; ++j)
+    if ((tid >> j) & 1) {
+      const int q = g.lbit[j];
+      loc_lo |= 1 << q;
+      so_lo += g.tw[q];
+    }
+  for (int q = 0; q < 8; ++q)
+    if ((tid >> q) & 1) d_lo += int64_t(1) << g.tdst[q];
+  int64_t so_hi[E], d_hi[E];
+  int loc_hi[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    so_hi[e] = 0, d_hi[e] = 0, loc_hi[e] = 0;
+#pragma unroll
+    for (int j = 8; j < NT; ++j)
+      if ((e >> (j - 8)) & 1) {
+        const int q = g.lbit[j];
+        loc_hi[e] |= 1 << q;
+        so_hi[e] += g.tw[q];
+        d_hi[e] += int64_t(1) << g.tdst[j];
+      }
+  }
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int64_t sb = 0, db = 0;
+    for (int r = 0; r < g.nr; ++r)
+      if ((t >> r) & 1) sb += g.rw[r], db += int64_t(1) << g.rdst[r];
+    V2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = src[sb + so_lo + so_hi[e]];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int loc = loc_lo | loc_hi[e];
+      tile[loc + (loc >> 5)] = v[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = tid + 256 * e;
+      dst[db + d_lo + d_hi[e]] = tile[k + (k >> 5)];
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_planar_merge(const float *__restrict__ re, const float *__restrict__ im, int64_t n,
                                float2 *__restrict__ dst) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1848,8 +1917,73 @@ void launch_bf16_split(const void *src, int64_t n, void *out, void *stream) {
   check_launch("bf16_split");
 }
 
+// Tile bits of the LDS gather: alternately the next destination bit and the bit of
+// the next source weight (1, 2, 4, ...) while it exists, until `cap` bits.
+bool gather_tiles(int nb, const int64_t *w, int cap, GatherTileArgs &g) {
+  bool in_t[kMaxBits] = {};
+  std::vector<int> src_order;   // tile bits that are the source's low bits, in source order
+  int nt = 0, a = 0, b = 0;
+  bool more_b = true;
+  while (nt < cap && (a < nb || more_b)) {
+    if (a < nb) {
+      if (!in_t[a]) in_t[a] = true, ++nt;
+      ++a;
+    }
+    if (nt < cap && more_b) {
+      int p = -1;
+      for (int q = 0; q < nb; ++q)
+        if (w[q] == (int64_t(1) << b)) p = q;
+      if (p < 0) {
+        more_b = false;
+      } else {
+        if (!in_t[p]) in_t[p] = true, ++nt;
+        src_order.push_back(p);
+        ++b;
+      }
+    }
+  }
+  g = GatherTileArgs{};
+  int pos_of[kMaxBits];
+  for (int q = 0; q < nb; ++q) {
+    if (in_t[q]) {
+      pos_of[q] = g.nt;
+      g.tdst[g.nt] = q;
+      g.tw[g.nt] = w[q];
+      ++g.nt;
+    } else {
+      g.rdst[g.nr] = q;
+      g.rw[g.nr] = w[q];
+      ++g.nr;
+    }
+  }
+  int j = 0;
+  bool used[12] = {};
+  for (int p : src_order) g.lbit[j++] = pos_of[p], used[pos_of[p]] = true;
+  for (int q = 0; q < g.nt; ++q)
+    if (!used[q]) g.lbit[j++] = q;
+  return g.nt == nt && j == g.nt && g.nt >= 6;
+}
+
 void launch_gather(int prec, const void *src, void *dst, int nb, const int64_t *w, void *stream) {
   if (nb < 0 || nb > kMaxBits) throw Error(QX_ERR_STATE, "internal: gather bits");
+  hipStream_t st = (hipStream_t)stream;
+  if (nb >= 10) {
+    // tiles of 2^12 complex64 or 2^11 complex128 elements: 33 KiB of LDS with padding
+    const int cap = prec == QX_C128 ? 11 : 12;
+    GatherTileArgs t;
+    if (gather_tiles(nb, w, cap, t) && t.nt == cap) {
+      const int64_t ntiles = int64_t(1) << t.nr;
+      const int grid = (int)std::min<int64_t>(ntiles, 1 << 16);
+      if (prec == QX_C128)
+        hipLaunchKernelGGL((k_gather_tiled<double, 11>), dim3(grid), dim3(256), 0, st, (const double2 *)src,
+                           (double2 *)dst, ntiles, t);
+      else
+        hipLaunchKernelGGL((k_gather_tiled<float, 12>), dim3(grid), dim3(256), 0, st, (const float2 *)src,
+                           (float2 *)dst, ntiles, t);
+      check_launch("gather_tiled");
+      return;
+    }
+  }
   GatherArgs g;
   g.nb = nb;
   for (int b = 0; b < nb; ++b) g.w[b] = w[b];
