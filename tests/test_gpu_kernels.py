@@ -216,6 +216,26 @@ def test_rocblas_step_in_graph_replays():
     ex.close()
 
 
+def test_split_bf16_product_in_graph_replays():
+    """The split bf16 product (library products of the hi / lo planes) captured into
+    the call's graph: every replay, with eager calls of another plan on the device
+    between the replays, equals the eager result bit for bit."""
+    import qxtest
+    dsl, data, ref = qxtest.interleaved_gemm_program(nk=10, nm=10, nn=10, seed=6)
+    eager = Executor(dsl, data, precision="c64_bf16x3")
+    want = np.asarray(eager.amplitudes([""])).reshape(-1)
+    graphed = Executor(dsl, data, precision="c64_bf16x3", graph=1)
+    other_dsl, other_data, other_ref = plain_step(78, 1, 7, 6, 8, 1, 0, 1)
+    other = Executor(other_dsl, other_data, kernel=8)
+    for rep in range(3):
+        got = np.asarray(graphed.amplitudes([""])).reshape(-1)
+        assert np.array_equal(got, want), f"replay {rep}"
+        assert err(np.asarray(other.amplitudes([""])).reshape(-1), other_ref) < 1e-12
+    assert err(want, ref) < 1e-4
+    for ex in (eager, graphed, other):
+        ex.close()
+
+
 @pytest.mark.parametrize("view,prec,a_data", [(False, "c128", False), (True, "c128", False), (False, "c64_4m", False),
                                               (False, "c128", True)])
 def test_regrouped_contraction_matches_oracle(view, prec, a_data):
