@@ -1655,6 +1655,40 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
 // steps of one launch are independent.
 constexpr int kSmallThreads = 256;
 
+// Sum over k = k0, k0 + dk, ... < nk of A[oa + offset_a(k)] * B[ob + offset_b(k)] for a
+// small-step descriptor, in that order: the operands of 8 terms are loaded before
+// they are used, so that their loads are in flight together.
+template <typename T>
+__device__ __forceinline__ typename Vec<T>::type small_dot(const SmallDesc &s, const typename Vec<T>::type *A,
+                                                           const typename Vec<T>::type *B, int64_t oa, int64_t ob,
+                                                           int k0, int dk, int nk) {
+  using V = typename Vec<T>::type;
+  constexpr int U = 8;
+  V acc{0, 0};
+  int k = k0;
+  for (; k + (U - 1) * dk < nk; k += U * dk) {
+    V av[U], bv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int kk = k + u * dk;
+      int64_t xa = oa, xb = ob;
+      for (int b = 0; b < s.nbk; ++b)
+        if ((kk >> b) & 1) xa += s.ka[b], xb += s.kb[b];
+      av[u] = A[xa];
+      bv[u] = B[xb];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) cmac<T>(acc, av[u], bv[u], s.conj_a, s.conj_b);
+  }
+  for (; k < nk; k += dk) {
+    int64_t xa = oa, xb = ob;
+    for (int b = 0; b < s.nbk; ++b)
+      if ((k >> b) & 1) xa += s.ka[b], xb += s.kb[b];
+    cmac<T>(acc, A[xa], B[xb], s.conj_a, s.conj_b);
+  }
+  return acc;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *__restrict__ d) {
   using V = typename Vec<T>::type;
@@ -1670,13 +1704,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *
     int64_t oa = 0, ob = 0;
     for (int b = 0; b < s.nbc; ++b)
       if ((c >> b) & 1) oa += s.wa[b], ob += s.wb[b];
-    V acc{0, 0};
-    for (int k = sub; k < nk; k += 1 << lt) {
-      int64_t xa = oa, xb = ob;
-      for (int b = 0; b < s.nbk; ++b)
-        if ((k >> b) & 1) xa += s.ka[b], xb += s.kb[b];
-      cmac<T>(acc, A[xa], B[xb], s.conj_a, s.conj_b);
-    }
+    V acc = small_dot<T>(s, A, B, oa, ob, sub, 1 << lt, nk);
     part[threadIdx.x] = acc;
     __syncthreads();
     if (sub == 0) {
@@ -1693,14 +1721,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *
     int64_t oa = 0, ob = 0;
     for (int b = 0; b < s.nbc; ++b)
       if ((c >> b) & 1) oa += s.wa[b], ob += s.wb[b];
-    V acc{0, 0};
-    for (int k = 0; k < nk; ++k) {
-      int64_t xa = oa, xb = ob;
-      for (int b = 0; b < s.nbk; ++b)
-        if ((k >> b) & 1) xa += s.ka[b], xb += s.kb[b];
-      cmac<T>(acc, A[xa], B[xb], s.conj_a, s.conj_b);
-    }
-    C[c] = acc;
+    C[c] = small_dot<T>(s, A, B, oa, ob, 0, 1, nk);
   }
 }
 
