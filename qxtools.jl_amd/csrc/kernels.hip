@@ -1650,10 +1650,9 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
 
 // ---------------------------------------------------------------------------
 // Small-step batch: workgroup i contracts C elements [c0, c1) of the step of
-// descriptor i (lanes along C, the K loop in registers; a C smaller than the
-// workgroup splits each element's K range over 256 / |C| lanes, summed in LDS).  The
-// steps of one launch are independent.
-constexpr int kSmallThreads = 256;
+// descriptor i (lanes along C, the K loop in registers; a range shorter than the
+// workgroup splits each element's K range over 256 / (c1 - c0) lanes, summed in
+// LDS).  The steps of one launch are independent.
 
 // Sum over k = k0, k0 + dk, ... < nk of A[oa + offset_a(k)] * B[ob + offset_b(k)] for a
 // small-step descriptor, in that order: the operands of 8 terms are loaded before
@@ -1696,11 +1695,12 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *
   const V *A = (const V *)s.A;
   const V *B = (const V *)s.B;
   V *C = (V *)s.C;
-  const int nc = 1 << s.nbc, nk = 1 << s.nbk;
-  if (nc < kSmallThreads && nk > 1) {   // (one workgroup holds the whole C here)
+  const int nk = 1 << s.nbk;
+  const int len = s.c1 - s.c0;   // a power of two
+  if (len < kSmallThreads && nk > 1) {   // lanes split K: 2^lt lanes per element of the chunk
     __shared__ V part[kSmallThreads];
-    const int lt = 8 - s.nbc;   // log2 of the lanes per element (kSmallThreads = 2^8)
-    const int c = threadIdx.x >> lt, sub = threadIdx.x & ((1 << lt) - 1);
+    const int lt = 8 - (__ffs(len) - 1);   // kSmallThreads = 2^8
+    const int c = s.c0 + (threadIdx.x >> lt), sub = threadIdx.x & ((1 << lt) - 1);
     int64_t oa = 0, ob = 0;
     for (int b = 0; b < s.nbc; ++b)
       if ((c >> b) & 1) oa += s.wa[b], ob += s.wb[b];

@@ -114,7 +114,8 @@ struct RadixArgs {
 // Small slice-free steps (gate-block building) run many per launch: one workgroup
 // per step, its bit weights read from a device table of these descriptors.
 constexpr int kSmallMaxBits = 18;
-constexpr int kSmallChunk = 1024;   // C elements per workgroup of a small-step batch
+constexpr int kSmallChunk = 1024;   // most C elements per workgroup of a small-step batch
+constexpr int kSmallThreads = 256;  // lanes per workgroup of a small-step batch
 struct SmallDesc {
   const void *A = nullptr, *B = nullptr;
   void *C = nullptr;

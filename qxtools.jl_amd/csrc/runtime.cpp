@@ -351,11 +351,17 @@ void launch_step(qx_plan *P, const qx::Lowered &L, const qx::Step &st, void *A, 
 // Append x to a small-step descriptor table, its C split into workgroup ranges of
 // kSmallChunk elements; returns the number of descriptors added.
 int push_small(std::vector<qx::SmallDesc> &d, qx::SmallDesc x) {
+  // chunks of C (powers of two) sized for about 32 multiply-adds per lane of the
+  // workgroup: a chunk of fewer elements than lanes splits K over its lanes
   const int nc = 1 << x.nbc;
+  const int64_t nk = int64_t(1) << x.nbk;
+  int len = qx::kSmallChunk;
+  while (len > 1 && len * nk > 32 * qx::kSmallThreads) len >>= 1;
+  len = std::min(len, nc);
   int n = 0;
-  for (int c0 = 0; c0 < nc; c0 += qx::kSmallChunk, ++n) {
+  for (int c0 = 0; c0 < nc; c0 += len, ++n) {
     x.c0 = c0;
-    x.c1 = std::min(nc, c0 + qx::kSmallChunk);
+    x.c1 = c0 + len;
     d.push_back(x);
   }
   return n;
