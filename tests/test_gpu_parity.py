@@ -1,7 +1,7 @@
 """GPU parity: libqxhip (HIP, gfx950) against the CPU FP64 oracle and closed forms.
 
 Bar: plain complex128 path within 1e-12 relative per amplitude of the oracle
-(north-star tolerance for the FP64 path); bookkeeping (slices, leg orders,
+(the FP64 path's tolerance); bookkeeping (slices, leg orders,
 output fixing) checked through exact closed forms and the slice-sum identity."""
 
 import math
