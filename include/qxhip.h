@@ -171,6 +171,14 @@ qx_status qx_sample_rejection(qx_plan *plan, int64_t num_samples, uint64_t seed,
 qx_status qx_amplitudes_vjp(qx_plan *plan, const char *bits, int64_t nb, const double *amp_cot,
                             double *grad);
 
+/* The loss sum over bitstrings and result elements of |amp|^2, in one pass: writes the
+ * nb * result_elems amplitudes to `amps` (interleaved) and the loss's gradient per
+ * data key to `grad` (layout of qx_amplitudes_vjp), forming the cotangent 2 amp on the
+ * device from the same pass's forward.  QX_ERR_UNSUPPORTED when slice variables would
+ * loop (the full amplitudes are then known only after every slice): use
+ * qx_amplitudes + qx_amplitudes_vjp.  No reference counterpart (gradients are new). */
+qx_status qx_norm_vjp(qx_plan *plan, const char *bits, int64_t nb, double *amps, double *grad);
+
 /* JSON with per-step shapes (B, M, N, K), algorithmic flops and bytes, kernel
  * choice and, when "profile" is on, the accumulated time of every step. */
 qx_status qx_plan_stats(const qx_plan *plan, char *json, size_t cap);

@@ -755,6 +755,31 @@ __global__ void k_add_mapped(typename Vec<TD>::type *__restrict__ dst, const typ
   }
 }
 
+// Cotangent of the loss sum_b sum_e |amp_{b,e}|^2 at the root: 2 amp, read from the
+// root's forward value of the batch (root value and cotangent share one layout); the
+// batch's amplitudes are also written out in FP64 (rows past nvalid get a zero
+// cotangent).
+template <typename T>
+__global__ void k_root_cot_norm(typename Vec<T>::type *__restrict__ dst, const typename Vec<T>::type *__restrict__ val,
+                                int64_t dep, int64_t batch, int64_t nvalid, int64_t relems,
+                                const int64_t *__restrict__ relem_off, double *__restrict__ amp, int64_t row0) {
+  using V = typename Vec<T>::type;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < batch * relems;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t beta = t % batch, e = t / batch;
+    const int64_t o = relem_off[e] + beta * dep;
+    V c{0, 0};
+    if (beta < nvalid) {
+      const V v = val[o];
+      c = V{v.x + v.x, v.y + v.y};
+      double *p = amp + 2 * ((row0 + beta) * relems + e);
+      p[0] = (double)v.x;
+      p[1] = (double)v.y;
+    }
+    dst[o] = c;
+  }
+}
+
 template <typename T>
 __global__ void k_root_cot(typename Vec<T>::type *__restrict__ dst, int64_t dep, int64_t batch, int64_t nvalid,
                            int64_t relems, const int64_t *__restrict__ relem_off, const double *__restrict__ amp,
@@ -1911,6 +1936,19 @@ void launch_add_mapped(int prec, int dst_double, void *dst, const void *src, int
     hipLaunchKernelGGL((k_add_mapped<float, float>), dim3(g), dim3(256), 0, s, (float2 *)dst, (const float2 *)src,
                        n, a);
   check_launch("add_mapped");
+}
+
+void launch_root_cot_norm(int prec, void *dst, const void *val, int64_t dep, int64_t batch, int64_t nvalid,
+                          int64_t relems, const int64_t *d_relem_off, double *amp, int64_t row0, void *stream) {
+  const int g = grid_for(batch * relems, 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == QX_C128)
+    hipLaunchKernelGGL(k_root_cot_norm<double>, dim3(g), dim3(256), 0, s, (double2 *)dst, (const double2 *)val, dep,
+                       batch, nvalid, relems, d_relem_off, amp, row0);
+  else
+    hipLaunchKernelGGL(k_root_cot_norm<float>, dim3(g), dim3(256), 0, s, (float2 *)dst, (const float2 *)val, dep,
+                       batch, nvalid, relems, d_relem_off, amp, row0);
+  check_launch("root_cot_norm");
 }
 
 void launch_root_cot(int prec, void *dst, int64_t dep, int64_t batch, int64_t nvalid, int64_t relems,

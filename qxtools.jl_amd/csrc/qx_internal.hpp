@@ -354,6 +354,9 @@ void launch_add_mapped(int prec, int dst_double, void *dst, const void *src, int
 // cotangent of the saved tensor from amp_cot rows [row0, row0 + nvalid)
 void launch_root_cot(int prec, void *dst, int64_t dep, int64_t batch, int64_t nvalid, int64_t relems,
                      const int64_t *d_relem_off, const double *amp, int64_t row0, void *stream);
+// root cotangent 2 * root value (the loss sum |amp|^2); amplitudes of the batch into `amp`
+void launch_root_cot_norm(int prec, void *dst, const void *val, int64_t dep, int64_t batch, int64_t nvalid,
+                          int64_t relems, const int64_t *d_relem_off, double *amp, int64_t row0, void *stream);
 void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream);
 void launch_add_batch(int prec, const AddDesc *d_desc, int n, void *stream);
 // bf16 split of n complex64 values: out holds four planar n-element bf16 arrays

@@ -1006,10 +1006,15 @@ const qx::VjpLowered &ensure_vjp(qx_plan *P, int64_t batch, uint64_t fused) {
   return P->vjp_lowered.at(key);
 }
 
-void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double *grad) {
+// The pullback of the amplitudes of nb bitstrings for the cotangent `amp_cot`, or, with
+// `amp_cot` null, of the loss sum |amp|^2 (cotangent 2 amp formed on the device from
+// the same pass's forward; its amplitudes go to `amps_out`; needs every slice variable
+// contracted inline, else QX_ERR_UNSUPPORTED).
+void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double *grad, double *amps_out = nullptr) {
   if (!P->finalized) throw qx::Error(QX_ERR_STATE, "plan is not finalized");
   if (nb <= 0) throw qx::Error(QX_ERR_ARGUMENT, "need at least one bitstring");
-  if (!bits || !amp_cot || !grad) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
+  if (!bits || !grad || (!amp_cot && !amps_out)) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
+  const bool norm = amp_cot == nullptr;
   check_bits(P, bits, nb);
   int64_t batch = 1;
   while (batch < nb && batch < P->opt_batch) batch <<= 1;
@@ -1032,7 +1037,8 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
   }
   moved |= P->grad.reserve(std::max<size_t>((size_t)ke * 16, 16));
   moved |= P->ampcot.reserve((size_t)nb * R * 16);
-  hip_check(hipMemcpyAsync(P->ampcot.p, amp_cot, (size_t)nb * R * 16, hipMemcpyHostToDevice, stream), "copy");
+  if (!norm)
+    hip_check(hipMemcpyAsync(P->ampcot.p, amp_cot, (size_t)nb * R * 16, hipMemcpyHostToDevice, stream), "copy");
   size_t nbits = (size_t)nb * nch;
   moved |= P->bits.reserve(std::max<size_t>(nbits, 16));
   if (nbits) hip_check(hipMemcpyAsync(P->bits.p, bits, nbits, hipMemcpyHostToDevice, stream), "copy");
@@ -1050,9 +1056,12 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
     hip_check(hipMemcpyAsync(P->out_q.p, oq.data(), oq.size() * 4, hipMemcpyHostToDevice, stream), "copy");
   }
   const qx::Value &rc = V.cot[V.root];
+  const qx::Value &rv = L.values[P->prog.save];
+  if (norm && (looped_slices(P, fused) != 1 || !rc.dep || rc.beta_stride != rv.beta_stride))
+    throw qx::Error(QX_ERR_UNSUPPORTED, "the norm pullback needs every slice variable contracted inline and a "
+                                        "bitstring-dependent root");
   std::vector<int64_t> roff(R);
   {
-    const qx::Value &rv = L.values[P->prog.save];
     for (int64_t e = 0; e < R; ++e) {
       int64_t r = e, o = 0;
       for (size_t j = 0; j < rv.dims.size(); ++j) {
@@ -1224,8 +1233,13 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
       qx::launch_fill_outputs(P->prec, bs.arena, (const int64_t *)P->out_off.p, nout, (const int *)P->out_q.p,
                               (const char *)P->bits.p + row0 * nch, nvalid, batch, nch, stream);
       run_fwd(2);
-      qx::launch_root_cot(P->prec, value_ptr(P, rc, varval, bs), rc.dep ? rc.beta_stride : 0, batch, nvalid, R,
-                          (const int64_t *)P->relem_off.p, (const double *)P->ampcot.p, row0, stream);
+      if (norm)
+        qx::launch_root_cot_norm(P->prec, value_ptr(P, rc, varval, bs), value_ptr(P, rv, varval, bs), rc.beta_stride,
+                                 batch, nvalid, R, (const int64_t *)P->relem_off.p, (double *)P->ampcot.p, row0,
+                                 stream);
+      else
+        qx::launch_root_cot(P->prec, value_ptr(P, rc, varval, bs), rc.dep ? rc.beta_stride : 0, batch, nvalid, R,
+                            (const int64_t *)P->relem_off.p, (const double *)P->ampcot.p, row0, stream);
       run_bwd(2);
     }
     run_bwd(1);
@@ -1241,7 +1255,7 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
                                       nrows * (section_launches(L, 2) + bwd[2] + 3));
   P->last_launches = launches;
   if (P->opt_graph && !P->opt_profile && launches <= kMaxGraphLaunches) {
-    auto key = std::make_tuple(batch, nb, int64_t(-1), (int64_t)fused);
+    auto key = std::make_tuple(batch, nb, int64_t(norm ? -2 : -1), (int64_t)fused);
     auto it = P->vjp_graphs.find(key);
     if (it == P->vjp_graphs.end()) {
       hipGraph_t g = nullptr;
@@ -1266,6 +1280,8 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
     enqueue();
   }
   hip_check(hipMemcpyAsync(grad, P->grad.p, (size_t)ke * 16, hipMemcpyDeviceToHost, stream), "copy out");
+  if (norm)
+    hip_check(hipMemcpyAsync(amps_out, P->ampcot.p, (size_t)nb * R * 16, hipMemcpyDeviceToHost, stream), "copy out");
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
@@ -1532,6 +1548,15 @@ qx_status qx_amplitudes_vjp(qx_plan *plan, const char *bits, int64_t nb, const d
     if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
     hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
     vjp(plan, bits, nb, amp_cot, grad);
+  });
+}
+
+qx_status qx_norm_vjp(qx_plan *plan, const char *bits, int64_t nb, double *amps, double *grad) {
+  return guard([&] {
+    if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
+    if (!amps) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
+    hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+    vjp(plan, bits, nb, nullptr, grad, amps);
   });
 }
 

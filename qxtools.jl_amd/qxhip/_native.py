@@ -15,12 +15,14 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqxhip.so
 
 PRECISIONS = {"c128": 0, "c64_4m": 1, "c64_3m": 2, "c64_bf16x3": 3}
 
+QX_ERR_UNSUPPORTED = -8   # valid program outside what the library executes (include/qxhip.h)
+
 SYMBOLS = [
     "qx_ctx_create", "qx_ctx_destroy", "qx_plan_from_dsl", "qx_plan_set_tensor",
     "qx_plan_set_option", "qx_plan_finalize", "qx_plan_info", "qx_plan_key",
     "qx_amplitudes", "qx_amplitudes_vjp", "qx_plan_stats", "qx_plan_destroy", "qx_last_error",
     "qx_dsl_inspect", "qx_dsl_inspect_ex", "qx_amplitudes_uniform", "qx_sample_rejection",
-    "qx_measure_mfma",
+    "qx_measure_mfma", "qx_norm_vjp",
 ]
 
 
@@ -61,6 +63,7 @@ def lib() -> C.CDLL:
         L.qx_sample_rejection.argtypes = [p, i64, C.c_uint64, C.c_double, C.c_int, i64, C.c_char_p, dp, dp,
                                           C.POINTER(i64)]
         L.qx_measure_mfma.argtypes = [p, C.c_int, dp]
+        L.qx_norm_vjp.argtypes = [p, C.c_char_p, i64, dp, dp]
         L.qx_last_error.argtypes = []
         L.qx_last_error.restype = C.c_char_p
         for f in SYMBOLS:
@@ -224,6 +227,22 @@ class Plan:
             out[k] = grad[o:o + n]
             o += n
         return out
+
+    def norm_vjp(self, bitstrings) -> tuple:
+        """(amplitudes, {key: gradient}) of the loss sum |amp|^2 in one pass
+        (`qx_norm_vjp`); QxError with status QX_ERR_UNSUPPORTED when slices loop."""
+        bitstrings = list(bitstrings)
+        nb = len(bitstrings)
+        sizes = self.key_sizes()
+        grad = np.zeros(sum(n for _, n in sizes), dtype=np.complex128)
+        amps = np.zeros(nb * self.result_elems, dtype=np.complex128)
+        _check(lib().qx_norm_vjp(self._h, self._bits(bitstrings), nb, _dptr(amps.view(np.float64)),
+                                 _dptr(grad.view(np.float64))))
+        out, o = {}, 0
+        for k, n in sizes:
+            out[k] = grad[o:o + n]
+            o += n
+        return (amps if self.result_elems == 1 else amps.reshape(nb, self.result_elems)), out
 
     def set_option(self, name: str, value: int) -> None:
         _check(lib().qx_plan_set_option(self._h, name.encode(), int(value)))

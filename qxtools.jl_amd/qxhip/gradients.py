@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from . import _native
 from .circuits import Circuit, gate_matrix_derivative, gate_qubits
 from .compute_graph import build_compute_graph
 from .planning import contraction_scheme, sweep_contraction_plan
@@ -74,8 +75,15 @@ class ParamCircuit:
         return out
 
     def loss_and_grad(self, bits) -> tuple:
-        """L = sum_b |amp_b|^2 and dL/dtheta."""
-        amps = self.amplitudes(bits)
+        """L = sum_b |amp_b|^2 and dL/dtheta: one pass of forward and pullback when
+        every slice variable is contracted inline, else the amplitudes first and the
+        pullback of the cotangent 2 amp after."""
+        try:
+            amps, kg = self.executor.plan.norm_vjp(bits)
+        except _native.QxError as e:
+            if e.status != _native.QX_ERR_UNSUPPORTED:
+                raise
+            amps = self.amplitudes(bits)
+            kg = self.executor.vjp(bits, 2.0 * amps)
         loss = float(np.sum(np.abs(amps) ** 2))
-        kg = self.executor.vjp(bits, 2.0 * amps)
         return loss, self.param_grad(kg), amps

@@ -84,3 +84,53 @@ def test_param_gradient_c64():
     g64 = gradients.ParamCircuit(c, precision="c128").loss_and_grad(bits)[1]
     g32 = gradients.ParamCircuit(c, precision="c64_4m").loss_and_grad(bits)[1]
     assert _nerr(g32, g64) < 1e-4
+
+
+@pytest.mark.parametrize("planner,nslice,nb,batch,prec", [("sweep", 0, 5, 2, "c128"), ("greedy", 2, 6, 4, "c128"),
+                                                          ("greedy", 2, 4, 4, "c64_4m")])
+def test_norm_vjp_equals_amplitudes_then_vjp(planner, nslice, nb, batch, prec):
+    """qx_norm_vjp (the loss sum |amp|^2 in one pass, cotangent 2 amp formed on the
+    device) against qx_amplitudes followed by qx_amplitudes_vjp of 2 amp, eager and
+    replayed from its graph, with other calls between the replays; the gradient also
+    against the oracle's environment tensors."""
+    circ = circuits.create_rqc_circuit(2, 3, 5, seed=12)
+    t, plan, g = qxtest.graph_for(circ, planner=planner, nslice=nslice)
+    dsl = g.to_dsl()
+    bits = circuits_bits = ["010110", "111000", "+-0101", "000000", "101011", "0110-+"][:nb]
+    tol = 1e-11 if prec == "c128" else 1e-4
+    for graph in (0, 1):
+        ex = Executor(dsl, g.data, batch=batch, precision=prec, graph=graph)
+        amps_ref = ex.amplitudes(bits)
+        grad_ref = ex.vjp(bits, 2.0 * amps_ref)
+        for rep in range(2):
+            amps, grad = ex.plan.norm_vjp(bits)
+            assert _nerr(amps, amps_ref) < (1e-12 if prec == "c128" else 1e-5), (graph, rep)
+            for k in grad_ref:
+                assert _nerr(grad[k], grad_ref[k]) < tol, (graph, rep, k)
+            _ = ex.amplitudes(circuits_bits[:1])
+        if prec == "c128" and graph == 0:
+            ref = O.env_gradient(dsl, g.data, bits, 2.0 * O.amplitudes(dsl, g.data, bits))
+            for k in ref:
+                assert _nerr(grad[k], ref[k]) < 1e-11, k
+        ex.close()
+
+
+def test_norm_vjp_refuses_looped_slices():
+    """With slice variables looped (fuse_slices 0) the full amplitudes are known only
+    after every slice: the norm pullback reports QX_ERR_UNSUPPORTED and the parameter
+    gradient falls back to amplitudes followed by the pullback, with the same result."""
+    from qxhip import _native
+    circ = circuits.create_rqc_circuit(2, 2, 5, seed=4)
+    t, plan, g = qxtest.graph_for(circ, planner="greedy", nslice=2)
+    dsl = g.to_dsl()
+    bits = ["0110", "1001"]
+    ex = Executor(dsl, g.data, batch=2, fuse_slices=0)
+    with pytest.raises(_native.QxError) as e:
+        ex.plan.norm_vjp(bits)
+    assert e.value.status == _native.QX_ERR_UNSUPPORTED
+    amps = ex.amplitudes(bits)
+    ref = O.env_gradient(dsl, g.data, bits, 2.0 * amps)
+    grad = ex.vjp(bits, 2.0 * amps)
+    for k in ref:
+        assert _nerr(grad[k], ref[k]) < 1e-11, k
+    ex.close()
