@@ -153,12 +153,18 @@ class Plan:
         return out
 
     def key_sizes(self) -> list:
+        """[(data key, elements)] in the gradient layout; fixed once the program is
+        parsed, so queried once."""
+        cached = getattr(self, "_key_sizes", None)
+        if cached is not None:
+            return list(cached)
         out = []
         buf = C.create_string_buffer(512)
         ne = C.c_int64()
         for i in range(len(self.keys_needed())):
             _check(lib().qx_plan_key(self._h, i, buf, 512, C.byref(ne)))
             out.append((buf.value.decode(), ne.value))
+        self._key_sizes = tuple(out)
         return out
 
     def info(self) -> tuple:
