@@ -58,6 +58,7 @@ qx_status guard(F &&f) {
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  std::vector<char> held;   // the bytes of the last upload(), while the buffer stays
   // returns true when the buffer moved (captured graphs that use it are stale)
   bool reserve(size_t n) {
     if (n <= bytes) return false;
@@ -75,6 +76,14 @@ struct DevBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    held.clear();
+  }
+  // Copy n bytes from host memory in stream order, unless the buffer already holds
+  // exactly them from the previous upload (small per-call tables).
+  void upload(const void *src, size_t n, hipStream_t s) {
+    if (held.size() == n && n > 0 && std::memcmp(held.data(), src, n) == 0) return;
+    hip_check(hipMemcpyAsync(p, src, n, hipMemcpyHostToDevice, s), "copy");
+    held.assign((const char *)src, (const char *)src + n);
   }
 };
 
@@ -684,8 +693,8 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   moved |= P->out_off.reserve(std::max<size_t>(ooff.size() * 8, 16));
   moved |= P->out_q.reserve(std::max<size_t>(oq.size() * 4, 16));
   if (nout) {
-    hip_check(hipMemcpyAsync(P->out_off.p, ooff.data(), ooff.size() * 8, hipMemcpyHostToDevice, stream), "copy");
-    hip_check(hipMemcpyAsync(P->out_q.p, oq.data(), oq.size() * 4, hipMemcpyHostToDevice, stream), "copy");
+    P->out_off.upload(ooff.data(), ooff.size() * 8, stream);
+    P->out_q.upload(oq.data(), oq.size() * 4, stream);
   }
   // root element offsets (logical column-major order of the saved tensor)
   const qx::Value &rv = L.values[P->prog.save];
@@ -701,7 +710,7 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
     roff[e] = o;
   }
   moved |= P->relem_off.reserve(std::max<size_t>(R * 8, 16));
-  hip_check(hipMemcpyAsync(P->relem_off.p, roff.data(), R * 8, hipMemcpyHostToDevice, stream), "copy");
+  P->relem_off.upload(roff.data(), R * 8, stream);
   size_t nbits = (size_t)nb * nch;
   if (bits) {
     moved |= P->bits.reserve(std::max<size_t>(nbits, 16));
@@ -1052,8 +1061,8 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
   moved |= P->out_off.reserve(std::max<size_t>(ooff.size() * 8, 16));
   moved |= P->out_q.reserve(std::max<size_t>(oq.size() * 4, 16));
   if (nout) {
-    hip_check(hipMemcpyAsync(P->out_off.p, ooff.data(), ooff.size() * 8, hipMemcpyHostToDevice, stream), "copy");
-    hip_check(hipMemcpyAsync(P->out_q.p, oq.data(), oq.size() * 4, hipMemcpyHostToDevice, stream), "copy");
+    P->out_off.upload(ooff.data(), ooff.size() * 8, stream);
+    P->out_q.upload(oq.data(), oq.size() * 4, stream);
   }
   const qx::Value &rc = V.cot[V.root];
   const qx::Value &rv = L.values[P->prog.save];
@@ -1073,7 +1082,7 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
     }
   }
   moved |= P->relem_off.reserve(std::max<size_t>(R * 8, 16));
-  hip_check(hipMemcpyAsync(P->relem_off.p, roff.data(), R * 8, hipMemcpyHostToDevice, stream), "copy");
+  P->relem_off.upload(roff.data(), R * 8, stream);
   if (moved) P->drop_graphs();
 
   Bases bs;
