@@ -234,9 +234,21 @@ class Plan:
             o += n
         return out
 
-    def norm_vjp(self, bitstrings) -> tuple:
+    def key_offsets(self) -> dict:
+        """{data key: (offset, elements)} in the flat gradient of the pullbacks."""
+        cached = getattr(self, "_key_offsets", None)
+        if cached is None:
+            cached, o = {}, 0
+            for k, n in self.key_sizes():
+                cached[k] = (o, n)
+                o += n
+            self._key_offsets = cached
+        return cached
+
+    def norm_vjp(self, bitstrings, flat: bool = False) -> tuple:
         """(amplitudes, {key: gradient}) of the loss sum |amp|^2 in one pass
-        (`qx_norm_vjp`); QxError with status QX_ERR_UNSUPPORTED when slices loop."""
+        (`qx_norm_vjp`); with `flat`, the gradient as one array in the layout of
+        `key_offsets`.  QxError with status QX_ERR_UNSUPPORTED when slices loop."""
         bitstrings = list(bitstrings)
         nb = len(bitstrings)
         sizes = self.key_sizes()
@@ -244,6 +256,8 @@ class Plan:
         amps = np.zeros(nb * self.result_elems, dtype=np.complex128)
         _check(lib().qx_norm_vjp(self._h, self._bits(bitstrings), nb, _dptr(amps.view(np.float64)),
                                  _dptr(grad.view(np.float64))))
+        if flat:
+            return (amps if self.result_elems == 1 else amps.reshape(nb, self.result_elems)), grad
         out, o = {}, 0
         for k, n in sizes:
             out[k] = grad[o:o + n]

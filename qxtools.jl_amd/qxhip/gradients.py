@@ -74,16 +74,30 @@ class ParamCircuit:
         np.add.at(out, self._tp, per_term)
         return out
 
+    def _param_grad_flat(self, g_flat: np.ndarray) -> np.ndarray:
+        """`param_grad` from the pullback's flat gradient (layout of the plan's
+        `key_offsets`): the terms' entries gathered by one precomputed index."""
+        out = np.zeros(self.num_params)
+        if not self.terms:
+            return out
+        if getattr(self, "_tidx", None) is None:
+            offs = self.executor.plan.key_offsets()
+            self._tidx = np.concatenate([np.arange(offs[k][0], offs[k][0] + offs[k][1]) for k in self._tkeys])
+        per_term = np.add.reduceat(np.real(np.conj(g_flat[self._tidx]) * self._td), self._tstart)
+        np.add.at(out, self._tp, per_term)
+        return out
+
     def loss_and_grad(self, bits) -> tuple:
         """L = sum_b |amp_b|^2 and dL/dtheta: one pass of forward and pullback when
         every slice variable is contracted inline, else the amplitudes first and the
         pullback of the cotangent 2 amp after."""
         try:
-            amps, kg = self.executor.plan.norm_vjp(bits)
+            amps, g_flat = self.executor.plan.norm_vjp(bits, flat=True)
         except _native.QxError as e:
             if e.status != _native.QX_ERR_UNSUPPORTED:
                 raise
             amps = self.amplitudes(bits)
             kg = self.executor.vjp(bits, 2.0 * amps)
+            return float(np.sum(np.abs(amps) ** 2)), self.param_grad(kg), amps
         loss = float(np.sum(np.abs(amps) ** 2))
-        return loss, self.param_grad(kg), amps
+        return loss, self._param_grad_flat(g_flat), amps

@@ -134,3 +134,19 @@ def test_norm_vjp_refuses_looped_slices():
     for k in ref:
         assert _nerr(grad[k], ref[k]) < 1e-11, k
     ex.close()
+
+
+@pytest.mark.parametrize("planner,nslice", [("sweep", 0), ("greedy", 2)])
+def test_param_gradient_one_pass_equals_two_calls(planner, nslice):
+    """loss_and_grad's one-pass path (qx_norm_vjp, flat gradient gathered by one
+    index) against the two calls (amplitudes, then the pullback of 2 amp with the
+    per-key chain rule)."""
+    c = circuits.create_param_rqc_circuit(2, 3, 3, seed=8)
+    pc = gradients.ParamCircuit(c, planner=planner, nslice=nslice, batch=4)
+    bits = ["010110", "110001", "001111", "111000", "+-0101"]
+    loss, grad, amps = pc.loss_and_grad(bits)
+    amps2 = pc.amplitudes(bits)
+    grad2 = pc.param_grad(pc.executor.vjp(bits, 2.0 * amps2))
+    assert _nerr(amps, amps2) < 1e-12
+    assert loss == pytest.approx(float(np.sum(np.abs(amps2) ** 2)), rel=1e-12)
+    assert _nerr(grad, grad2) < 1e-11
