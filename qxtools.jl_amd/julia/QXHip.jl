@@ -148,6 +148,34 @@ function amplitudes_vjp(p::Plan, bitstrings::AbstractVector{<:AbstractString}, c
 end
 
 """
+    norm_loss_and_gradient(p, bitstrings) -> (amplitudes, Dict(key => gradient))
+
+The loss sum |amp|^2 over the bitstrings and its gradient with respect to every data
+key in one pass (`qx_norm_vjp`: the cotangent 2 amp is formed on the device from the
+same pass's forward).  Throws when slice variables would loop; use
+`amplitudes` + `amplitudes_vjp(p, bitstrings, 2 .* amps)` there.
+"""
+function norm_loss_and_gradient(p::Plan, bitstrings::AbstractVector{<:AbstractString})
+    ks = keys_and_sizes(p)
+    g = Vector{ComplexF64}(undef, sum(last, ks))
+    nb = length(bitstrings)
+    amps = Vector{ComplexF64}(undef, nb * p.result_elems)
+    bits = packbits(p, bitstrings)
+    lock(p.ctx.lock) do
+        check(ccall((:qx_norm_vjp, libqxhip), Int32,
+                    (Ptr{Cvoid}, Cstring, Int64, Ptr{ComplexF64}, Ptr{ComplexF64}),
+                    p.handle, bits, nb, amps, g))
+    end
+    out = Dict{Symbol,Vector{ComplexF64}}()
+    o = 0
+    for (k, n) in ks
+        out[k] = g[o+1:o+n]
+        o += n
+    end
+    (p.result_elems == 1 ? amps : reshape(amps, p.result_elems, nb)), out
+end
+
+"""
     gpu_amplitudes(data, p, bitstrings)
 
 Amplitudes as a function of the tensor data, differentiable through ChainRulesCore:
