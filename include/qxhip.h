@@ -82,6 +82,15 @@ qx_status qx_plan_from_dsl(qx_ctx *ctx, const char *dsl, size_t len, int precisi
  * `nelem` interleaved complex128 values in column-major order; it is copied. */
 qx_status qx_plan_set_tensor(qx_plan *plan, const char *key, const double *z, int64_t nelem);
 
+/* Replace the values of data key `key` after qx_plan_finalize, in the same device
+ * buffer (layout of qx_plan_set_tensor, same element count), so lowered steps and
+ * captured graphs stay valid; later calls see the new values.  The differentiable
+ * seam: new parameters of a variational loop (the ComplexF64 gate tensors that
+ * convert_to_tnc builds, src/tn_conversion.jl:10-40, and that single_amplitude
+ * contracts, src/simulation.jl:86-91) enter here without re-parsing, re-lowering or
+ * re-tuning the plan.  Returns after the copy has reached the device. */
+qx_status qx_plan_update_tensor(qx_plan *plan, const char *key, const double *z, int64_t nelem);
+
 /* Plan options, set before qx_plan_finalize ("profile", "graph", "fuse_slices" and
  * "autotune" may also be set later; setting "profile" clears the accumulated step
  * times):

@@ -1425,6 +1425,34 @@ qx_status qx_plan_set_tensor(qx_plan *plan, const char *key, const double *z, in
   });
 }
 
+qx_status qx_plan_update_tensor(qx_plan *plan, const char *key, const double *z, int64_t nelem) {
+  return guard([&] {
+    if (!plan || !key || (!z && nelem > 0)) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
+    if (!plan->finalized) throw qx::Error(QX_ERR_STATE, "plan is not finalized (use qx_plan_set_tensor)");
+    auto it = plan->prog.key_elems.find(key);
+    if (it == plan->prog.key_elems.end())
+      throw qx::Error(QX_ERR_DATA, std::string("data key '") + key + "' is not loaded by the program");
+    if (it->second != nelem)
+      throw qx::Error(QX_ERR_DATA, std::string("data key '") + key + "' needs " + std::to_string(it->second) +
+                                       " elements, got " + std::to_string(nelem));
+    const auto &keys = plan->prog.keys;
+    const size_t i = (size_t)(std::find(keys.begin(), keys.end(), std::string(key)) - keys.begin());
+    hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+    // same buffer, new contents: lowered steps, descriptor tables and captured graphs
+    // address the buffer, not its values, so all of them stay valid.  The copy is
+    // stream-ordered after earlier calls' work and synchronous towards the caller.
+    hipStream_t s = plan->ctx->stream;
+    if (plan->prec == QX_C128) {
+      hip_check(hipMemcpyAsync(plan->data[i].p, z, (size_t)nelem * 16, hipMemcpyHostToDevice, s), "upload");
+    } else {
+      std::vector<float> f(z, z + 2 * nelem);
+      hip_check(hipMemcpyAsync(plan->data[i].p, f.data(), (size_t)nelem * 8, hipMemcpyHostToDevice, s), "upload");
+    }
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    plan->host_data[key].assign(z, z + 2 * nelem);
+  });
+}
+
 qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
   return guard([&] {
     if (!plan || !name) throw qx::Error(QX_ERR_ARGUMENT, "null argument");

@@ -22,7 +22,7 @@ SYMBOLS = [
     "qx_plan_set_option", "qx_plan_finalize", "qx_plan_info", "qx_plan_key",
     "qx_amplitudes", "qx_amplitudes_vjp", "qx_plan_stats", "qx_plan_destroy", "qx_last_error",
     "qx_dsl_inspect", "qx_dsl_inspect_ex", "qx_amplitudes_uniform", "qx_sample_rejection",
-    "qx_measure_mfma", "qx_norm_vjp",
+    "qx_measure_mfma", "qx_norm_vjp", "qx_plan_update_tensor",
 ]
 
 
@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
         L.qx_ctx_destroy.restype = None
         L.qx_plan_from_dsl.argtypes = [p, C.c_char_p, sz, C.c_int, C.POINTER(p)]
         L.qx_plan_set_tensor.argtypes = [p, C.c_char_p, dp, i64]
+        L.qx_plan_update_tensor.argtypes = [p, C.c_char_p, dp, i64]
         L.qx_plan_set_option.argtypes = [p, C.c_char_p, i64]
         L.qx_plan_finalize.argtypes = [p]
         L.qx_plan_info.argtypes = [p] + [C.POINTER(i64)] * 5
@@ -141,6 +142,12 @@ class Plan:
         _check(lib().qx_plan_finalize(self._h))
         info = self.info()
         self.num_chars, self.result_elems, self.num_slices = info[:3]
+
+    def update_tensor(self, key: str, value) -> None:
+        """New values of data key `key` after finalize, in the same device buffer
+        (`qx_plan_update_tensor`): later calls, captured graphs included, see them."""
+        arr = np.ascontiguousarray(np.asarray(value).reshape(-1, order="F"), dtype=np.complex128)
+        _check(lib().qx_plan_update_tensor(self._h, key.encode(), _dptr(arr.view(np.float64)), arr.size))
 
     def keys_needed(self) -> list:
         n = C.c_int64()

@@ -15,7 +15,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import _native
-from .circuits import Circuit, gate_matrix_derivative, gate_qubits
+from .circuits import Circuit, gate_matrix, gate_matrix_derivative, gate_qubits
 from .compute_graph import build_compute_graph
 from .planning import contraction_scheme, sweep_contraction_plan
 from .simulation import Executor
@@ -37,28 +37,65 @@ class ParamCircuit:
             groups, plan, _ = contraction_scheme(tnc, nslice, **plan_kwargs)
         self.graph = build_compute_graph(tnc, plan, groups, dedupe=False)
         key_of = {c.name: c.args[0] for c in self.graph.commands if c.kind == "load"}
-        self.terms = []   # (param index, key, dT/dtheta in the key's array layout)
-        for g, tn_names in zip(circ.ops, names):
+        self._pgates = []   # (gate index, data key) of every parameterised gate
+        for gi, (g, tn_names) in enumerate(zip(circ.ops, names)):
             if g.param is None:
                 continue
-            scratch = TensorNetworkCircuit(circ.num_qubits)
-            dnames = scratch.push(gate_qubits(g), gate_matrix_derivative(g))
-            if len(dnames) != len(tn_names) or len(tn_names) != 1:
+            if len(tn_names) != 1:
                 raise ValueError("parameterised gates must convert to one tensor")
-            d = scratch.tn[dnames[0]].data
-            key = key_of[tn_names[0]]
-            if d.shape != self.graph.data[key].shape:
-                raise ValueError(f"derivative of gate {g.name} has a different tensor structure")
-            self.terms.append((g.param, key, d))
-        # the chain rule over all parameters as one segmented reduction
-        self._tp = np.array([p for p, _, _ in self.terms], dtype=np.int64)
-        self._tkeys = [k for _, k, _ in self.terms]
-        self._td = np.concatenate([d.reshape(-1, order="F") for _, _, d in self.terms]) if self.terms else np.zeros(0)
-        self._tstart = np.cumsum([0] + [d.size for _, _, d in self.terms[:-1]]).astype(np.int64)
+            self._pgates.append((gi, key_of[tn_names[0]]))
+        self._set_terms()
         self.dsl = self.graph.to_dsl()
         self.executor = Executor(self.dsl, self.graph.data, precision=precision, device=device, batch=batch,
                                  graph=graph)
         self.num_params = circ.num_params
+
+    def _gate_tensor(self, g, deriv: bool) -> np.ndarray:
+        """The gate's tensor (or dT/dtheta) through the same conversion as the network."""
+        scratch = TensorNetworkCircuit(self.circ.num_qubits)
+        m = gate_matrix_derivative(g) if deriv else gate_matrix(g)
+        (name,) = scratch.push(gate_qubits(g), m)
+        return scratch.tn[name].data
+
+    def _set_terms(self) -> None:
+        """Chain-rule terms (param index, key, dT/dtheta in the key's array layout) at
+        the current angles, as one segmented reduction over all parameters."""
+        self.terms = []
+        for gi, key in self._pgates:
+            g = self.circ.ops[gi]
+            d = self._gate_tensor(g, True)
+            if d.shape != self.graph.data[key].shape:
+                raise ValueError(f"derivative of gate {g.name} has a different tensor structure")
+            self.terms.append((g.param, key, d))
+        self._tp = np.array([p for p, _, _ in self.terms], dtype=np.int64)
+        self._tkeys = [k for _, k, _ in self.terms]
+        self._td = np.concatenate([d.reshape(-1, order="F") for _, _, d in self.terms]) if self.terms else np.zeros(0)
+        self._tstart = np.cumsum([0] + [d.size for _, _, d in self.terms[:-1]]).astype(np.int64)
+
+    def params(self) -> np.ndarray:
+        return self.circ.params()
+
+    def set_params(self, theta) -> None:
+        """Move the plan to new angles: every parameterised gate's tensor is rebuilt
+        and re-uploaded into its device buffer (`qx_plan_update_tensor`), so the
+        lowering, tuning and captured graphs are kept; the chain-rule terms follow."""
+        theta = np.asarray(theta, dtype=np.float64)
+        if theta.shape != (self.num_params,):
+            raise ValueError(f"need {self.num_params} angles, got shape {theta.shape}")
+        new = self.circ.with_params(theta)
+        tensors = []
+        for gi, key in self._pgates:
+            t = TensorNetworkCircuit(new.num_qubits)
+            (name,) = t.push(gate_qubits(new.ops[gi]), gate_matrix(new.ops[gi]))
+            arr = t.tn[name].data
+            if arr.shape != self.graph.data[key].shape:
+                raise ValueError(f"gate {new.ops[gi].name} at the new angle has a different tensor structure")
+            tensors.append((key, arr))
+        for key, arr in tensors:
+            self.executor.plan.update_tensor(key, arr)
+            self.graph.data[key] = arr
+        self.circ = new
+        self._set_terms()
 
     def amplitudes(self, bits) -> np.ndarray:
         return self.executor.amplitudes(bits)

@@ -65,6 +65,10 @@ class Executor:
     def amplitudes(self, bitstrings, slice_begin: int = 0, slice_end: int | None = None) -> np.ndarray:
         return self.plan.amplitudes(bitstrings, slice_begin, slice_end)
 
+    def update_tensor(self, key: str, value) -> None:
+        """New values of one data key after finalize (`qx_plan_update_tensor`)."""
+        self.plan.update_tensor(key, value)
+
     def vjp(self, bitstrings, amp_cot) -> dict:
         return self.plan.vjp(bitstrings, amp_cot)
 

@@ -143,3 +143,26 @@ def interleaved_gemm_program(nk=11, nm=8, nn=8, seed=0, view_a=False, a_data=Fal
     B = O.ncon(Y, xb_in, G2, [901, 2], xb)
     ref = O.ncon(A, xa, B, xb, cl).reshape(-1, order="F")
     return "\n".join(lines) + "\n", data, ref
+
+
+def users_guide_ghz2():
+    """(dsl, data) of the reference's DSL example (`docs/src/users_guide.md:71-90`,
+    kept as `tests/golden/users_guide_ghz2.qx`): two-qubit GHZ (H on qubit 1, CNOT
+    with control 1 and target 2) with both slice variables.  Its data keys are built
+    here from the gates, following the program's leg use:
+    * data_1 (2): the input state |0> (t4 and t5 are the two inputs);
+    * data_2 (2,2): H, leg 1 its output (the control wire's hyperindex), leg 2 its input;
+    * data_3 (2,2,2): the CNOT's target part, [out, in, c] = (X^c)[out, in], leg 3 on
+      the control's hyperindex (the CNOT is diagonal on its control);
+    * data_4 (2): the control part on the hyperindex, all ones."""
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "users_guide_ghz2.qx")
+    with open(path) as f:
+        dsl = f.read()
+    h = circuits.gate_matrix(circuits.Gate("h", (1,)))
+    x = circuits.gate_matrix(circuits.Gate("x", (1,)))
+    cnot_t = np.stack([np.eye(2, dtype=np.complex128), x], axis=2)
+    data = {"data_1": np.array([1.0, 0.0], dtype=np.complex128), "data_2": h, "data_3": cnot_t,
+            "data_4": np.ones(2, dtype=np.complex128)}
+    return dsl, data

@@ -215,3 +215,24 @@ def test_matrix_core_probe_rate_is_physical():
     for prec, peak in (("c128", 78.6), ("c64_4m", 157.3)):
         t = ctx.measure_mfma(prec)
         assert 0.2 * peak < t < 1.1 * peak, (prec, t)
+
+
+def test_users_guide_dsl_example_on_device():
+    """The reference's own DSL example (`docs/src/users_guide.md:71-90`: views on v1 and
+    v2, hyper labels, an outer product, a scalar result) executed on the device: all
+    16 bitstrings over {0,1,+,-}^2 against the oracle (1e-12) and the GHZ closed form,
+    over all slices (slice variables contracted inline and looped) and per slice."""
+    import itertools
+
+    dsl, data = qxtest.users_guide_ghz2()
+    bits = ["".join(p) for p in itertools.product("01+-", repeat=2)]
+    ref = O.amplitudes(dsl, data, bits)
+    s2 = 1 / math.sqrt(2)
+    for fuse in (1, 0):
+        ex = Executor(dsl, data, batch=4, fuse_slices=fuse)
+        assert ex.num_slices == 4
+        out = ex.amplitudes(bits)
+        _close(out, ref)
+        assert np.allclose(out[[0, 1, 4, 5]], [s2, 0, 0, s2], rtol=0, atol=1e-15)   # 00 01 10 11
+        parts = sum(ex.amplitudes(bits, s, s + 1) for s in range(4))
+        _close(parts, ref)

@@ -218,3 +218,19 @@ def test_env_gradient_matches_finite_differences():
     eye = np.eye(len(th))
     fd = np.array([(loss(th + h * eye[p]) - loss(th - h * eye[p])) / (2 * h) for p in range(len(th))])
     assert np.max(np.abs(grad - fd)) < 1e-8
+
+
+def test_users_guide_dsl_example():
+    """The reference's own DSL example (two-qubit GHZ, `docs/src/users_guide.md:71-90`)
+    with data keys built from its gates: every bitstring over {0,1,+,-}^2 equals the
+    FP64 statevector of H(1), CNOT(1->2); "00" and "11" are 1/sqrt2, "01" and "10" 0."""
+    import itertools
+
+    dsl, data = qxtest.users_guide_ghz2()
+    bits = ["".join(p) for p in itertools.product("01+-", repeat=2)]
+    out = O.amplitudes(dsl, data, bits)
+    ref = qxtest.statevector_amps(circuits.create_ghz_circuit(2), bits)
+    assert np.max(np.abs(out - ref)) < 1e-15
+    s2 = 1 / np.sqrt(2)
+    assert np.allclose(out[[0, 1, 4, 5]], [s2, 0, 0, s2], rtol=0, atol=1e-16)   # 00 01 10 11
+    assert len(O.slice_vars(O.parse(dsl))) == 2
