@@ -58,26 +58,63 @@ def plan_roofline(stats: dict, prec: str, ms_per_step: float, steps: int) -> flo
     return t / (ms_per_step * 1e-3)
 
 
-def cpu_baseline(dsl: str, data: dict, bits: list, seconds: float, threads: int = 0) -> dict:
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(dsl: str, data: dict, bits: list, seconds: float, threads: int = 0, gpu_amps=None) -> dict:
     """The oracle (numpy TTGT, OpenBLAS zgemm) contracting whole amplitudes one
     bitstring at a time, as the reference's per-bitstring loop does, on `threads`
-    BLAS threads (0: the host's cores, at most 16)."""
+    BLAS threads (0: the host's cores, at most 16).  The amplitudes it computes are
+    kept and compared with the GPU's for the same bitstrings (`gpu_amps`)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import qxoracle
     from threadpoolctl import threadpool_limits
 
     cores = threads if threads > 0 else min(os.cpu_count() or 1, 16)
     done = 0
+    ref = []
     t0 = time.perf_counter()
     with threadpool_limits(limits=cores, user_api="blas"):
         while done == 0 or (time.perf_counter() - t0 < seconds and done < len(bits)):
-            qxoracle.amplitudes(dsl, data, [bits[done]])
+            # the saved tensor's elements in column-major order, as the GPU returns them
+            ref.append(np.asarray(qxoracle.amplitudes(dsl, data, [bits[done]]))[0].reshape(-1, order="F"))
             done += 1
             log(f"cpu baseline: {done} amplitude(s) in {time.perf_counter() - t0:.1f} s")
     el = time.perf_counter() - t0
-    return {"value": done / el, "unit": "amplitudes/s", "cores": cores, "kind": "port",
-            "sample": f"{done} of the workload's bitstrings, each contracted over the whole plan "
-                      f"by the oracle ({el:.1f} s)"}
+    out = {"value": done / el, "unit": "amplitudes/s", "cores": cores, "kind": "port",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+           "sample": f"{done} of the workload's bitstrings, each contracted over the whole plan "
+                     f"by the oracle ({el:.1f} s)"}
+    if gpu_amps is not None:
+        g = np.asarray(gpu_amps).reshape(len(bits), -1)[:done]
+        r = np.stack(ref)
+        out["gpu_vs_oracle_max_rel_err"] = float(np.max(np.abs(g - r) / np.abs(r)))
+        out["compared_amplitudes"] = int(r.size)
+    return out
+
+
+def golden_check(workload: str, bits: list, amps) -> dict | None:
+    """Per-amplitude relative error of the GPU result against the committed FP64
+    statevector amplitudes of the workload, where such a fixture exists."""
+    path = os.path.join(ROOT, "tests", "golden", f"{workload}_amplitudes.npz")
+    if not os.path.exists(path):
+        return None
+    with np.load(path, allow_pickle=False) as z:
+        if list(z["bits"]) != list(bits):
+            return None
+        ref = z["sv_amps"]
+    rel = np.abs(np.asarray(amps).reshape(-1) - ref) / np.abs(ref)
+    return {"reference": "FP64 statevector (2^25 entries)", "amplitudes": int(ref.size),
+            "max_rel_err": float(rel.max()), "p99_rel_err": float(np.quantile(rel, 0.99)),
+            "median_rel_err": float(np.median(rel))}
 
 
 def measured_traffic(workload: str, prec: str, kernel: str):
@@ -318,10 +355,20 @@ def main() -> int:
         ref = c128_reference(dsl, data, bits, w)
         out["error_vs_c128"] = c64_comparison(dsl, data, bits, ref, a, precs=(prec,))[prec]
         out["error_vs_c128"].pop("value"), out["error_vs_c128"].pop("ms_per_step")
+    gold = golden_check(w.name, bits, amps)
+    if gold is not None:
+        out["parity_vs_statevector"] = gold
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(dsl, data, bits, a.cpu_seconds, a.cpu_threads)
-    _ = np.asarray(amps)
+        out["cpu_baseline"] = cpu_baseline(dsl, data, bits, a.cpu_seconds, a.cpu_threads, gpu_amps=amps)
     print(json.dumps(out), flush=True)
+    # the plain complex128 path must match the FP64 references it is checked against
+    if prec == "c128":
+        errs = [gold["max_rel_err"]] if gold else []
+        if "cpu_baseline" in out:
+            errs.append(out["cpu_baseline"]["gpu_vs_oracle_max_rel_err"])
+        if errs and max(errs) > 1e-10:
+            log(f"parity failure: GPU amplitudes differ from the FP64 references by {max(errs):.3e}")
+            return 1
     return 0
 
 
