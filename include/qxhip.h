@@ -147,6 +147,20 @@ qx_status qx_plan_key(const qx_plan *plan, int64_t i, char *name, size_t cap, in
 qx_status qx_amplitudes(qx_plan *plan, const char *bits, int64_t nb, int64_t slice_begin,
                         int64_t slice_end, double *out);
 
+/* qx_amplitudes that allocates no device memory, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, results and slice
+ * handling as qx_amplitudes; it contracts in batches of the plan's "batch" option and
+ * uses the plan's arena, workspace and tables as an earlier qx_amplitudes call of at
+ * least that many bitstrings sized them (QX_ERR_STATE otherwise).  Its per-call device
+ * buffers (FP64 accumulators, staged bitstrings, index tables and results: about
+ * 48 * nb * result_elems + nb * num_chars bytes) live in a fixed 8 MiB scratch area
+ * of the library, one per device; larger calls return QX_ERR_ARGUMENT.  Every call on
+ * a device shares that area: calls that overlap on one device (several threads or
+ * contexts) race.  No graph capture, autotuning or profiling. */
+qx_status qx_amplitudes_noalloc(qx_plan *plan, const char *bits, int64_t nb, int64_t slice_begin,
+                                int64_t slice_end, double *out);
+
 /* Output sampling on the device (the output methods of output_params_dict,
  * src/outputs.jl:47-78, executed by QXContexts and described in
  * docs/src/features.md:64-110).  Candidate j (0-based) of stream `seed` is the

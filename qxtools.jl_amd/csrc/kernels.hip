@@ -10,6 +10,9 @@
 // every bitstring-dependent tensor, guarantees it for batched steps).
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include <type_traits>
 
 #include <vector>
@@ -2144,6 +2147,37 @@ void launch_add_batch(int prec, const AddDesc *d_desc, int n, void *stream) {
   else
     hipLaunchKernelGGL(k_add_batch<float>, dim3(n), dim3(256), 0, s, d_desc);
   check_launch("add_batch");
+}
+
+}  // namespace qx
+
+// Per-call device memory of the allocation-free entry point (accumulators, staged
+// bitstrings, output and index tables).  Host code never passes `scratch` itself to a
+// kernel or copy: its host-side name is not a device address; the address comes from
+// hipGetSymbolAddress, per device.
+static __device__ __align__(256) unsigned char scratch[qx::kScratchBytes];
+
+namespace qx {
+
+int scratch_address(int device, void **ptr) {
+  static std::mutex mu;
+  static std::map<int, void *> addr;   // device -> address of its copy of `scratch`
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) {
+    *ptr = it->second;
+    return (int)hipSuccess;
+  }
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return (int)e;
+  if (cur != device) return (int)hipErrorInvalidDevice;
+  void *p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return (int)e;
+  addr[device] = p;
+  *ptr = p;
+  return (int)hipSuccess;
 }
 
 }  // namespace qx

@@ -381,4 +381,10 @@ void launch_reject(const double *amps, const char *cand, int64_t nb, int nch, ui
                    void *stream);
 void launch_zero(void *p, int64_t bytes, void *stream);
 
+// The library's fixed device scratch area (a file-scope device array of kScratchBytes,
+// one copy per device): its device address on `device`, looked up once per device
+// with that device current and kept.  Returns the HIP status of the lookup.
+constexpr size_t kScratchBytes = size_t(8) << 20;
+int scratch_address(int device, void **ptr);
+
 }  // namespace qx

@@ -27,6 +27,7 @@ struct qx_ctx {
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;   // created by the first plan that runs a vendor BLAS step
   void *blas_ws = nullptr;         // its workspace (the handle allocates nothing itself)
+  void *scratch = nullptr;         // device address of the library's fixed scratch area
 };
 
 namespace {
@@ -423,11 +424,12 @@ void prepare_small(qx_plan *P, const qx::Lowered &L, const Bases &bs) {
 }
 
 // Launch the level of batched small steps that starts at step s.
-void run_small_level(qx_plan *P, const qx::Lowered &L, size_t s) {
+void run_small_level(qx_plan *P, const qx::Lowered &L, size_t s, bool timed = true) {
   hipStream_t stream = P->ctx->stream;
   const qx_plan::SmallTable &T = P->small.at(&L);
   size_t ev = 0;
-  if (P->opt_profile) {
+  timed = timed && P->opt_profile;
+  if (timed) {
     ev = P->event_step.size();
     while (P->events.size() < 2 * (ev + 1)) {
       hipEvent_t e;
@@ -439,17 +441,18 @@ void run_small_level(qx_plan *P, const qx::Lowered &L, size_t s) {
   }
   const qx::SmallDesc *d = (const qx::SmallDesc *)T.desc.p + T.index[s];
   qx::launch_small_batch(P->prec, d, T.count[s], stream);
-  if (P->opt_profile) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
+  if (timed) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
 }
 
-void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int64_t> &varval) {
+void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int64_t> &varval, bool timed = true) {
   const qx::Step &st = L.steps[s];
   void *A = value_ptr(P, L.values[st.in_a], varval);
   void *B = value_ptr(P, L.values[st.in_b], varval);
   void *C = value_ptr(P, L.values[st.node], varval);
   hipStream_t stream = P->ctx->stream;
   size_t ev = 0;
-  if (P->opt_profile) {
+  timed = timed && P->opt_profile;
+  if (timed) {
     ev = P->event_step.size();
     while (P->events.size() < 2 * (ev + 1)) {
       hipEvent_t e;
@@ -460,7 +463,7 @@ void run_step(qx_plan *P, const qx::Lowered &L, size_t s, const std::vector<int6
     hip_check(hipEventRecord(P->events[2 * ev], stream), "hipEventRecord");
   }
   launch_step(P, L, st, A, B, C, varval);
-  if (P->opt_profile) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
+  if (timed) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
 }
 
 std::string kern_variant(const qx::Step &st);
@@ -639,6 +642,59 @@ void check_bits(qx_plan *P, const char *bits, int64_t nb) {
   }
 }
 
+// Device buffers of one amplitudes call (besides the plan's arena, workspace and data).
+struct CallBufs {
+  void *acc = nullptr;        // nb * R FP64 Neumaier (sum, compensation) pairs, complex
+  void *outbuf = nullptr;     // nb * R complex128 results
+  void *out_off = nullptr;    // per output tensor: its arena slot (int64)
+  void *out_q = nullptr;      // per output tensor: its qubit (int32)
+  void *relem_off = nullptr;  // per result element: offset in the saved tensor (int64)
+  void *bits = nullptr;       // nb * num_chars bitstring characters
+};
+
+// The device work of one amplitudes call, from zeroing the accumulators to the final
+// sum: steps of section 0 once, per looped slice those of section 1, per batch of
+// bitstrings the output fill, section 2 and the accumulation into FP64 pairs.
+// `timed`: record per-launch events when the plan's "profile" option is on.
+void enqueue_amplitudes(qx_plan *P, const qx::Lowered &L, uint64_t fused, int64_t s0, int64_t s1, int64_t nb,
+                        int64_t batch, const CallBufs &cb, Tuner *tuner, bool timed) {
+  hipStream_t stream = P->ctx->stream;
+  const int64_t nch = P->prog.num_chars;
+  const int64_t R = L.result_elems;
+  const int nout = (int)P->prog.outputs.size();
+  const qx::Value &rv = L.values[P->prog.save];
+  qx::launch_zero(cb.acc, nb * R * 32, stream);
+  const size_t nv = P->prog.vars.size();
+  std::vector<int64_t> varval(nv, 0);
+  auto run_section = [&](int sec) {
+    for (size_t s = 0; s < L.steps.size(); ++s) {
+      if (L.steps[s].section != sec) continue;
+      if (L.steps[s].level >= 0) {   // batched small steps: one launch per level
+        if (P->small.at(&L).count[s] > 0) run_small_level(P, L, s, timed);
+        continue;
+      }
+      if (tuner)
+        tune_step(P, L, s, varval, *tuner);
+      else
+        run_step(P, L, s, varval, timed);
+    }
+  };
+  run_section(0);
+  for (int64_t sl = s0; sl < s1; ++sl) {
+    slice_values(P, fused, sl, varval);
+    run_section(1);
+    for (int64_t row0 = 0; row0 < nb; row0 += batch) {
+      int64_t nvalid = std::min<int64_t>(batch, nb - row0);
+      qx::launch_fill_outputs(P->prec, P->arena.p, (const int64_t *)cb.out_off, nout, (const int *)cb.out_q,
+                              (const char *)cb.bits + row0 * nch, nvalid, batch, nch, stream);
+      run_section(2);
+      qx::launch_accumulate(P->prec, value_ptr(P, rv, varval), rv.dep ? rv.beta_stride : 0, batch, nvalid, R,
+                            (const int64_t *)cb.relem_off, (double *)cb.acc, row0, stream);
+    }
+  }
+  qx::launch_finalize((const double *)cb.acc, (double *)cb.outbuf, nb * R, stream);
+}
+
 // Amplitudes of nb bitstrings summed over slices [s0, s1).  `bits` null: the
 // bitstrings are already in P->bits on the device (sampling); `out` null: the
 // results stay in P->outbuf.
@@ -729,39 +785,10 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
 
   P->event_step.clear();
 
-  // the device work of one call, from zeroing the accumulators to the final sum
-  auto enqueue = [&]() {
-    qx::launch_zero(P->acc.p, nb * R * 32, stream);
-    const size_t nv = P->prog.vars.size();
-    std::vector<int64_t> varval(nv, 0);
-    auto run_section = [&](int sec) {
-      for (size_t s = 0; s < L.steps.size(); ++s) {
-        if (L.steps[s].section != sec) continue;
-        if (L.steps[s].level >= 0) {   // batched small steps: one launch per level
-          if (P->small.at(&L).count[s] > 0) run_small_level(P, L, s);
-          continue;
-        }
-        if (tuner)
-          tune_step(P, L, s, varval, *tuner);
-        else
-          run_step(P, L, s, varval);
-      }
-    };
-    run_section(0);
-    for (int64_t sl = s0; sl < s1; ++sl) {
-      slice_values(P, fused, sl, varval);
-      run_section(1);
-      for (int64_t row0 = 0; row0 < nb; row0 += batch) {
-        int64_t nvalid = std::min<int64_t>(batch, nb - row0);
-        qx::launch_fill_outputs(P->prec, P->arena.p, (const int64_t *)P->out_off.p, nout, (const int *)P->out_q.p,
-                                (const char *)P->bits.p + row0 * nch, nvalid, batch, nch, stream);
-        run_section(2);
-        qx::launch_accumulate(P->prec, value_ptr(P, rv, varval), rv.dep ? rv.beta_stride : 0, batch, nvalid, R,
-                              (const int64_t *)P->relem_off.p, (double *)P->acc.p, row0, stream);
-      }
-    }
-    qx::launch_finalize((const double *)P->acc.p, (double *)P->outbuf.p, nb * R, stream);
-  };
+  CallBufs cb;
+  cb.acc = P->acc.p, cb.outbuf = P->outbuf.p, cb.out_off = P->out_off.p, cb.out_q = P->out_q.p;
+  cb.relem_off = P->relem_off.p, cb.bits = P->bits.p;
+  auto enqueue = [&]() { enqueue_amplitudes(P, L, fused, s0, s1, nb, batch, cb, tuner.get(), true); };
   const int64_t nrows = (nb + batch - 1) / batch;
   const int64_t launches = 2 + section_launches(L, 0) +
                            (s1 - s0) * (section_launches(L, 1) + nrows * (section_launches(L, 2) + 2));
@@ -795,6 +822,83 @@ void amplitudes(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   if (tuner) finish_tuning(P, L, *tuner);
   collect_profile(P, L);
+}
+
+// qx_amplitudes without device allocations: the plan's batch option as the batch
+// size, its arena, workspace, small-step tables (and BLAS handle) as an earlier
+// qx_amplitudes call left them, and the per-call buffers in the library's fixed
+// scratch area.  No graph capture, tuning or event timing.
+void amplitudes_noalloc(qx_plan *P, const char *bits, int64_t nb, int64_t s0, int64_t s1, double *out) {
+  if (!P->finalized) throw qx::Error(QX_ERR_STATE, "plan is not finalized");
+  if (nb <= 0) throw qx::Error(QX_ERR_ARGUMENT, "need at least one bitstring");
+  if (!bits || !out) throw qx::Error(QX_ERR_ARGUMENT, "null bitstring or output pointer");
+  int64_t nslices = 1;
+  for (int64_t d : P->prog.var_dims) nslices *= d;
+  if (s0 < 0 || s1 > nslices || s0 >= s1) throw qx::Error(QX_ERR_ARGUMENT, "slice range out of bounds");
+  check_bits(P, bits, nb);
+  const int64_t batch = P->opt_batch;
+  const uint64_t fused = (s0 == 0 && s1 == nslices) ? choose_fused(P, batch, false) : 0;
+  if (fused) {
+    s0 = 0;
+    s1 = looped_slices(P, fused);
+  }
+  const qx::Lowered &L = ensure_lowered(P, batch, fused);
+  const size_t es = P->esize();
+  const char *warm = ": run one qx_amplitudes call of at least the plan's batch option in bitstrings first";
+  if ((size_t)L.arena_elems * es > P->arena.bytes || (size_t)L.ws_elems * es > P->ws.bytes || !P->arena.p)
+    throw qx::Error(QX_ERR_STATE, std::string("plan memory not sized for this call") + warm);
+  auto sm = P->small.find(&L);
+  if (sm == P->small.end() || sm->second.arena != P->arena.p || sm->second.index.size() != L.steps.size())
+    throw qx::Error(QX_ERR_STATE, std::string("small-step tables not built for this call") + warm);
+  for (auto &st : L.steps)
+    if (st.kern == qx::Kern::Blas && !P->ctx->blas)
+      throw qx::Error(QX_ERR_STATE, std::string("library GEMM handle not created") + warm);
+  if (!P->ctx->scratch) throw qx::Error(QX_ERR_STATE, "context has no scratch area");
+  // per-call buffers, 256-byte aligned, in the scratch area
+  const int64_t nch = P->prog.num_chars;
+  const int64_t R = L.result_elems;
+  std::vector<int64_t> ooff;
+  std::vector<int> oq;
+  for (int o : P->prog.outputs) {
+    ooff.push_back(L.values[o].slot);
+    oq.push_back(P->prog.nodes[o].qubit);
+  }
+  const qx::Value &rv = L.values[P->prog.save];
+  std::vector<int64_t> roff(R);
+  for (int64_t e = 0; e < R; ++e) {
+    int64_t r = e, o = 0;
+    for (size_t j = 0; j < rv.dims.size(); ++j) {
+      o += (r % rv.dims[j]) * rv.strides[j];
+      r /= rv.dims[j];
+    }
+    roff[e] = o;
+  }
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t b_acc = up((size_t)nb * R * 32), b_out = up((size_t)nb * R * 16), b_roff = up((size_t)R * 8),
+               b_ooff = up(ooff.size() * 8), b_oq = up(oq.size() * 4), b_bits = up((size_t)nb * nch);
+  const size_t total = b_acc + b_out + b_roff + b_ooff + b_oq + b_bits;
+  if (total > qx::kScratchBytes)
+    throw qx::Error(QX_ERR_ARGUMENT, "call needs " + std::to_string(total) + " bytes of scratch (nb * result_elems = " +
+                                         std::to_string(nb * R) + "), over the fixed " +
+                                         std::to_string(qx::kScratchBytes) + " bytes");
+  char *base = (char *)P->ctx->scratch;
+  CallBufs cb;
+  cb.acc = base;
+  cb.outbuf = base + b_acc;
+  cb.relem_off = (char *)cb.outbuf + b_out;
+  cb.out_off = (char *)cb.relem_off + b_roff;
+  cb.out_q = (char *)cb.out_off + b_ooff;
+  cb.bits = (char *)cb.out_q + b_oq;
+  hipStream_t stream = P->ctx->stream;
+  hip_check(hipMemcpyAsync(cb.relem_off, roff.data(), (size_t)R * 8, hipMemcpyHostToDevice, stream), "copy");
+  if (!ooff.empty()) {
+    hip_check(hipMemcpyAsync(cb.out_off, ooff.data(), ooff.size() * 8, hipMemcpyHostToDevice, stream), "copy");
+    hip_check(hipMemcpyAsync(cb.out_q, oq.data(), oq.size() * 4, hipMemcpyHostToDevice, stream), "copy");
+  }
+  if (nb * nch > 0) hip_check(hipMemcpyAsync(cb.bits, bits, (size_t)nb * nch, hipMemcpyHostToDevice, stream), "copy");
+  enqueue_amplitudes(P, L, fused, s0, s1, nb, batch, cb, nullptr, false);
+  hip_check(hipMemcpyAsync(out, cb.outbuf, (size_t)nb * R * 16, hipMemcpyDeviceToHost, stream), "copy out");
+  hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
 // Stage nb Philox bitstrings (candidates j0 .. j0+nb-1 of stream `seed`) in P->bits.
@@ -1345,7 +1449,13 @@ qx_status qx_ctx_create(int device, qx_ctx **out) {
     hip_check(hipSetDevice(device), "hipSetDevice");
     auto *c = new qx_ctx;
     c->device = device;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    // the device address of the fixed scratch area (looked up once per device)
+    hipError_t e = (hipError_t)qx::scratch_address(device, &c->scratch);
+    if (e != hipSuccess) {
+      delete c;
+      hip_check(e, "hipGetSymbolAddress (scratch)");
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
       delete c;
       hip_check(e, "hipStreamCreate");
@@ -1559,6 +1669,15 @@ qx_status qx_amplitudes(qx_plan *plan, const char *bits, int64_t nb, int64_t sli
     if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
     hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
     amplitudes(plan, bits, nb, slice_begin, slice_end, out);
+  });
+}
+
+qx_status qx_amplitudes_noalloc(qx_plan *plan, const char *bits, int64_t nb, int64_t slice_begin, int64_t slice_end,
+                                double *out) {
+  return guard([&] {
+    if (!plan) throw qx::Error(QX_ERR_ARGUMENT, "null plan");
+    hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+    amplitudes_noalloc(plan, bits, nb, slice_begin, slice_end, out);
   });
 }
 

@@ -22,7 +22,7 @@ SYMBOLS = [
     "qx_plan_set_option", "qx_plan_finalize", "qx_plan_info", "qx_plan_key",
     "qx_amplitudes", "qx_amplitudes_vjp", "qx_plan_stats", "qx_plan_destroy", "qx_last_error",
     "qx_dsl_inspect", "qx_dsl_inspect_ex", "qx_amplitudes_uniform", "qx_sample_rejection",
-    "qx_measure_mfma", "qx_norm_vjp", "qx_plan_update_tensor",
+    "qx_measure_mfma", "qx_norm_vjp", "qx_plan_update_tensor", "qx_amplitudes_noalloc",
 ]
 
 
@@ -54,6 +54,7 @@ def lib() -> C.CDLL:
         L.qx_plan_info.argtypes = [p] + [C.POINTER(i64)] * 5
         L.qx_plan_key.argtypes = [p, i64, C.c_char_p, sz, C.POINTER(i64)]
         L.qx_amplitudes.argtypes = [p, C.c_char_p, i64, i64, i64, dp]
+        L.qx_amplitudes_noalloc.argtypes = [p, C.c_char_p, i64, i64, i64, dp]
         L.qx_amplitudes_vjp.argtypes = [p, C.c_char_p, i64, dp, dp]
         L.qx_plan_stats.argtypes = [p, C.c_char_p, sz]
         L.qx_plan_destroy.argtypes = [p]
