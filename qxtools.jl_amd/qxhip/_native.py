@@ -11,7 +11,8 @@ import os
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqxhip.so")
+# QXHIP_LIB names another build of the same ABI (the sanitizer build of the host code)
+LIB_PATH = os.environ.get("QXHIP_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "libqxhip.so")
 
 PRECISIONS = {"c128": 0, "c64_4m": 1, "c64_3m": 2, "c64_bf16x3": 3}
 
