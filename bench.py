@@ -136,32 +136,20 @@ def measured_traffic(workload: str, prec: str, kernel: str):
 
 def c64_comparison(dsl: str, data: dict, bits: list, ref: np.ndarray, a, precs=("c64_4m", "c64_3m")) -> dict:
     """Other precisions on the same plan: throughput and per-amplitude relative error
-    against `ref` (the complex128 GPU result)."""
-    import torch
-
-    from qxhip.simulation import Executor
-
+    against `ref` (the complex128 GPU result).  A precision whose distinguishing product
+    ran on no step of the plan (3M: the matrix-core GEMMs; split bf16: plain steps with
+    M, N, K >= 1024) is the 4M path there and is left out of the line."""
     res = {}
     for prec in precs:
-        ex = Executor(dsl, data, precision=prec, device=0, batch=min(256, len(bits)), graph=a.graph,
-                      autotune=a.autotune)
-        if a.autotune:
-            ex.amplitudes(bits)
-        for _ in range(a.warmup):
-            ex.amplitudes(bits)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(a.steps):
-            amps = ex.amplitudes(bits)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        amps = np.asarray(amps).reshape(-1)
-        rel = np.abs(amps - ref.reshape(-1)) / np.abs(ref.reshape(-1))
-        res[prec] = {"value": amps.size * a.steps / el, "ms_per_step": 1e3 * el / a.steps,
-                     "rel_err_max": float(rel.max()), "rel_err_p99": float(np.quantile(rel, 0.99)),
-                     "rel_err_median": float(np.median(rel)),
-                     "normwise_err": float(np.linalg.norm(amps - ref.reshape(-1)) / np.linalg.norm(ref))}
-        ex.close()
+        amps, v, ms, var = precision_line(dsl, data, bits, prec, a.steps, a.warmup, False, graph=a.graph,
+                                          autotune=a.autotune)
+        if prec == "c64_3m" and not any(x.startswith(("k_plain_gemm", "k_contract_gemm_mfma")) for x in var):
+            log("c64_3m: no step ran a three-product GEMM on this plan (same as 4M); left out")
+            continue
+        if prec == "c64_bf16x3" and not any("bf16x3" in x or x.startswith("rocblas") for x in var):
+            log("c64_bf16x3: no step met the split product's size (same as 4M); left out")
+            continue
+        res[prec] = {"value": v, "ms_per_step": ms, "kernels": var, **error_stats(amps, ref.reshape(-1))}
         log(f"{prec}: {res[prec]['value']:.0f} amplitudes/s, max rel err {res[prec]['rel_err_max']:.2e}")
     return res
 
@@ -173,6 +161,72 @@ def c128_reference(dsl, data, bits, w):
     r = np.asarray(ex.amplitudes(bits))
     ex.close()
     return r
+
+
+def precision_line(dsl, data, bits, prec, steps, warmup, batch_last, graph=1, autotune=1, **opts) -> tuple:
+    """(amplitudes, amplitudes/s, ms per call, kernel variants run) of one precision on
+    a plan: an autotuning call, warm-up calls, then `steps` timed calls."""
+    import torch
+
+    from qxhip.simulation import Executor
+
+    ex = Executor(dsl, data, precision=prec, device=0, batch=min(256, len(bits)), graph=graph, autotune=autotune,
+                  batch_last=batch_last, **opts)
+    try:
+        if autotune:
+            ex.amplitudes(bits)
+        for _ in range(warmup):
+            ex.amplitudes(bits)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            amps = ex.amplitudes(bits)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        batch = next(b for b in ex.stats()["batches"] if b.get("active"))
+        variants = sorted({st["variant"] for st in batch["steps"]})
+        per_call = len(bits) * ex.plan.result_elems
+    finally:
+        ex.close()
+    return np.asarray(amps).reshape(-1), per_call * steps / el, 1e3 * el / steps, variants
+
+
+def error_stats(amps, ref) -> dict:
+    rel = np.abs(amps - ref) / np.abs(ref)
+    return {"rel_err_max": float(rel.max()), "rel_err_p99": float(np.quantile(rel, 0.99)),
+            "rel_err_median": float(np.median(rel)),
+            "normwise_err": float(np.linalg.norm(amps - ref) / np.linalg.norm(ref))}
+
+
+def batching_bench(a) -> dict:
+    """The batching workload: 7x7 grid, depth 12 (the stated depth 24 has no plan that
+    fits one GPU), 2^10 slices, 5 open output legs x 32 bitstring prefixes = 1024
+    amplitudes per call.  Plain complex128, complex64 4M and the split bf16 product,
+    each with per-amplitude error against complex128; and complex128 with one launch
+    per contraction (no chain passes, no level batching, no graph) for comparison."""
+    from qxhip import workloads
+
+    w = workloads.WORKLOADS["rqc_7x7_d12_s10_open5"]
+    dsl, data, bits = w.load()
+    steps, warm = max(1, min(a.steps, 5)), 1
+    out = {"workload": w.name, "qubits": w.qubits, "depth": w.depth, "sliced_bonds": w.sliced,
+           "amplitudes_per_call": None, "note": w.note}
+    ref, v, ms, var = precision_line(dsl, data, bits, "c128", steps, warm, w.batch_last)
+    out["amplitudes_per_call"] = int(ref.size)
+    out["c128"] = {"value": v, "ms_per_step": ms, "kernels": var}
+    log(f"batching workload c128: {v:.0f} amplitudes/s")
+    for prec in ("c64_4m", "c64_bf16x3"):
+        amps, v, ms, var = precision_line(dsl, data, bits, prec, steps, warm, w.batch_last)
+        out[prec] = {"value": v, "ms_per_step": ms, "vs_c128": v / out["c128"]["value"], "kernels": var,
+                     **error_stats(amps, ref)}
+        log(f"batching workload {prec}: {v:.0f} amplitudes/s, max rel err {out[prec]['rel_err_max']:.2e}")
+    amps, v, ms, var = precision_line(dsl, data, bits, "c128", steps, 0, w.batch_last, graph=0, autotune=0,
+                                      chains=0)
+    out["c128_one_launch_per_contraction"] = {"value": v, "ms_per_step": ms,
+                                              "batched_speedup": out["c128"]["value"] / v,
+                                              "max_rel_diff": float(np.max(np.abs(amps - ref) / np.abs(ref)))}
+    log(f"batching workload c128 one launch per contraction: {v:.0f} amplitudes/s")
+    return out
 
 
 def grad_bench(a) -> dict:
@@ -236,6 +290,8 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=1, help="replay each call from a captured hipGraph")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64 comparison")
     ap.add_argument("--grad", action="store_true", help="gradient / forward time of the parameterised workload")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batching workload and the gradient ratio in the headline line")
     ap.add_argument("--batch-last", type=int, default=None,
                     help="lay the bitstring batch leg out slowest in intermediates (default: per workload)")
     ap.add_argument("--autotune", type=int, default=1,
@@ -350,11 +406,18 @@ def main() -> int:
         out["copy_calibration_bytes"] = a.copy_mib << 20
         out["copy_gbps"] = copy_gbps
     if not a.no_c64 and prec == "c128":
-        out["c64"] = c64_comparison(dsl, data, bits, np.asarray(amps), a)
+        out["c64"] = c64_comparison(dsl, data, bits, np.asarray(amps), a, precs=("c64_4m", "c64_3m", "c64_bf16x3"))
     elif not a.no_c64:
         ref = c128_reference(dsl, data, bits, w)
         out["error_vs_c128"] = c64_comparison(dsl, data, bits, ref, a, precs=(prec,))[prec]
         out["error_vs_c128"].pop("value"), out["error_vs_c128"].pop("ms_per_step")
+    if not a.no_extras:
+        out["batching"] = batching_bench(a)
+        g = grad_bench(a)
+        out["gradient"] = {k: g[k] for k in ("value", "forward_ms", "forward_plus_gradient_ms", "pullback_ms",
+                                              "params", "config")}
+        out["gradient"]["metric"] = g["metric"]
+        log(f"gradient / forward: {g['value']:.2f}")
     gold = golden_check(w.name, bits, amps)
     if gold is not None:
         out["parity_vs_statevector"] = gold

@@ -92,7 +92,7 @@ qx_status qx_plan_set_tensor(qx_plan *plan, const char *key, const double *z, in
 qx_status qx_plan_update_tensor(qx_plan *plan, const char *key, const double *z, int64_t nelem);
 
 /* Plan options, set before qx_plan_finalize ("profile", "graph", "fuse_slices" and
- * "autotune" may also be set later; setting "profile" clears the accumulated step
+ * "autotune" may also be set later, "small_walk" too; setting "profile" clears the accumulated step
  * times):
  *   "batch"       bitstrings contracted together (power of two, default 256)
  *   "arena_mb"    device arena cap in MiB (default: 90% of free memory)
@@ -111,6 +111,10 @@ qx_status qx_plan_update_tensor(qx_plan *plan, const char *key, const double *z,
  *                 0 = one launch per step
  *   "batch_last"  1 = lay the bitstring batch leg out slowest in bitstring-dependent
  *                 intermediates (default 0: fastest)
+ *   "small_walk"  1 = a section whose batched small steps total at most 2^22
+ *                 multiply-adds runs all their dependency levels in one launch of one
+ *                 workgroup; 0 (default) = one launch per level (may be set at any time;
+ *                 the walk is latency-bound and slower on the plans measured so far)
  *   "kernel"      -1 (default) = heuristic kernel choice; 0 gather, 1 rows, 2-4
  *                 FP tile GEMMs, 5 / 7 matrix-core GEMM (K chunks of 16 / 32), 6 reduce,
  *                 8 rocBLAS strided-batched GEMM (steps whose operands are plain

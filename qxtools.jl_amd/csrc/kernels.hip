@@ -1117,6 +1117,350 @@ void launch_contract_gemm(int prec, const void *A, const void *B, void *C, void 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Plain complex GEMM on matrix cores: C (r x s, ldc) = op(X) (r x k) . op(Y) (k x s)
+// for column-major X, Y, C with batch strides -- the contractions whose operands the
+// lowering laid out as plain matrices (op 1 = stored transposed).  A workgroup of four
+// waves (2 x 2) owns a TM x TN tile of C for one batch element and one K range:
+// K stages of TK values are loaded with lanes along the operand's contiguous index
+// (16 bytes per lane for complex128), staged in LDS as [k][m] / [k][n] (one element of
+// padding per row), and the next stage is fetched into registers while the matrix
+// cores work on the current one.  Each wave holds WM x WN of C as 16 x 16 blocks of
+// 16x16x4 products: 4M (Re += Xr.Yr - Xi.Yi, Im += Xr.Yi + Xi.Yr, four products per
+// block) or 3M (RR = Xr.Yr, II = Xi.Yi, SS = (Xr+Xi).(Yr+Yi); Re = RR - II,
+// Im = SS - RR - II, three products).  Accumulation in the operand precision.  With
+// ksplit > 1 each K range writes a dense r x s partial per batch element to `ws`
+// (k_plain_sum adds them into C).  Work units are ordered with the C column tile
+// fastest and dealt to the eight XCDs in contiguous runs, so the units of one XCD share
+// their X panels in its L2.
+template <typename T, bool GAUSS> struct PlainTile;
+template <> struct PlainTile<double, false> { static constexpr int WM = 32, WN = 32, TK = 16; };
+template <> struct PlainTile<float, false> { static constexpr int WM = 64, WN = 64, TK = 16; };
+template <> struct PlainTile<float, true> { static constexpr int WM = 64, WN = 32, TK = 16; };
+
+template <typename T, int OPX, int OPY, bool GAUSS>
+__global__ __launch_bounds__(256) void k_plain_gemm(const typename Vec<T>::type *__restrict__ X,
+                                                    const typename Vec<T>::type *__restrict__ Y,
+                                                    typename Vec<T>::type *__restrict__ C,
+                                                    typename Vec<T>::type *__restrict__ ws, BlasArgs b) {
+  using V = typename Vec<T>::type;
+  using M = Mfma16<T>;
+  using v4 = typename M::v4;
+  constexpr int WM = PlainTile<T, GAUSS>::WM, WN = PlainTile<T, GAUSS>::WN, TK = PlainTile<T, GAUSS>::TK;
+  constexpr int TM = 2 * WM, TN = 2 * WN, PA = TM + 1, PB = TN + 1;
+  constexpr int BI = WM / 16, BJ = WN / 16;
+  constexpr int EA = TM * TK / 256, EB = TN * TK / 256;
+  __shared__ V sA[TK * PA];
+  __shared__ V sB[TK * PB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lq = lane >> 4;
+  const int64_t tr = (b.r + TM - 1) / TM, ts = (b.s + TN - 1) / TN;
+  const int64_t units = b.batch * tr * ts * b.ksplit;
+  int64_t u = blockIdx.x;
+  {
+    const int64_t nb = gridDim.x;   // one unit per workgroup; contiguous runs per XCD
+    if (nb % 8 == 0) u = (u % 8) * (nb / 8) + u / 8;
+  }
+  if (u >= units) return;
+  const int64_t st = u % ts;
+  int64_t rest = u / ts;
+  const int64_t ks = rest % b.ksplit;
+  rest /= b.ksplit;
+  const int64_t rt = rest % tr;
+  const int64_t bat = rest / tr;
+  const int64_t m0 = rt * TM, n0 = st * TN;
+  const int64_t stages = b.k / TK / b.ksplit;
+  const int64_t k0 = ks * stages * TK;
+  // this thread's elements of the X and Y stages: offsets within a stage and masks
+  int64_t offa[EA], offb[EB];
+  int slota[EA], slotb[EB];
+  bool oka[EA], okb[EB];
+#pragma unroll
+  for (int j = 0; j < EA; ++j) {
+    const int e = tid + 256 * j;
+    const int m = OPX ? e / TK : e % TM, kk = OPX ? e % TK : e / TM;
+    oka[j] = m0 + m < b.r;
+    offa[j] = OPX ? kk + (m0 + m) * b.ldx : (m0 + m) + kk * b.ldx;
+    slota[j] = kk * PA + m;
+  }
+#pragma unroll
+  for (int j = 0; j < EB; ++j) {
+    const int e = tid + 256 * j;
+    const int n = OPY ? e % TN : e / TK, kk = OPY ? e / TN : e % TK;
+    okb[j] = n0 + n < b.s;
+    offb[j] = OPY ? (n0 + n) + kk * b.ldy : kk + (n0 + n) * b.ldy;
+    slotb[j] = kk * PB + n;
+  }
+  const V *Xb = X + bat * b.sx + (OPX ? k0 : k0 * b.ldx);
+  const V *Yb = Y + bat * b.sy + (OPY ? k0 * b.ldy : k0);
+  const int64_t xstep = OPX ? TK : TK * b.ldx, ystep = OPY ? TK * b.ldy : TK;
+  V ra[EA], rb[EB];
+  auto fetch = [&](int64_t sg) {
+    const V *xa = Xb + sg * xstep;
+    const V *yb = Yb + sg * ystep;
+#pragma unroll
+    for (int j = 0; j < EA; ++j) ra[j] = oka[j] ? xa[offa[j]] : V{0, 0};
+#pragma unroll
+    for (int j = 0; j < EB; ++j) rb[j] = okb[j] ? yb[offb[j]] : V{0, 0};
+  };
+  v4 acc0[BI][BJ], acc1[BI][BJ], acc2[BI][BJ];   // 4M: Re, Im; 3M: RR, II, SS
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j) acc0[i][j] = v4{0, 0, 0, 0}, acc1[i][j] = v4{0, 0, 0, 0}, acc2[i][j] = v4{0, 0, 0, 0};
+  fetch(0);
+  for (int64_t sg = 0; sg < stages; ++sg) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < EA; ++j) sA[slota[j]] = ra[j];
+#pragma unroll
+    for (int j = 0; j < EB; ++j) sB[slotb[j]] = rb[j];
+    __syncthreads();
+    if (sg + 1 < stages) fetch(sg + 1);
+#pragma unroll
+    for (int kc = 0; kc < TK; kc += 4) {
+      V fa[BI], fb[BJ];
+#pragma unroll
+      for (int i = 0; i < BI; ++i) fa[i] = sA[(kc + lq) * PA + WM * wm + 16 * i + li];
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) fb[j] = sB[(kc + lq) * PB + WN * wn + 16 * j + li];
+      if (GAUSS) {
+        T fs[BI], gs[BJ];
+#pragma unroll
+        for (int i = 0; i < BI; ++i) fs[i] = fa[i].x + fa[i].y;
+#pragma unroll
+        for (int j = 0; j < BJ; ++j) gs[j] = fb[j].x + fb[j].y;
+#pragma unroll
+        for (int i = 0; i < BI; ++i)
+#pragma unroll
+          for (int j = 0; j < BJ; ++j) {
+            acc0[i][j] = M::mma(fa[i].x, fb[j].x, acc0[i][j]);
+            acc1[i][j] = M::mma(fa[i].y, fb[j].y, acc1[i][j]);
+            acc2[i][j] = M::mma(fs[i], gs[j], acc2[i][j]);
+          }
+      } else {
+#pragma unroll
+        for (int i = 0; i < BI; ++i) {
+          const T nai = -fa[i].y;
+#pragma unroll
+          for (int j = 0; j < BJ; ++j) {
+            acc0[i][j] = M::mma(fa[i].x, fb[j].x, acc0[i][j]);
+            acc1[i][j] = M::mma(fa[i].x, fb[j].y, acc1[i][j]);
+          }
+#pragma unroll
+          for (int j = 0; j < BJ; ++j) {
+            acc0[i][j] = M::mma(nai, fb[j].y, acc0[i][j]);
+            acc1[i][j] = M::mma(fa[i].y, fb[j].x, acc1[i][j]);
+          }
+        }
+      }
+    }
+  }
+  // results: to C, or as this K range's dense partial
+  const bool part = b.ksplit > 1;
+  V *dst = part ? ws + (ks * b.batch + bat) * b.r * b.s : C + bat * b.sc;
+  const int64_t ld = part ? b.r : b.ldc;
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j) {
+      const int64_t n = n0 + WN * wn + 16 * j + li;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t m = m0 + WM * wm + 16 * i + M::row(lq, e);
+        V o;
+        if (GAUSS) {
+          o.x = acc0[i][j][e] - acc1[i][j][e];
+          o.y = acc2[i][j][e] - acc0[i][j][e] - acc1[i][j][e];
+        } else {
+          o.x = acc0[i][j][e];
+          o.y = acc1[i][j][e];
+        }
+        if (m < b.r && n < b.s) dst[m + n * ld] = o;
+      }
+    }
+}
+
+// C (r x s per batch element, ldc, batch stride sc) = sum of the ksplit dense partials.
+template <typename T>
+__global__ void k_plain_sum(typename Vec<T>::type *__restrict__ C, const typename Vec<T>::type *__restrict__ ws,
+                            BlasArgs b) {
+  using V = typename Vec<T>::type;
+  const int64_t per = b.r * b.s, n = per * b.batch;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    V acc = ws[i];
+    for (int s = 1; s < b.ksplit; ++s) {
+      const V v = ws[(int64_t)s * n + i];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    const int64_t bat = i / per, e = i % per;
+    C[bat * b.sc + (e % b.r) + (e / b.r) * b.ldc] = acc;
+  }
+}
+
+// Split bf16 product of a plain complex64 GEMM (QX_C64_BF16X3): every real product
+// P.Q is Ph.Qh + Ph.Ql + Pl.Qh with h = bf16(x) and l = bf16(x - h) (round to nearest
+// even), on 16x16x32 bf16 matrix cores accumulating in FP32 -- twelve bf16 products per
+// complex product (Re = Xr.Yr - Xi.Yi with the Xi parts negated, Im = Xr.Yi + Xi.Yr).
+// The split happens while a K stage is staged in LDS: each lane loads two complex64
+// values adjacent in k, rounds them, and stores the packed pairs into four planes
+// (re hi, re lo, im hi, im lo) laid out [m][k] / [n][k] with 8 values of padding per row,
+// so a lane's fragment (8 consecutive k values of one row) is one 16-byte LDS read.
+// Tiles of 64 x 64 of C per workgroup (2 x 2 waves of 2 x 2 blocks), K stages of 32,
+// the next stage fetched into registers during the products; K split and unit order as
+// in k_plain_gemm.  A lane's k slot is the same in both operands, so the products pair
+// equal k whatever order the matrix core sums them in; the result layout is that of
+// the 16x16x4 FP32 product (Mfma16<float>).
+template <int OPX, int OPY>
+__global__ __launch_bounds__(256) void k_plain_gemm_bf16x3(const float2 *__restrict__ X, const float2 *__restrict__ Y,
+                                                           float2 *__restrict__ C, float2 *__restrict__ ws,
+                                                           BlasArgs b) {
+  typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  constexpr int TM = 64, TN = 64, TK = 32, RW = (TK + 8) / 2;   // row: 20 dwords (16 used)
+  constexpr int BI = 2, BJ = 2, NP = 4;                           // planes: rh, rl, ih, il
+  __shared__ uint32_t sA[NP][TM][RW];
+  __shared__ uint32_t sB[NP][TN][RW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lq = lane >> 4;
+  const int64_t tr = (b.r + TM - 1) / TM, ts = (b.s + TN - 1) / TN;
+  const int64_t units = b.batch * tr * ts * b.ksplit;
+  int64_t u = blockIdx.x;
+  {
+    const int64_t nb = gridDim.x;
+    if (nb % 8 == 0) u = (u % 8) * (nb / 8) + u / 8;
+  }
+  if (u >= units) return;
+  const int64_t st = u % ts;
+  int64_t rest = u / ts;
+  const int64_t ks = rest % b.ksplit;
+  rest /= b.ksplit;
+  const int64_t rt = rest % tr;
+  const int64_t bat = rest / tr;
+  const int64_t m0 = rt * TM, n0 = st * TN;
+  const int64_t stages = b.k / TK / b.ksplit;
+  const int64_t k0 = ks * stages * TK;
+  // four k pairs per thread and operand: (row, pair) of each, and the two elements'
+  // offsets inside a stage
+  int ra_row[4], ra_pair[4], rb_row[4], rb_pair[4];
+  int64_t oa0[4], oa1[4], ob0[4], ob1[4];
+  bool oka[4], okb[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int m, p;
+    if (OPX) p = tid % 16, m = tid / 16 + 16 * j;   // k contiguous in X: lanes along k
+    else m = tid % 64, p = tid / 64 + 4 * j;        // m contiguous in X: lanes along m
+    ra_row[j] = m, ra_pair[j] = p;
+    oka[j] = m0 + m < b.r;
+    const int64_t kk = 2 * p;
+    oa0[j] = OPX ? kk + (m0 + m) * b.ldx : (m0 + m) + kk * b.ldx;
+    oa1[j] = OPX ? oa0[j] + 1 : oa0[j] + b.ldx;
+    int n, q;
+    if (OPY) n = tid % 64, q = tid / 64 + 4 * j;    // Y stored n x k: lanes along n
+    else q = tid % 16, n = tid / 16 + 16 * j;       // Y stored k x n: lanes along k
+    rb_row[j] = n, rb_pair[j] = q;
+    okb[j] = n0 + n < b.s;
+    const int64_t kq = 2 * q;
+    ob0[j] = OPY ? (n0 + n) + kq * b.ldy : kq + (n0 + n) * b.ldy;
+    ob1[j] = OPY ? ob0[j] + b.ldy : ob0[j] + 1;
+  }
+  const float2 *Xb = X + bat * b.sx + (OPX ? k0 : k0 * b.ldx);
+  const float2 *Yb = Y + bat * b.sy + (OPY ? k0 * b.ldy : k0);
+  const int64_t xstep = OPX ? TK : TK * b.ldx, ystep = OPY ? TK * b.ldy : TK;
+  float2 xa0[4], xa1[4], yb0[4], yb1[4];
+  auto fetch = [&](int64_t sg) {
+    const float2 *xa = Xb + sg * xstep;
+    const float2 *yb = Yb + sg * ystep;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      xa0[j] = oka[j] ? xa[oa0[j]] : float2{0, 0};
+      xa1[j] = oka[j] ? xa[oa1[j]] : float2{0, 0};
+      yb0[j] = okb[j] ? yb[ob0[j]] : float2{0, 0};
+      yb1[j] = okb[j] ? yb[ob1[j]] : float2{0, 0};
+    }
+  };
+  // (hi, lo) planes of a pair of values adjacent in k, packed two per dword
+  auto split_store = [&](uint32_t (*pl)[TM][RW], int row, int pair, float2 v0, float2 v1) {
+    const uint16_t rh0 = bf16_bits(v0.x), rh1 = bf16_bits(v1.x), ih0 = bf16_bits(v0.y), ih1 = bf16_bits(v1.y);
+    pl[0][row][pair] = (uint32_t)rh0 | ((uint32_t)rh1 << 16);
+    pl[1][row][pair] = (uint32_t)bf16_bits(v0.x - bf16_value(rh0)) | ((uint32_t)bf16_bits(v1.x - bf16_value(rh1)) << 16);
+    pl[2][row][pair] = (uint32_t)ih0 | ((uint32_t)ih1 << 16);
+    pl[3][row][pair] = (uint32_t)bf16_bits(v0.y - bf16_value(ih0)) | ((uint32_t)bf16_bits(v1.y - bf16_value(ih1)) << 16);
+  };
+  f4 accr[BI][BJ], acci[BI][BJ];
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j) accr[i][j] = f4{0, 0, 0, 0}, acci[i][j] = f4{0, 0, 0, 0};
+  auto frag = [&](const uint32_t (*pl)[TM][RW], int plane, int row) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(&pl[plane][row][4 * lq]);
+    return v;
+  };
+  auto as_bf8 = [](uint4 v) { return __builtin_bit_cast(bf8, v); };
+  auto neg = [](uint4 v) { return uint4{v.x ^ 0x80008000u, v.y ^ 0x80008000u, v.z ^ 0x80008000u, v.w ^ 0x80008000u}; };
+  fetch(0);
+  for (int64_t sg = 0; sg < stages; ++sg) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      split_store(sA, ra_row[j], ra_pair[j], xa0[j], xa1[j]);
+      split_store(sB, rb_row[j], rb_pair[j], yb0[j], yb1[j]);
+    }
+    __syncthreads();
+    if (sg + 1 < stages) fetch(sg + 1);
+    bf8 yrh[BJ], yrl[BJ], yih[BJ], yil[BJ];
+#pragma unroll
+    for (int j = 0; j < BJ; ++j) {
+      const int row = 32 * wn + 16 * j + li;
+      yrh[j] = as_bf8(frag(sB, 0, row));
+      yrl[j] = as_bf8(frag(sB, 1, row));
+      yih[j] = as_bf8(frag(sB, 2, row));
+      yil[j] = as_bf8(frag(sB, 3, row));
+    }
+#pragma unroll
+    for (int i = 0; i < BI; ++i) {
+      const int row = 32 * wm + 16 * i + li;
+      const uint4 ih = frag(sA, 2, row), il = frag(sA, 3, row);
+      const bf8 xrh = as_bf8(frag(sA, 0, row)), xrl = as_bf8(frag(sA, 1, row));
+      const bf8 xih = as_bf8(ih), xil = as_bf8(il), nxih = as_bf8(neg(ih)), nxil = as_bf8(neg(il));
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) {
+        f4 r = accr[i][j], m = acci[i][j];
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xrh, yrh[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xrh, yih[j], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xrh, yrl[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xrh, yil[j], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xrl, yrh[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xrl, yih[j], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nxih, yih[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xih, yrh[j], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nxih, yil[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xih, yrl[j], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nxil, yih[j], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xil, yrh[j], m, 0, 0, 0);
+        accr[i][j] = r, acci[i][j] = m;
+      }
+    }
+  }
+  const bool part = b.ksplit > 1;
+  float2 *dst = part ? ws + (ks * b.batch + bat) * b.r * b.s : C + bat * b.sc;
+  const int64_t ld = part ? b.r : b.ldc;
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j) {
+      const int64_t n = n0 + 32 * wn + 16 * j + li;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t m = m0 + 32 * wm + 16 * i + Mfma16<float>::row(lq, e);
+        if (m < b.r && n < b.s) dst[m + n * ld] = float2{accr[i][j][e], acci[i][j][e]};
+      }
+    }
+}
+
 // Skinny contraction: C[h, m, n] = sum_k A[h, m, k] B[h, k, n] for at most 32 (m, n)
 // pairs (N <= 2) and a long K.  Workgroup (h, chunk): each lane walks K indices of the
 // chunk (low K bits, A's smallest strides), loads its A value for every m and B value
@@ -1716,17 +2060,16 @@ __device__ __forceinline__ typename Vec<T>::type small_dot(const SmallDesc &s, c
   return acc;
 }
 
+// One chunk of a small step (descriptor `s`) by the workgroup's kSmallThreads lanes.
 template <typename T>
-__global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *__restrict__ d) {
+__device__ __forceinline__ void small_item(const SmallDesc &s, typename Vec<T>::type *part) {
   using V = typename Vec<T>::type;
-  const SmallDesc &s = d[blockIdx.x];
   const V *A = (const V *)s.A;
   const V *B = (const V *)s.B;
   V *C = (V *)s.C;
   const int nk = 1 << s.nbk;
   const int len = s.c1 - s.c0;   // a power of two
   if (len < kSmallThreads && nk > 1) {   // lanes split K: 2^lt lanes per element of the chunk
-    __shared__ V part[kSmallThreads];
     const int lt = 8 - (__ffs(len) - 1);   // kSmallThreads = 2^8
     const int c = s.c0 + (threadIdx.x >> lt), sub = threadIdx.x & ((1 << lt) - 1);
     int64_t oa = 0, ob = 0;
@@ -1743,6 +2086,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *
       }
       C[c] = sum;
     }
+    __syncthreads();   // `part` is free again
     return;
   }
   for (int c = s.c0 + threadIdx.x; c < s.c1; c += blockDim.x) {
@@ -1750,6 +2094,30 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *
     for (int b = 0; b < s.nbc; ++b)
       if ((c >> b) & 1) oa += s.wa[b], ob += s.wb[b];
     C[c] = small_dot<T>(s, A, B, oa, ob, 0, 1, nk);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSmallThreads) void k_small_batch(const SmallDesc *__restrict__ d) {
+  __shared__ typename Vec<T>::type part[kSmallThreads];
+  small_item<T>(d[blockIdx.x], part);
+}
+
+// Every dependency level of a section's small steps in one launch by one workgroup:
+// the levels' chunks in order, a workgroup barrier between levels (so a level reads
+// what the previous ones wrote).  For sections whose small steps are tiny in total:
+// the levels' launch gaps cost more than their arithmetic.  One workgroup keeps every
+// producer and consumer on one CU; workgroups on several XCDs would need an L2
+// write-back and invalidate at each level to see each other's results, which costs
+// more than a launch.
+template <typename T>
+__global__ __launch_bounds__(kSmallThreads) void k_small_walk(const SmallDesc *__restrict__ d,
+                                                              const int *__restrict__ level_off, int nlevels) {
+  __shared__ typename Vec<T>::type part[kSmallThreads];
+  for (int L = 0; L < nlevels; ++L) {
+    const int i1 = level_off[L + 1];
+    for (int i = level_off[L]; i < i1; ++i) small_item<T>(d[i], part);
+    __syncthreads();
   }
 }
 
@@ -2129,6 +2497,56 @@ void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const C
   check_launch("contract_chain");
 }
 
+int64_t plain_gemm_units(int prec, const BlasArgs &b) {
+  // C tile of a workgroup: 2 x 2 waves of PlainTile<T, GAUSS>; 64 x 64 for the split bf16 product
+  const int tm = prec == QX_C128 || b.split ? 64 : 128, tn = prec == QX_C128 || prec == QX_C64_3M || b.split ? 64 : 128;
+  return b.batch * ((b.r + tm - 1) / tm) * ((b.s + tn - 1) / tn);
+}
+
+void launch_plain_gemm(int prec, const void *X, const void *Y, void *C, void *ws, const BlasArgs &b, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t units = plain_gemm_units(prec, b) * b.ksplit;
+  if (units > (int64_t(1) << 31) - 1) throw Error(QX_ERR_UNSUPPORTED, "plain GEMM: too many tiles");
+  const dim3 g((unsigned)units);
+  const int sel = b.opx * 2 + b.opy;
+#define QX_PLAIN(TT, V2, OX, OY, GA) \
+  hipLaunchKernelGGL((k_plain_gemm<TT, OX, OY, GA>), g, dim3(256), 0, s, (const V2 *)X, (const V2 *)Y, (V2 *)C, \
+                     (V2 *)ws, b)
+#define QX_PLAIN_OPS(TT, V2, GA)                   \
+  switch (sel) {                                   \
+    case 0: QX_PLAIN(TT, V2, 0, 0, GA); break;     \
+    case 1: QX_PLAIN(TT, V2, 0, 1, GA); break;     \
+    case 2: QX_PLAIN(TT, V2, 1, 0, GA); break;     \
+    default: QX_PLAIN(TT, V2, 1, 1, GA); break;    \
+  }
+  if (b.split) {
+    if (prec == QX_C128) throw Error(QX_ERR_STATE, "internal: split bf16 product of complex128 operands");
+    switch (sel) {
+      case 0: hipLaunchKernelGGL((k_plain_gemm_bf16x3<0, 0>), g, dim3(256), 0, s, (const float2 *)X, (const float2 *)Y, (float2 *)C, (float2 *)ws, b); break;
+      case 1: hipLaunchKernelGGL((k_plain_gemm_bf16x3<0, 1>), g, dim3(256), 0, s, (const float2 *)X, (const float2 *)Y, (float2 *)C, (float2 *)ws, b); break;
+      case 2: hipLaunchKernelGGL((k_plain_gemm_bf16x3<1, 0>), g, dim3(256), 0, s, (const float2 *)X, (const float2 *)Y, (float2 *)C, (float2 *)ws, b); break;
+      default: hipLaunchKernelGGL((k_plain_gemm_bf16x3<1, 1>), g, dim3(256), 0, s, (const float2 *)X, (const float2 *)Y, (float2 *)C, (float2 *)ws, b); break;
+    }
+  } else if (prec == QX_C128) {
+    QX_PLAIN_OPS(double, double2, false)
+  } else if (prec == QX_C64_3M) {
+    QX_PLAIN_OPS(float, float2, true)
+  } else {
+    QX_PLAIN_OPS(float, float2, false)
+  }
+#undef QX_PLAIN_OPS
+#undef QX_PLAIN
+  check_launch("plain_gemm");
+  if (b.ksplit > 1) {
+    const int gs = grid_for(b.r * b.s * b.batch, 256);
+    if (prec == QX_C128)
+      hipLaunchKernelGGL(k_plain_sum<double>, dim3(gs), dim3(256), 0, s, (double2 *)C, (const double2 *)ws, b);
+    else
+      hipLaunchKernelGGL(k_plain_sum<float>, dim3(gs), dim3(256), 0, s, (float2 *)C, (const float2 *)ws, b);
+    check_launch("plain_sum");
+  }
+}
+
 void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream) {
   if (n <= 0) return;
   hipStream_t s = (hipStream_t)stream;
@@ -2137,6 +2555,16 @@ void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream) 
   else
     hipLaunchKernelGGL(k_small_batch<float>, dim3(n), dim3(kSmallThreads), 0, s, d_desc);
   check_launch("small_batch");
+}
+
+void launch_small_walk(int prec, const SmallDesc *d_desc, const int *d_level_off, int nlevels, void *stream) {
+  if (nlevels <= 0) return;
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == QX_C128)
+    hipLaunchKernelGGL(k_small_walk<double>, dim3(1), dim3(kSmallThreads), 0, s, d_desc, d_level_off, nlevels);
+  else
+    hipLaunchKernelGGL(k_small_walk<float>, dim3(1), dim3(kSmallThreads), 0, s, d_desc, d_level_off, nlevels);
+  check_launch("small_walk");
 }
 
 void launch_add_batch(int prec, const AddDesc *d_desc, int n, void *stream) {

@@ -131,7 +131,17 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
 
 // Kernel configurations a power-of-two step can run with.
 enum Choice { kBits = 0, kRows = 1, kGemm0 = 2, kGemm1 = 3, kGemm2 = 4, kMfma = 5, kReduce = 6, kMfma32 = 7,
-              kBlas = 8, kSkinny = 9, kTtgt = 10, kNumChoices = 11 };
+              kBlas = 8, kSkinny = 9, kTtgt = 10, kPlain = 11, kNumChoices = 12 };
+
+// K split of the hand-written plain GEMM: enough work units to give every CU two
+// workgroups, each K range at least 8 stages (128 values) long.
+void plain_split(BlasArgs &b, int precision) {
+  b.own = 1;
+  b.ksplit = 1;
+  const int64_t units = plain_gemm_units(precision, b);
+  const int64_t stages = b.k / (b.split ? 32 : 16);
+  while (units * b.ksplit < 512 && stages / (b.ksplit * 2) >= 8) b.ksplit *= 2;
+}
 
 namespace {
 struct CB {
@@ -295,6 +305,17 @@ bool configure_kernel(Step &st, int precision, int choice) {
       while (st.c_elems * b.ksplit < 8192 && (int64_t(1) << nk) / b.ksplit >= 128) b.ksplit *= 2;
       break;
     }
+    case kPlain: {
+      BlasArgs ba;
+      if (!plain_gemm(st, ba)) return false;
+      // complex64 steps with M, N, K >= 1024 under QX_C64_BF16X3: the split bf16 product
+      ba.split = precision == QX_C64_BF16X3 && ba.r >= 1024 && ba.s >= 1024 && ba.k >= 1024;
+      plain_split(ba, precision);
+      st.ws_elems = ba.ksplit > 1 ? ba.ksplit * ba.batch * ba.r * ba.s : 0;
+      st.kern = Kern::Blas;
+      st.blas = ba;
+      return true;
+    }
     case kTtgt: {
       // a very large step whose operands are not plain matrices: A and B gathered into
       // plain copies, the library GEMM, the result gathered into C (the copies cost a
@@ -326,10 +347,11 @@ bool configure_kernel(Step &st, int precision, int choice) {
       ba.ta = M * K * H, ba.tb = K * N * H;
       const int64_t tc = M * N * H;
       st.ws_elems = ba.ta + ba.tb + tc + 512;
-      if (precision == QX_C64_BF16X3 && M >= 1024 && N >= 1024 && K >= 1024) {
-        ba.split = 1, ba.ex = ba.ta, ba.ey = ba.tb, ba.ec = tc;
-        st.ws_elems += ba.ex + ba.ey + ba.ec;
-      }
+      // the copies' product on the hand-written GEMM (its K-split partials after the
+      // copies), as the split bf16 product for QX_C64_BF16X3 with M, N, K >= 1024
+      ba.split = precision == QX_C64_BF16X3 && M >= 1024 && N >= 1024 && K >= 1024;
+      plain_split(ba, precision);
+      if (ba.ksplit > 1) st.ws_elems += ba.ksplit * tc + 128;
       st.kern = Kern::Blas;
       st.blas = ba;
       return true;
@@ -440,16 +462,16 @@ bool configure_kernel(Step &st, int precision, int choice) {
 //    unit stride, else the gather kernel with K split over workgroups
 //  * Bits   : everything else with power-of-two dims
 //  * Radix  : any other dims
-//  * Blas   : complex128 steps of >= 2^30 flops whose operands are plain matrices
-//             (rocBLAS' zgemm reaches about twice the FP64 rate of the tile kernels);
-//             with QX_C64_BF16X3, plain steps with M, N, K >= 1024 (split bf16 product)
+//  * Plain  : steps of >= 2^30 flops whose operands are plain matrices: the
+//             hand-written matrix-core GEMM (k_plain_gemm); with QX_C64_BF16X3, plain
+//             steps with M, N, K >= 1024 as its split bf16 product
+//  * Blas   : the vendor BLAS, a tuned candidate only
 int default_choice(const Step &st, int precision) {
   const StepBits sb = step_bits(st);
   BlasArgs ba;
-  if (precision == QX_C128 && 8.0 * st.H * st.M * st.N * st.K >= double(int64_t(1) << 30) && plain_gemm(st, ba))
-    return kBlas;
   if (precision == QX_C64_BF16X3 && plain_gemm(st, ba) && ba.r >= 1024 && ba.s >= 1024 && ba.k >= 1024)
-    return kBlas;   // the split bf16 product
+    return kPlain;   // the split bf16 product
+  if (8.0 * st.H * st.M * st.N * st.K >= double(int64_t(1) << 30) && plain_gemm(st, ba)) return kPlain;
   if (sb.nr >= 12 && sb.nk <= 4 && sb.nk + sb.nn >= 1 && !sb.cbits.empty() && sb.cbits[0].wa != 0 && rows_ok(sb))
     return kRows;
   if (sb.nk >= 3 && sb.nn >= 3 && sb.nm >= 5) {
@@ -543,7 +565,7 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
   if (sb.nk >= 12 && sb.nm + sb.nn <= 5 && sb.nn <= 1) alts.push_back(kSkinny);
   if (sb.nk >= 4 && sb.nn >= 4 && sb.nm >= 4 && 8.0 * st.H * st.M * st.N * st.K >= double(int64_t(1) << 34))
     alts.push_back(kTtgt);
-  if (sb.nk >= 4 && sb.nn >= 4 && sb.nm >= 4) alts.push_back(kBlas);
+  if (sb.nk >= 4 && sb.nn >= 4 && sb.nm >= 4) alts.insert(alts.end(), {kPlain, kBlas});
   if (sb.nk <= 6 || st.c_elems <= (int64_t(1) << 16)) alts.push_back(kBits);
   const int def = default_choice(st, precision);
   for (int c : alts) {

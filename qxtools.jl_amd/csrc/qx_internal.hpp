@@ -216,6 +216,9 @@ struct BlasArgs {
   int ttgt = 0, nga = 0, ngb = 0, ngc = 0;
   int64_t ga[kMaxBits] = {0}, gb[kMaxBits] = {0}, gc[kMaxBits] = {0};
   int64_t ta = 0, tb = 0;   // elements of the A and B copies (the result copy: r s batch)
+  // own = 1: the hand-written matrix-core GEMM (k_plain_gemm) instead of the vendor
+  // BLAS; its K split over ksplit workgroups with dense partials in the workspace
+  int own = 0, ksplit = 1;
 };
 
 enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows, Chain, Blas };
@@ -358,6 +361,9 @@ void launch_root_cot(int prec, void *dst, int64_t dep, int64_t batch, int64_t nv
 void launch_root_cot_norm(int prec, void *dst, const void *val, int64_t dep, int64_t batch, int64_t nvalid,
                           int64_t relems, const int64_t *d_relem_off, double *amp, int64_t row0, void *stream);
 void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream);
+// Levels [0, nlevels) of small-step chunks in one workgroup: level L is the descriptors
+// [d_level_off[L], d_level_off[L + 1]), run after every earlier level.
+void launch_small_walk(int prec, const SmallDesc *d_desc, const int *d_level_off, int nlevels, void *stream);
 void launch_add_batch(int prec, const AddDesc *d_desc, int n, void *stream);
 // bf16 split of n complex64 values: out holds four planar n-element bf16 arrays
 // (re hi, re lo, im hi, im lo); hi = x rounded to bf16 (nearest, ties to even),
@@ -370,6 +376,11 @@ void launch_planar_merge(const float *re, const float *im, int64_t n, void *dst,
 // d_ranges: n (offset, length) pairs in 16-byte units from base; one workgroup each.
 void launch_zero_batch(void *base, const int64_t *d_ranges, int n, void *stream);
 void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args, void *stream);
+// C = op(X) op(Y) by the hand-written matrix-core GEMM (b.own): plain column-major
+// operands as for the vendor BLAS; with b.ksplit > 1, `ws` holds ksplit * batch * r * s
+// elements of partials.  plain_gemm_units: C tiles of one K range.
+void launch_plain_gemm(int prec, const void *X, const void *Y, void *C, void *ws, const BlasArgs &b, void *stream);
+int64_t plain_gemm_units(int prec, const BlasArgs &b);
 // Output sampling: candidate bitstrings from Philox4x64-10 and the sequential
 // empirical-supremum rejection test over one round of candidates.
 void launch_gen_bits(char *bits, int64_t nb, int nch, uint64_t seed, int64_t j0, void *stream);

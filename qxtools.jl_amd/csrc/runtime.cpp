@@ -104,6 +104,7 @@ struct qx_plan {
   int64_t opt_kernel = -1;                  // >= 0: force that kernel wherever it can run
   int64_t opt_chains = 1;                   // fold runs of gate-block steps into chain passes
   int64_t opt_batch_last = 0;               // batched leg slowest in bitstring-dependent values
+  int64_t opt_small_walk = 0;               // small sections' levels in one launch (k_small_walk)
   bool finalized = false;
 
   std::vector<DevBuf> data;                 // per key
@@ -125,6 +126,10 @@ struct qx_plan {
     void *arena = nullptr;                  // arena base the descriptors point into
     std::vector<int> index;                 // per step: descriptor index, or -1
     std::vector<int> count;                 // per step that starts a level: steps in the level
+    // one-workgroup walk of a section's levels (k_small_walk): per section, the offset
+    // of its level table in `walk` (device ints) and its level count (0: not walked)
+    DevBuf walk;
+    int walk_at[3] = {0, 0, 0}, walk_levels[3] = {0, 0, 0};
   };
   std::map<const qx::Lowered *, SmallTable> small;   // per lowering (forward or pullback)
   // batched slice-free backward levels of one pullback lowering
@@ -166,7 +171,7 @@ struct qx_plan {
     grad.release();
     ampcot.release();
     ws.release();
-    for (auto &kv : small) kv.second.desc.release();
+    for (auto &kv : small) kv.second.desc.release(), kv.second.walk.release();
     for (auto &kv : bwd) kv.second.desc.release(), kv.second.adds.release();
     for (auto e : events) (void)hipEventDestroy(e);
     for (auto e : tune_events) (void)hipEventDestroy(e);
@@ -255,7 +260,7 @@ rocblas_handle blas_handle(qx_ctx *c) {
 // Create the BLAS handle before a capture when any of `steps` needs it.
 void blas_before_capture(qx_ctx *c, const std::vector<qx::Step> &steps) {
   for (auto &st : steps)
-    if (st.kern == qx::Kern::Blas) {
+    if (st.kern == qx::Kern::Blas && !st.blas.own) {
       blas_handle(c);
       return;
     }
@@ -263,24 +268,30 @@ void blas_before_capture(qx_ctx *c, const std::vector<qx::Step> &steps) {
 
 // C = op(X) op(Y) by rocBLAS (strided batched, no copies): the step's operands are
 // plain column-major matrices (see qx::BlasArgs).
-void launch_blas(qx_plan *P, const qx::BlasArgs &b, const void *A, const void *B, void *C) {
+void launch_blas(qx_plan *P, const qx::BlasArgs &b, const void *A, const void *B, void *C, void *part = nullptr) {
   if (b.ttgt) {
-    // plain copies after the split planes: gather A and B, multiply, gather C
+    // plain copies after the split planes: gather A and B, multiply, gather C (the
+    // hand-written GEMM's K-split partials after the copies)
     auto up = [](int64_t bytes) { return (bytes + 255) / 256 * 256; };
     const int64_t es = (int64_t)P->esize();
     char *w = (char *)P->ws.p + (b.split ? up(8 * b.ex) + up(8 * b.ey) + 2 * up(4 * b.ec) : 0);
     char *pa = w, *pb = pa + up(b.ta * es), *pc = pb + up(b.tb * es);
+    char *pp = pc + up(b.r * b.s * b.batch * es);
     hipStream_t stream = P->ctx->stream;
     qx::launch_gather(P->prec, A, pa, b.nga, b.ga, stream);
     qx::launch_gather(P->prec, B, pb, b.ngb, b.gb, stream);
     qx::BlasArgs inner = b;
     inner.ttgt = 0;
-    launch_blas(P, inner, pa, pb, pc);
+    launch_blas(P, inner, pa, pb, pc, pp);
     qx::launch_gather(P->prec, pc, C, b.ngc, b.gc, stream);
     return;
   }
-  rocblas_handle h = blas_handle(P->ctx);
   const void *X = b.swap ? B : A, *Y = b.swap ? A : B;
+  if (b.own) {
+    qx::launch_plain_gemm(P->prec, X, Y, C, part ? part : P->ws.p, b, P->ctx->stream);
+    return;
+  }
+  rocblas_handle h = blas_handle(P->ctx);
   const rocblas_operation ox = b.opx ? rocblas_operation_transpose : rocblas_operation_none;
   const rocblas_operation oy = b.opy ? rocblas_operation_transpose : rocblas_operation_none;
   rocblas_status r;
@@ -419,8 +430,71 @@ void prepare_small(qx_plan *P, const qx::Lowered &L, const Bases &bs) {
     hip_check(hipMemcpy(T.desc.p, d.data(), d.size() * sizeof(qx::SmallDesc), hipMemcpyHostToDevice),
               "copy descriptors");
   }
+  // level tables of the sections whose small steps are few enough for one workgroup:
+  // at most 2^22 multiply-adds and 4096 chunks in all (the levels' descriptors are
+  // contiguous, in level order)
+  std::vector<int> walk;
+  for (int sec = 0; sec < 3; ++sec) {
+    T.walk_at[sec] = (int)walk.size();
+    T.walk_levels[sec] = 0;
+    std::vector<int> offs;
+    double macs = 0;
+    int chunks = 0;
+    bool other_seen = false;   // a step of the section outside the levels, before a level
+    for (size_t s = 0; s < L.steps.size(); ++s) {
+      if (L.steps[s].section != sec) continue;
+      if (L.steps[s].level < 0) {
+        other_seen = true;
+        continue;
+      }
+      if (T.count[s] == 0) continue;
+      if (other_seen || (!offs.empty() && offs.back() != T.index[s])) {
+        offs.clear();   // levels of a section are contiguous; anything else keeps one launch per level
+        break;
+      }
+      if (offs.empty()) offs.push_back(T.index[s]);
+      for (int i = T.index[s]; i < T.index[s] + T.count[s]; ++i) {
+        macs += double(d[i].c1 - d[i].c0) * double(int64_t(1) << d[i].nbk);
+        ++chunks;
+      }
+      offs.push_back(T.index[s] + T.count[s]);
+    }
+    if (offs.size() >= 2 && macs <= double(int64_t(1) << 22) && chunks <= 4096) {
+      walk.insert(walk.end(), offs.begin(), offs.end());
+      T.walk_levels[sec] = (int)offs.size() - 1;
+    }
+  }
+  if (!walk.empty()) {
+    T.walk.reserve(walk.size() * sizeof(int));
+    hip_check(hipMemcpy(T.walk.p, walk.data(), walk.size() * sizeof(int), hipMemcpyHostToDevice), "copy level table");
+  }
   T.arena = bs.arena;
   P->drop_graphs();
+}
+
+// Whether section `sec` of L runs its small levels as one walk, and that launch.
+bool walks(const qx_plan *P, const qx_plan::SmallTable &T, int sec) {
+  return P->opt_small_walk && T.walk_levels[sec] > 0;
+}
+
+void run_small_walk(qx_plan *P, const qx::Lowered &L, size_t s, int sec, bool timed = true) {
+  hipStream_t stream = P->ctx->stream;
+  const qx_plan::SmallTable &T = P->small.at(&L);
+  size_t ev = 0;
+  timed = timed && P->opt_profile;
+  if (timed) {
+    ev = P->event_step.size();
+    while (P->events.size() < 2 * (ev + 1)) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      P->events.push_back(e);
+    }
+    P->event_step.push_back((int)s);
+    hip_check(hipEventRecord(P->events[2 * ev], stream), "hipEventRecord");
+  }
+  qx::launch_small_walk(P->prec, (const qx::SmallDesc *)T.desc.p, (const int *)T.walk.p + T.walk_at[sec],
+                        T.walk_levels[sec], stream);
+  if (timed) hip_check(hipEventRecord(P->events[2 * ev + 1], stream), "hipEventRecord");
 }
 
 // Launch the level of batched small steps that starts at step s.
@@ -666,11 +740,19 @@ void enqueue_amplitudes(qx_plan *P, const qx::Lowered &L, uint64_t fused, int64_
   qx::launch_zero(cb.acc, nb * R * 32, stream);
   const size_t nv = P->prog.vars.size();
   std::vector<int64_t> varval(nv, 0);
+  const qx_plan::SmallTable &ST = P->small.at(&L);
   auto run_section = [&](int sec) {
+    bool walked = false;
     for (size_t s = 0; s < L.steps.size(); ++s) {
       if (L.steps[s].section != sec) continue;
-      if (L.steps[s].level >= 0) {   // batched small steps: one launch per level
-        if (P->small.at(&L).count[s] > 0) run_small_level(P, L, s, timed);
+      if (L.steps[s].level >= 0) {   // batched small steps: one launch per level, or one walk
+        if (ST.count[s] == 0) continue;
+        if (!walks(P, ST, sec)) {
+          run_small_level(P, L, s, timed);
+        } else if (!walked) {
+          run_small_walk(P, L, s, sec, timed);
+          walked = true;
+        }
         continue;
       }
       if (tuner)
@@ -851,7 +933,7 @@ void amplitudes_noalloc(qx_plan *P, const char *bits, int64_t nb, int64_t s0, in
   if (sm == P->small.end() || sm->second.arena != P->arena.p || sm->second.index.size() != L.steps.size())
     throw qx::Error(QX_ERR_STATE, std::string("small-step tables not built for this call") + warm);
   for (auto &st : L.steps)
-    if (st.kern == qx::Kern::Blas && !P->ctx->blas)
+    if (st.kern == qx::Kern::Blas && !st.blas.own && !P->ctx->blas)
       throw qx::Error(QX_ERR_STATE, std::string("library GEMM handle not created") + warm);
   if (!P->ctx->scratch) throw qx::Error(QX_ERR_STATE, "context has no scratch area");
   // per-call buffers, 256-byte aligned, in the scratch area
@@ -989,7 +1071,7 @@ const char *kern_name(const qx::Step &st) {
     case qx::Kern::Reduce: return "reduce";
     case qx::Kern::Rows: return "rows";
     case qx::Kern::Chain: return "chain";
-    case qx::Kern::Blas: return "blas";
+    case qx::Kern::Blas: return st.blas.own ? "plain_gemm" : "blas";
     default: return "bits";
   }
 }
@@ -1015,6 +1097,11 @@ std::string kern_variant(const qx::Step &st) {
     case qx::Kern::Radix: return "k_contract_radix";
     case qx::Kern::Chain: return "k_contract_chain";
     case qx::Kern::Blas:
+      if (st.blas.own)
+        return std::string(st.blas.split ? "k_plain_gemm_bf16x3<" : "k_plain_gemm<") + (st.blas.opx ? "T" : "N") +
+               (st.blas.opy ? "T" : "N") +
+               (st.blas.swap ? ",BA" : ",AB") + (st.blas.ksplit > 1 ? ",ks" + std::to_string(st.blas.ksplit) : "") +
+               ">" + (st.blas.ttgt ? "+gather" : "");
       return std::string("rocblas_gemm_strided_batched<") + (st.blas.opx ? "T" : "N") + (st.blas.opy ? "T" : "N") +
              (st.blas.swap ? ",BA>" : ",AB>") + (st.blas.ttgt ? "+gather" : "");
     default: return "k_contract_bits";
@@ -1197,11 +1284,19 @@ void vjp(qx_plan *P, const char *bits, int64_t nb, const double *amp_cot, double
   std::vector<int64_t> varval(nv, 0);
   prepare_small(P, L, bs);
   auto run_fwd = [&](int sec) {
+    const qx_plan::SmallTable &ST = P->small.at(&L);
+    bool walked = false;
     for (size_t s = 0; s < L.steps.size(); ++s) {
       const qx::Step &st = L.steps[s];
       if (st.section != sec) continue;
-      if (st.level >= 0) {   // batched small steps: one launch per level
-        if (P->small.at(&L).count[s] > 0) run_small_level(P, L, s);
+      if (st.level >= 0) {   // batched small steps: one launch per level, or one walk
+        if (ST.count[s] == 0) continue;
+        if (!walks(P, ST, sec)) {
+          run_small_level(P, L, s);
+        } else if (!walked) {
+          run_small_walk(P, L, s, sec);
+          walked = true;
+        }
         continue;
       }
       launch_step(P, L, st, value_ptr(P, L.values[st.in_a], varval, bs),
@@ -1418,11 +1513,16 @@ std::string stats_json(const qx_plan *P) {
       double flops = 8.0 * st.H * st.M * st.N * st.K;
       double bytes = (double)P->esize() * st.H * (st.M * st.K + st.K * st.N + st.M * st.N);
       if (st.kern == qx::Kern::Chain) flops = st.chain_flops, bytes = (double)P->esize() * st.chain_elems;
+      std::string variant = kern_variant(st);
+      if (st.level >= 0 && P->opt_small_walk) {   // levels walked by one workgroup
+        auto it = P->small.find(&L);
+        if (it != P->small.end() && it->second.walk_levels[st.section] > 0) variant = "k_small_walk";
+      }
       os << "{\"name\":\"" << P->prog.nodes[st.node].name << "\",\"section\":" << st.section
          << ",\"fused_steps\":" << (st.kern == qx::Kern::Chain ? st.chain.nsteps : 1)
          << ",\"B\":" << st.H << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K
          << ",\"flops\":" << flops << ",\"bytes\":" << bytes << ",\"kernel\":\""
-         << kern_name(st) << "\",\"variant\":\"" << kern_variant(st) << "\",\"config\":\"" << cand_label(st)
+         << kern_name(st) << "\",\"variant\":\"" << variant << "\",\"config\":\"" << cand_label(st)
          << "\"";
       if (P->opt_profile && &L == P->last_lowered && s < P->step_ms.size())
         os << ",\"ms\":" << P->step_ms[s] << ",\"calls\":" << P->step_calls[s];
@@ -1567,7 +1667,7 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
   return guard([&] {
     if (!plan || !name) throw qx::Error(QX_ERR_ARGUMENT, "null argument");
     std::string n(name);
-    bool runtime_opt = (n == "profile" || n == "graph" || n == "fuse_slices" || n == "autotune");
+    bool runtime_opt = (n == "profile" || n == "graph" || n == "fuse_slices" || n == "autotune" || n == "small_walk");
     if (plan->finalized && !runtime_opt) throw qx::Error(QX_ERR_STATE, "option '" + n + "' must be set before finalize");
     if (n == "batch") {
       if (value <= 0 || (value & (value - 1)) || value > (1 << 20))
@@ -1586,8 +1686,11 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
       plan->opt_chains = value;
     } else if (n == "batch_last") {
       plan->opt_batch_last = value;
+    } else if (n == "small_walk") {
+      plan->opt_small_walk = value;
+      plan->drop_graphs();
     } else if (n == "kernel") {
-      if (value < -1 || value > 10) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..10");
+      if (value < -1 || value > 11) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..11");
       plan->opt_kernel = value;
     } else if (n == "profile") {
       // may be toggled at any time; every set clears the accumulated step times

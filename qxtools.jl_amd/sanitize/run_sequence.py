@@ -33,6 +33,24 @@ def sliced_plan():
     return g.to_dsl(), g.data, simulation.amplitudes_uniform(6, 99, 8)
 
 
+def plain_program(nh, nm, nn, nk, layout, seed=0):
+    """One ncon of plain (batched) column-major matrices in transpose / order variant
+    `layout` (bit 0: A stored K x M, bit 1: B stored N x K, bit 2: C stored N x M)."""
+    import numpy as np
+
+    labs = list(range(1, nh + nm + nn + nk + 1))
+    H, M, N, K = labs[:nh], labs[nh:nh + nm], labs[nh + nm:nh + nm + nn], labs[nh + nm + nn:]
+    al = (K + M if layout & 1 else M + K) + H
+    bl = (N + K if layout & 2 else K + N) + H
+    cl = (N + M if layout & 4 else M + N) + H
+    rng = np.random.default_rng(seed)
+    fmt = lambda ls: ",".join(map(str, ls))
+    dsl = "\n".join(["# version: 0.4.0", f"load A data_1 {fmt([2] * len(al))}", f"load B data_2 {fmt([2] * len(bl))}",
+                     f"ncon C {fmt(cl)} A {fmt(al)} B {fmt(bl)}", "save output C"]) + "\n"
+    data = {"data_1": rng.normal(size=[2] * len(al)) + 0j, "data_2": rng.normal(size=[2] * len(bl)) + 0j}
+    return dsl, data, [""]
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("workload")
@@ -45,8 +63,13 @@ def main() -> int:
     ap.add_argument("--precision", default=None)
     ap.add_argument("--vjp", action="store_true")
     ap.add_argument("--noalloc", action="store_true")
+    ap.add_argument("--kernel", type=int, default=None, help="force a kernel choice (plan option)")
     a = ap.parse_args()
-    if a.workload == "sliced":
+    if a.workload.startswith("plain:"):   # plain:nh,nm,nn,nk,layout
+        nh, nm, nn, nk, layout = (int(x) for x in a.workload[6:].split(","))
+        dsl, data, bits = plain_program(nh, nm, nn, nk, layout)
+        prec = a.precision or "c128"
+    elif a.workload == "sliced":
         dsl, data, bits = sliced_plan()
         prec = a.precision or "c128"
     else:
@@ -59,7 +82,7 @@ def main() -> int:
     lib.qx_sim_launches.restype = C.c_int64
     batch = a.batch or min(256, len(bits))
     ex = Executor(dsl, data, precision=prec, batch=batch, fuse_slices=a.fuse_slices, autotune=a.autotune,
-                  graph=a.graph)
+                  graph=a.graph, kernel=a.kernel)
     print(f"finalized {a.workload}: {ex.num_slices} slices, precision {prec}, batch {batch}", flush=True)
     for i in range(a.calls):
         t0 = time.time()

@@ -488,6 +488,19 @@ void launch_small_batch(int prec, const SmallDesc *d_desc, int n, void *stream) 
   });
 }
 
+void launch_small_walk(int prec, const SmallDesc *d_desc, const int *d_level_off, int nlevels, void *stream) {
+  launch(stream, [=]() {
+    need(d_level_off, (long double)(nlevels + 1) * 4, "level table");
+    for (int L = 0; L < nlevels; ++L)
+      if (d_level_off[L + 1] < d_level_off[L]) violation("level table not increasing");
+  });
+  if (nlevels > 0)   // the chunks themselves: as a batch of every level's descriptors
+    submit((hipStream_t)stream, [=]() {
+      const int n = d_level_off[nlevels] - d_level_off[0];
+      launch_small_batch(prec, d_desc + d_level_off[0], n, nullptr);
+    });
+}
+
 void launch_add_batch(int prec, const AddDesc *d_desc, int n, void *stream) {
   const size_t es = esize(prec);
   launch(stream, [=]() {
@@ -560,6 +573,28 @@ void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const C
     need(Xm, rout, "chain Xm");
     need(ws, packed * es, "chain packed gate blocks");
     for (auto &p : s) need(p.first, p.second, "chain gate block");
+  });
+}
+
+int64_t plain_gemm_units(int prec, const BlasArgs &b) {
+  const int tm = prec == QX_C128 || b.split ? 64 : 128, tn = prec == QX_C128 || prec == QX_C64_3M || b.split ? 64 : 128;
+  return b.batch * ((b.r + tm - 1) / tm) * ((b.s + tn - 1) / tn);
+}
+
+void launch_plain_gemm(int prec, const void *X, const void *Y, void *C, void *ws, const BlasArgs &b, void *stream) {
+  const size_t es = esize(prec);
+  if (b.r <= 0 || b.s <= 0 || b.k <= 0 || b.batch <= 0 || b.ksplit < 1 || b.k % ((b.split ? 32 : 16) * b.ksplit) ||
+      (b.split && prec == QX_C128))
+    violation("plain GEMM sizes out of range");
+  const long double rx = mat_reach(b.opx, b.r, b.k, b.ldx, b.sx, b.batch) * es;
+  const long double ry = mat_reach(b.opy, b.k, b.s, b.ldy, b.sy, b.batch) * es;
+  const long double rc = mat_reach(false, b.r, b.s, b.ldc, b.sc, b.batch) * es;
+  const long double rw = b.ksplit > 1 ? (long double)b.ksplit * b.batch * b.r * b.s * es : 0;
+  launch(stream, [=]() {
+    need(X, rx, "plain GEMM X");
+    need(Y, ry, "plain GEMM Y");
+    need(C, rc, "plain GEMM C");
+    need(ws, rw, "plain GEMM K-split partials");
   });
 }
 

@@ -70,3 +70,32 @@ def test_vjp_graph_replays_match_eager(planner, nslice, nb, batch):
         for k in want:
             assert np.array_equal(got[k], want[k]), (rep, k)
         _ = graphed.amplitudes(bits)             # forward graph between vjp replays
+
+
+@pytest.mark.parametrize("name", ["rqc_3x3_d16", "rqc_5x5_d20"])
+def test_small_level_walk_matches_level_launches(name):
+    """The small steps' dependency levels walked by one workgroup in one launch
+    (k_small_walk, plan option small_walk) give the one-launch-per-level results bit for
+    bit (same chunks, same arithmetic), eagerly and on graph replay, and the pullback too."""
+    from qxhip import workloads
+
+    dsl, data, bits = workloads.WORKLOADS[name].load()
+    bits = bits[:64]
+    walk = Executor(dsl, data, batch=64, graph=1, small_walk=1)
+    levels = Executor(dsl, data, batch=64)
+    try:
+        want = levels.amplitudes(bits)
+        for _ in range(3):
+            assert np.array_equal(walk.amplitudes(bits), want)
+        kinds = {s["variant"] for s in next(b for b in walk.stats()["batches"] if b["active"])["steps"]}
+        assert "k_small_walk" in kinds
+        kinds = {s["variant"] for s in next(b for b in levels.stats()["batches"] if b["active"])["steps"]}
+        assert "k_small_walk" not in kinds and "k_small_batch" in kinds
+        if name == "rqc_3x3_d16":
+            cot = np.linspace(0.5, 1.5, len(bits)) + 0.25j
+            gw, gl = walk.vjp(bits, cot), levels.vjp(bits, cot)
+            for k in gl:
+                assert np.array_equal(gw[k], gl[k]), k
+    finally:
+        walk.close()
+        levels.close()

@@ -147,9 +147,9 @@ def test_gemm_mfma_c128(case):
 FORCE_SHAPES = [(0, 12, 5, 5), (1, 7, 5, 10), (0, 10, 2, 3), (2, 8, 2, 2), (0, 2, 2, 8), (0, 6, 6, 6)]
 
 
-@pytest.mark.parametrize("choice,prec", [(c, "c128") for c in range(9)] + [(5, "c64_4m"), (5, "c64_3m"),
-                                                                           (7, "c64_4m"), (7, "c64_3m"),
-                                                                           (8, "c64_4m")])
+@pytest.mark.parametrize("choice,prec", [(c, "c128") for c in list(range(9)) + [11]] +
+                         [(5, "c64_4m"), (5, "c64_3m"), (7, "c64_4m"), (7, "c64_3m"), (8, "c64_4m"),
+                          (11, "c64_4m"), (11, "c64_3m")])
 @pytest.mark.parametrize("shape", FORCE_SHAPES)
 def test_forced_kernel_matches_oracle(shape, choice, prec):
     """Every kernel the autotuner may pick computes the same contraction: each
@@ -206,6 +206,60 @@ def test_rocblas_plain_layouts(shape, layout, prec):
     assert err(out, ref) < (5e-6 if prec != "c128" else 1e-12)
 
 
+@pytest.mark.parametrize("shape,layout", [((0, 10, 10, 10), l) for l in range(8)] +
+                         [((1, 10, 11, 10), 3), ((1, 10, 10, 11), 4)])
+def test_plain_gemm_split_bf16_layouts(shape, layout):
+    """The hand-written split bf16 product (M, N, K >= 1024) on every transpose /
+    operand-order combination (1024^3: the K range split over two workgroups per tile),
+    with a batch leg and an uneven tile count: normwise 1e-4 against the oracle (the
+    dropped lo.lo term and the split residual are ~2^-16 of each product)."""
+    dsl, data, ref = plain_step(970 + layout + 10 * sum(shape), *shape, layout & 1, layout >> 1 & 1, layout >> 2)
+    ex = Executor(dsl, data, precision="c64_bf16x3")
+    out = np.asarray(ex.amplitudes([""])).reshape(-1)
+    st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][0]
+    ex.close()
+    assert st["variant"].startswith("k_plain_gemm_bf16x3"), st["variant"]
+    assert err(out, ref) < 1e-4
+
+
+@pytest.mark.parametrize("layout", range(8))
+@pytest.mark.parametrize("shape", PLAIN_SHAPES + [(0, 4, 4, 12), (3, 4, 5, 4), (0, 5, 4, 13)])
+@pytest.mark.parametrize("prec,tol", [("c128", 1e-12), ("c64_4m", 5e-6), ("c64_3m", 2e-5)])
+def test_plain_gemm_layouts(shape, layout, prec, tol):
+    """The hand-written matrix-core GEMM (kernel 11) on every transpose / operand-order
+    combination of plain matrix layouts, with partial C tiles (16 or 32 rows or columns
+    against tiles of 64 or 128) and long K ranges split over workgroups (4096 and 8192
+    values into 256 or 512 outputs); complex128 4M, complex64 4M and 3M."""
+    dsl, data, ref = plain_step(950 + layout + 10 * sum(shape), *shape, layout & 1, layout >> 1 & 1, layout >> 2)
+    ex = Executor(dsl, data, precision=prec, kernel=11)
+    out = np.asarray(ex.amplitudes([""])).reshape(-1)
+    st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][0]
+    ex.close()
+    assert st["kernel"] == "plain_gemm", st["config"]
+    assert err(out, ref) < tol
+
+
+def test_plain_gemm_split_k_in_graph_replays():
+    """A plain GEMM whose K range is split over workgroups (partials summed by a second
+    kernel) captured into the call's graph: every replay equals the eager result bit
+    for bit, with another plan's eager calls on the device between the replays."""
+    dsl, data, ref = plain_step(81, 0, 5, 4, 13, 0, 1, 0)
+    eager = Executor(dsl, data, kernel=11)
+    want = np.asarray(eager.amplitudes([""])).reshape(-1)
+    st = next(b for b in eager.stats()["batches"] if b["active"])["steps"][0]
+    assert ",ks" in st["variant"], st["variant"]
+    graphed = Executor(dsl, data, kernel=11, graph=1)
+    other_dsl, other_data, other_ref = plain_step(82, 1, 7, 6, 8, 1, 0, 1)
+    other = Executor(other_dsl, other_data)
+    for rep in range(3):
+        got = np.asarray(graphed.amplitudes([""])).reshape(-1)
+        assert np.array_equal(got, want), f"replay {rep}"
+        assert err(np.asarray(other.amplitudes([""])).reshape(-1), other_ref) < 1e-12
+    assert err(want, ref) < 1e-12
+    for ex in (eager, graphed, other):
+        ex.close()
+
+
 def test_rocblas_step_in_graph_replays():
     """A rocBLAS step captured into the call's graph: every replay equals the oracle."""
     dsl, data, ref = plain_step(77, 1, 7, 6, 8, 1, 0, 1)
@@ -240,21 +294,22 @@ def test_split_bf16_product_in_graph_replays():
                                               (False, "c128", True)])
 def test_regrouped_contraction_matches_oracle(view, prec, a_data):
     """A large contraction whose operands the lowering regroups into plain matrices
-    (complex128: run by rocBLAS), also around a data operand's fixed layout; with
-    sliced operands the layouts stay as written."""
+    (run by the hand-written plain GEMM), also around a data operand's fixed layout;
+    with sliced operands the layouts stay as written."""
     import qxtest
     dsl, data, ref = qxtest.interleaved_gemm_program(view_a=view, a_data=a_data)
     ex = Executor(dsl, data, precision=prec)
     out = np.asarray(ex.amplitudes([""])).reshape(-1)
     st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][-1]
     ex.close()
-    assert (st["kernel"] == "blas") == (prec == "c128" and not view)
+    assert (st["kernel"] == "plain_gemm") == (not view)
     assert err(out, ref) < (5e-6 if prec != "c128" else 1e-12)
 
 
 def test_split_bf16_product_matches_oracle():
     """c64_bf16x3 on a 1024^3 contraction: three bf16 products of the hi / lo split,
-    accumulated in FP32.  The dropped lo.lo term and the split residual are ~2^-16 of
+    accumulated in FP32, on the hand-written bf16 matrix-core kernel (and on the vendor
+    BLAS as the tuned alternative).  The dropped lo.lo term and the split residual are ~2^-16 of
     each product, so the normwise error stays near 1e-5 (bar: 1e-4), against 1e-6-ish
     for the FP32 4M product of the same step."""
     import qxtest
@@ -263,8 +318,15 @@ def test_split_bf16_product_matches_oracle():
     out = np.asarray(ex.amplitudes([""])).reshape(-1)
     st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][-1]
     ex.close()
-    assert st["kernel"] == "blas"
+    assert st["kernel"] == "plain_gemm" and st["variant"].startswith("k_plain_gemm_bf16x3"), st["variant"]
     e_split = err(out, ref)
+    # the same split product on the vendor BLAS (planes in the workspace, 12 library GEMMs)
+    ex = Executor(dsl, data, precision="c64_bf16x3", kernel=8)
+    lib = np.asarray(ex.amplitudes([""])).reshape(-1)
+    st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][-1]
+    ex.close()
+    assert st["kernel"] == "blas"
+    assert err(lib, ref) < 1e-4 and err(out, lib) < 2e-5
     assert e_split < 1e-4
     ex = Executor(dsl, data, precision="c64_4m")
     e_4m = err(np.asarray(ex.amplitudes([""])).reshape(-1), ref)
@@ -309,11 +371,13 @@ def test_skinny_contraction_matches_oracle(shape, prec):
 @pytest.mark.parametrize("prec,tol", [("c128", 1e-12), ("c64_4m", 5e-6), ("c64_bf16x3", 1e-4)])
 def test_gathered_library_gemm_matches_oracle(prec, tol):
     """A 2^34-flop step with shuffled leg orders (kernel 10): A and B gathered into plain
-    copies, the library GEMM (the split bf16 product for c64_bf16x3), C gathered back."""
+    copies, the hand-written plain GEMM (the split bf16 product for c64_bf16x3), C
+    gathered back."""
     dsl, data, ref = single_step(811, 0, 10, 10, 11)
     ex = Executor(dsl, data, precision=prec, kernel=10)
     out = np.asarray(ex.amplitudes([""])).reshape(-1)
     st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][0]
     ex.close()
-    assert st["kernel"] == "blas" and st["variant"].endswith("+gather")
+    assert st["kernel"] == "plain_gemm" and st["variant"].endswith("+gather"), st["variant"]
+    assert ("bf16x3" in st["variant"]) == (prec == "c64_bf16x3")
     assert err(out, ref) < tol

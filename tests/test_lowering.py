@@ -116,8 +116,8 @@ def test_sections_split_shared_and_bitstring_work():
     secs = [s["section"] for s in info["steps"]]
     assert secs.count(2) < 40 < secs.count(0)   # outputs absorbed last by the sweep plan
     big = [s for s in info["steps"] if s["section"] == 2 and s["K"] >= 4096]
-    # the state is regrouped into a plain K x M matrix, so the library GEMM runs it
-    assert big and big[0]["N"] == 256 and big[0]["kernel"] == "blas"
+    # the state is regrouped into a plain K x M matrix, so the hand-written plain GEMM runs it
+    assert big and big[0]["N"] == 256 and big[0]["kernel"] == "plain_gemm"
 
 
 def test_kernel_choice_matches_gpu_test_shapes():
@@ -199,15 +199,19 @@ def test_batched_small_levels_do_not_share_memory():
 
 
 def test_split_bf16_precision_takes_large_plain_steps():
-    """c64_bf16x3 runs plain steps with M, N, K >= 1024 as split bf16 products (the
-    library GEMM with hi/lo planes); smaller or non-plain steps keep the complex64
-    kernels, and c64_4m never takes the library GEMM by default."""
+    """c64_bf16x3 runs plain steps with M, N, K >= 1024 as split bf16 products on the
+    hand-written bf16 matrix-core GEMM; smaller steps keep the complex64 products, and
+    no precision takes the vendor library by default (it stays a tuned candidate)."""
     import qxtest
     dsl, _, _ = qxtest.interleaved_gemm_program(nk=10, nm=10, nn=10)
-    assert _native.inspect(dsl, precision="c64_bf16x3")["steps"][-1]["kernel"] == "blas"
-    assert _native.inspect(dsl, precision="c64_4m")["steps"][-1]["kernel"] != "blas"
+    st = _native.inspect(dsl, precision="c64_bf16x3")["steps"][-1]
+    assert st["kernel"] == "plain_gemm" and st["variant"].startswith("k_plain_gemm_bf16x3")
+    assert any(c.startswith("rocblas") for c in st["candidates"])
+    st = _native.inspect(dsl, precision="c64_4m")["steps"][-1]
+    assert st["kernel"] == "plain_gemm" and not st["variant"].startswith("k_plain_gemm_bf16x3")
     dsl, _, _ = qxtest.interleaved_gemm_program(nk=11, nm=8, nn=8)   # M = N = 256
-    assert _native.inspect(dsl, precision="c64_bf16x3")["steps"][-1]["kernel"] != "blas"
+    st = _native.inspect(dsl, precision="c64_bf16x3")["steps"][-1]
+    assert st["kernel"] != "blas" and "bf16x3" not in st["variant"]
 
 
 def test_plain_matrix_steps_offer_rocblas():
@@ -223,12 +227,12 @@ def test_plain_matrix_steps_offer_rocblas():
 
 def test_large_contraction_operands_regrouped_into_plain_matrices():
     """The lowering regroups the interleaved legs of a large contraction's operands
-    (each produced for that contraction alone) into plain matrices, so the library
-    GEMM runs it; a viewed operand keeps its layout, and the step its tile kernel."""
+    (each produced for that contraction alone) into plain matrices, so the hand-written
+    plain GEMM runs it; a viewed operand keeps its layout, and the step its tile kernel."""
     import qxtest
     dsl, _, _ = qxtest.interleaved_gemm_program()
     last = _native.inspect(dsl)["steps"][-1]
-    assert last["kernel"] == "blas", last["config"]
+    assert last["kernel"] == "plain_gemm", last["config"]
     # A's contracted legs form one dense run and its free legs another
     ka = sorted(l[1] for l in last["klegs"])
     assert ka == [2 ** i * ka[0] for i in range(len(ka))]
@@ -236,14 +240,14 @@ def test_large_contraction_operands_regrouped_into_plain_matrices():
     assert ma == [2 ** i * ma[0] for i in range(len(ma))]
     dsl, _, _ = qxtest.interleaved_gemm_program(view_a=True)
     last = _native.inspect(dsl)["steps"][-1]
-    assert last["kernel"] != "blas"
+    assert last["kernel"] not in ("blas", "plain_gemm")
     assert not any(x.startswith("rocblas") for x in last["candidates"])
     # too small for the library GEMM: left in DSL order
     dsl, _, _ = qxtest.interleaved_gemm_program(nk=6, nm=6, nn=6)
-    assert _native.inspect(dsl)["steps"][-1]["kernel"] != "blas"
+    assert _native.inspect(dsl)["steps"][-1]["kernel"] not in ("blas", "plain_gemm")
     # a data operand keeps its layout (its shuffled K order): the intermediate follows it
     dsl, _, _ = qxtest.interleaved_gemm_program(a_data=True)
-    assert _native.inspect(dsl)["steps"][-1]["kernel"] == "blas"
+    assert _native.inspect(dsl)["steps"][-1]["kernel"] == "plain_gemm"
 
 
 def _lds_groups(swz, bits, nb):

@@ -45,7 +45,11 @@ def _run(env, *args):
     ("sliced", "--precision", "c64_bf16x3", "--autotune", "1", "--graph", "1"),
     ("rqc_6x6_d12_s6", "--autotune", "1", "--graph", "1", "--bits", "64"),
     ("rqc_7x7_d12_s10_open5", "--fuse-slices", "0", "--autotune", "1", "--graph", "1", "--calls", "3"),
-], ids=["config1", "config2", "sliced-looped", "sliced-bf16", "6x6", "7x7-looped"])
+    ("plain:0,4,4,12,3", "--kernel", "11", "--graph", "1"),
+    ("plain:2,6,5,7,5", "--kernel", "11", "--precision", "c64_3m"),
+    ("plain:1,7,6,8,6", "--kernel", "11", "--precision", "c64_4m", "--autotune", "1"),
+], ids=["config1", "config2", "sliced-looped", "sliced-bf16", "6x6", "7x7-looped", "plain-splitk", "plain-3m",
+        "plain-4m-tuned"])
 def test_call_sequences_under_sanitizers(san_env, args):
     out = _run(san_env, *args)
     assert "launches checked" in out
