@@ -1702,7 +1702,11 @@ __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const t
   const int t = threadIdx.x;
   uint32_t kmask = 0, hmask = 0;
   for (int i = 0; i < LK; ++i) kmask |= 1u << s.q[i];
-  for (int i = 0; i < s.h; ++i) hmask |= 1u << s.hq[i];
+  // fixed trip counts: the step's fields load in one batch of scalar loads (a loop over
+  // a runtime count loads and waits per field, and each wait also drains the LDS queue)
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (i < s.h) hmask |= 1u << s.hq[i];
   const int nfree = kChainTileBits - LK - s.h;
   auto koff = [&](int kap) {
     uint32_t o = 0;
@@ -1721,8 +1725,11 @@ __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const t
         if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, b));
         ++j;
       }
-    for (int i = 0; i < s.h; ++i, ++j)
-      if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, s.hq[i]));
+    for (int i = 0; i < 3; ++i)
+      if (i < s.h) {
+        if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, s.hq[i]));
+        ++j;
+      }
     return base;
   };
   auto srow = [&](uint32_t g) {
@@ -1768,13 +1775,20 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
   constexpr int NG = kChainTile / K, NB = K / 16, KB = K / 4;
   constexpr int WAVES = kChainThreads / 64, RT = (NG / 16) * NB / WAVES;
   static_assert(RT == 4, "row blocks per wave");
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the lane from the exec mask (v_mbcnt) rather than the work-item id: the compiler can
+  // recompute it where needed instead of keeping a lane value live (and spilling it)
+  // across the whole tile loop
+  const int lane = (int)__lane_id(), wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x) >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int nb = wave % NB;
   const int g0 = (wave / NB) * RT * 16;
   uint32_t kmask = 0, hmask = 0;
   for (int i = 0; i < LK; ++i) kmask |= 1u << s.q[i];
-  for (int i = 0; i < s.h; ++i) hmask |= 1u << s.hq[i];
+  // fixed trip counts: the step's fields load in one batch of scalar loads (a loop over
+  // a runtime count loads and waits per field, and each wait also drains the LDS queue)
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (i < s.h) hmask |= 1u << s.hq[i];
   const int nfree = kChainTileBits - LK - s.h;
   auto koff = [&](int kap) {
     uint32_t o = 0;
@@ -1791,8 +1805,11 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
         if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, b));
         ++j;
       }
-    for (int i = 0; i < s.h; ++i, ++j)
-      if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, s.hq[i]));
+    for (int i = 0; i < 3; ++i)
+      if (i < s.h) {
+        if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, s.hq[i]));
+        ++j;
+      }
     return base;
   };
   // the low four group bits (a row inside a 16-row block) go to the first four
@@ -1800,8 +1817,9 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
   uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
   {
     int j = 0;
-    for (int b = 0; b < kChainTileBits && j < 4; ++b)
-      if (!(((kmask | hmask) >> b) & 1)) {
+#pragma unroll
+    for (int b = 0; b < kChainTileBits; ++b)
+      if (j < 4 && !(((kmask | hmask) >> b) & 1)) {
         const uint32_t bit = tile_img<SWZ>(swz, b);
         if (j == 0) p0 = bit;
         else if (j == 1) p1 = bit;
@@ -1905,6 +1923,7 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   const int64_t ntiles = int64_t(1) << a.nouter;
   auto in_base = [&](int64_t w) {
     int64_t bi = 0;
+#pragma unroll 4
     for (int j = 0; j < a.nouter; ++j)
       if ((w >> j) & 1) bi += a.oin[j];
     return bi;
@@ -1913,26 +1932,30 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   // the top four from r
   constexpr int RB = 4;
   static_assert(kChainPer == 1 << RB && kChainThreads == 1 << (kChainTileBits - RB), "tile split");
-  int64_t lin = 0, lout = 0;
-  uint32_t sin = 0, sout = 0;   // LDS addresses of the lane's element r = 0 (load / store order)
+  // offsets of the lane's element r = 0 in X_0 and X_m (sums of distinct power-of-two
+  // strides of a tensor of fewer than 2^32 elements, which the host guarantees: 32 bits
+  // keep these registers, live across every step, few) and its LDS addresses (load /
+  // store order)
+  uint32_t lin = 0, lout = 0;
+  uint32_t sin = 0, sout = 0;
   for (int b = 0; b < kChainTileBits - RB; ++b)
     if ((t >> b) & 1) {
-      lin += a.win[b];
-      lout += a.wout[a.pout[b]];
+      lin += (uint32_t)a.win[b];
+      lout += (uint32_t)a.wout[a.pout[b]];
       sin = tile_cmb<SWZ>(sin, tile_img<SWZ>(a.swz, b));
       sout = tile_cmb<SWZ>(sout, tile_img<SWZ>(a.swz, a.pout[b]));
     }
   // offsets of element r of this lane: the top RB tile bits come from r (a
   // compile-time constant, so these fold to sums of kernel arguments)
   auto rin = [&](auto r) {
-    int64_t o = lin;
+    int64_t o = (int64_t)lin;
 #pragma unroll
     for (int i = 0; i < RB; ++i)
       if ((r >> i) & 1) o += a.win[kChainTileBits - RB + i];
     return o;
   };
   auto rout = [&](auto r) {
-    int64_t o = lout;
+    int64_t o = (int64_t)lout;
 #pragma unroll
     for (int i = 0; i < RB; ++i)
       if ((r >> i) & 1) o += a.wout[a.pout[kChainTileBits - RB + i]];
@@ -2008,6 +2031,7 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
       }
     }
     int64_t bo = 0;
+#pragma unroll 4
     for (int j = 0; j < a.nouter; ++j)
       if ((w >> j) & 1) bo += a.oout[j];
     E *Xme = reinterpret_cast<E *>(Xm);

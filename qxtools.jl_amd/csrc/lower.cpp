@@ -722,6 +722,9 @@ bool chain_view(const Program &p, const Lowered &L, const Step &st, ChainView &c
   cv.big = cv.x_is_a ? st.in_a : st.in_b;
   cv.small = cv.x_is_a ? st.in_b : st.in_a;
   if (std::min(ea, eb) > kChainMaxS || std::max(ea, eb) < (int64_t(1) << 14)) return false;
+  // the chain kernel keeps a lane's offsets inside the tile in 32 bits: they are sums
+  // of distinct power-of-two strides of the large tensor, below its extent
+  if (std::max(ea, eb) >= (int64_t(1) << 32)) return false;
   for (auto &g : st.cl) {
     if (g.dim != 2) return false;
     const int64_t sx = cv.x_is_a ? g.sa : g.sb, ss = cv.x_is_a ? g.sb : g.sa;
