@@ -1694,44 +1694,38 @@ __device__ __forceinline__ uint32_t tile_cmb(uint32_t x, uint32_t y) {
   return SWZ ? x ^ y : x | y;
 }
 
-template <typename T, int K, bool SWZ>
+// Per-step image table (kChainTabInts int32 per step in the workspace, written by
+// k_chain_pack for the launch's layout): entry e < 13 - k is the LDS image of group
+// bit e (the tile bits outside the step's contracted and batch bits in increasing
+// order, then its batch bits), entry 16 + i the image of contracted bit i.  Images of
+// distinct bits combine by XOR in either layout (plain images are distinct powers of
+// two), so the step code needs no per-step bit scans.
+//
+// Group g's element kappa sits at D(g) ^ koff(kappa), D(g) the XOR of the images of
+// g's set bits.  The FMA path: lane t owns groups t + kChainThreads i, whose D splits
+// into the lane part D(t) and the uniform part of i.
+template <typename T, int K>
 __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const typename Vec<T>::type *__restrict__ Sp,
-                                            const ChainStepArgs &s, const int32_t *swz) {
+                                            const ChainStepArgs &s, const int32_t *__restrict__ tb) {
   using V = typename Vec<T>::type;
   constexpr int LK = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : K == 16 ? 4 : 5;
+  constexpr int TB = kChainTileBits - 4;   // group bits from the lane (kChainThreads = 2^TB)
+  static_assert(kChainThreads == 1 << TB, "lane bits");
   const int t = threadIdx.x;
-  uint32_t kmask = 0, hmask = 0;
-  for (int i = 0; i < LK; ++i) kmask |= 1u << s.q[i];
-  // fixed trip counts: the step's fields load in one batch of scalar loads (a loop over
-  // a runtime count loads and waits per field, and each wait also drains the LDS queue)
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    if (i < s.h) hmask |= 1u << s.hq[i];
   const int nfree = kChainTileBits - LK - s.h;
+  uint32_t kimg[LK > 0 ? LK : 1];
+#pragma unroll
+  for (int i = 0; i < LK; ++i) kimg[i] = (uint32_t)tb[16 + i];
   auto koff = [&](int kap) {
     uint32_t o = 0;
 #pragma unroll
     for (int i = 0; i < LK; ++i)
-      if ((kap >> i) & 1) o = tile_cmb<SWZ>(o, tile_img<SWZ>(swz, s.q[i]));
+      if ((kap >> i) & 1) o ^= kimg[i];
     return o;
   };
-  // group index bits -> tile bits outside q: other positions first, batch bits last
-  // (the LDS address of the group's element kappa = 0)
-  auto deposit = [&](uint32_t g) {
-    uint32_t base = 0;
-    int j = 0;
-    for (int b = 0; b < kChainTileBits; ++b)
-      if (!(((kmask | hmask) >> b) & 1)) {
-        if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, b));
-        ++j;
-      }
-    for (int i = 0; i < 3; ++i)
-      if (i < s.h) {
-        if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, s.hq[i]));
-        ++j;
-      }
-    return base;
-  };
+  uint32_t dt = 0;
+#pragma unroll
+  for (int b = 0; b < TB; ++b) dt ^= ((t >> b) & 1) ? (uint32_t)tb[b] : 0u;
   auto srow = [&](uint32_t g) {
     int h = (int)(g >> nfree) & ((1 << s.h) - 1);
     h = __builtin_amdgcn_readfirstlane(h);
@@ -1739,35 +1733,59 @@ __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const t
   };
   static_assert(K <= kChainPer, "larger steps run on matrix cores");
   // whole groups per lane, updated in place
+#pragma unroll
   for (int i = 0; i < kChainPer / K; ++i) {
     const uint32_t g = (uint32_t)(t + kChainThreads * i);
-    const uint32_t base = deposit(g);
+    uint32_t base = dt;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if ((i >> b) & 1) base ^= (uint32_t)tb[TB + b];
     const V *Sr = srow(g);
     V in[K];
 #pragma unroll
-    for (int kap = 0; kap < K; ++kap) in[kap] = tile[tile_cmb<SWZ>(base, koff(kap))];
+    for (int kap = 0; kap < K; ++kap) in[kap] = tile[base ^ koff(kap)];
+    // the real and imaginary parts accumulate as two partial sums each: four
+    // independent FMA chains per output value
 #pragma unroll
     for (int nu = 0; nu < K; ++nu) {
-      V acc{0, 0};
+      T xa = 0, xb = 0, ya = 0, yb = 0;
 #pragma unroll
-      for (int kap = 0; kap < K; ++kap) cmac<T>(acc, Sr[nu * K + kap], in[kap], 0, 0);
-      tile[tile_cmb<SWZ>(base, koff(nu))] = acc;
+      for (int kap = 0; kap < K; ++kap) {
+        const V sv = Sr[nu * K + kap];
+        xa = fma(sv.x, in[kap].x, xa);
+        xb = fma(sv.y, in[kap].y, xb);
+        ya = fma(sv.x, in[kap].y, ya);
+        yb = fma(sv.y, in[kap].x, yb);
+      }
+      V o;
+      o.x = xa - xb;
+      o.y = ya + yb;
+      tile[base ^ koff(nu)] = o;
     }
   }
   lds_barrier();
 }
 
 // Steps with 16 or 32 contracted values on matrix cores: out[g][nu] = sum_kappa
-// in[g][kappa] S[kappa -> nu] as 16x16x4 blocks (rows = groups, columns = nu), the
-// complex product as four real products.  Wave w owns four row blocks of one nu
-// block; its 64 groups share the batch bits, so its S fragments (B operands: S at
-// nu = lane % 16, kappa = 4 kb + lane / 16) stay in registers for the whole step.
-// Operand and result layouts as in Mfma16.
-template <typename T, int K, bool SL, bool SWZ>
+// S[kappa -> nu] in[g][kappa], as 16x16x4 products of S^T (A operand: nu = lane % 16,
+// kappa = 4 kb + lane / 16) and in^T (B operand: kappa = 4 kb + lane / 16, group =
+// lane % 16), so that the result's columns are groups: 16 lanes with consecutive
+// lane % 16 read and write 16 groups that differ in the low group bits, the pattern
+// the tile swizzle spreads over the LDS banks (with nu on the columns, the writes
+// would spread over the contracted bits, which it cannot always spread).  Wave w owns
+// four blocks of 16 groups for one block of 16 nu; its 64 groups share the batch
+// bits, so its S fragments stay in registers for the whole step.  Operand and result
+// layouts as in Mfma16.  Addresses from the step's image table (as in chain_apply):
+// a group block's base from the uniform group bits 4 and up, a group inside it from
+// group bits 0-3.
+// G3: the complex product as three real products (Gauss: t1 = (ar + ai) br,
+// t2 = ar (bi - br), t3 = ai (br + bi); re = t1 - t3, im = t1 + t2), a quarter fewer
+// matrix-core ops for one vector add per A element.
+template <typename T, int K, bool SL, bool G3 = false>
 __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
                                                  const typename Vec<T>::type *__restrict__ Sp,
                                                  const typename Vec<T>::type *sS, const ChainStepArgs &s,
-                                                 const int32_t *swz) {
+                                                 const int32_t *__restrict__ tb) {
   using V = typename Vec<T>::type;
   using M = Mfma16<T>;
   using v4 = typename M::v4;
@@ -1782,59 +1800,34 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
   const int li = lane & 15, lk = lane >> 4;
   const int nb = wave % NB;
   const int g0 = (wave / NB) * RT * 16;
-  uint32_t kmask = 0, hmask = 0;
-  for (int i = 0; i < LK; ++i) kmask |= 1u << s.q[i];
-  // fixed trip counts: the step's fields load in one batch of scalar loads (a loop over
-  // a runtime count loads and waits per field, and each wait also drains the LDS queue)
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    if (i < s.h) hmask |= 1u << s.hq[i];
   const int nfree = kChainTileBits - LK - s.h;
+  uint32_t kimg[LK];
+#pragma unroll
+  for (int i = 0; i < LK; ++i) kimg[i] = (uint32_t)tb[16 + i];
   auto koff = [&](int kap) {
     uint32_t o = 0;
 #pragma unroll
     for (int i = 0; i < LK; ++i)
-      if ((kap >> i) & 1) o = tile_cmb<SWZ>(o, tile_img<SWZ>(swz, s.q[i]));
+      if ((kap >> i) & 1) o ^= kimg[i];
     return o;
   };
-  auto deposit = [&](uint32_t g) {
-    uint32_t base = 0;
-    int j = 0;
-    for (int b = 0; b < kChainTileBits; ++b)
-      if (!(((kmask | hmask) >> b) & 1)) {
-        if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, b));
-        ++j;
-      }
-    for (int i = 0; i < 3; ++i)
-      if (i < s.h) {
-        if ((g >> j) & 1) base = tile_cmb<SWZ>(base, tile_img<SWZ>(swz, s.hq[i]));
-        ++j;
-      }
-    return base;
-  };
-  // the low four group bits (a row inside a 16-row block) go to the first four
-  // free tile bits: a row's base is its block's base plus this scatter
-  uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
-  {
-    int j = 0;
-#pragma unroll
-    for (int b = 0; b < kChainTileBits; ++b)
-      if (j < 4 && !(((kmask | hmask) >> b) & 1)) {
-        const uint32_t bit = tile_img<SWZ>(swz, b);
-        if (j == 0) p0 = bit;
-        else if (j == 1) p1 = bit;
-        else if (j == 2) p2 = bit;
-        else p3 = bit;
-        ++j;
-      }
-  }
   auto scatter4 = [&](int row) {
-    return tile_cmb<SWZ>(tile_cmb<SWZ>((row & 1) ? p0 : 0u, (row & 2) ? p1 : 0u),
-                         tile_cmb<SWZ>((row & 4) ? p2 : 0u, (row & 8) ? p3 : 0u));
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if ((row >> b) & 1) o ^= (uint32_t)tb[b];
+    return o;
   };
+  // uniform part of the wave's row blocks: group bits 6 and up from g0 (a multiple of
+  // 64), bits 4-5 from the row block
+  uint32_t dw = 0;
+#pragma unroll
+  for (int b = 6; b < kChainTileBits - LK; ++b)
+    if ((g0 >> b) & 1) dw ^= (uint32_t)tb[b];
   uint32_t rbase[RT];
 #pragma unroll
-  for (int rt = 0; rt < RT; ++rt) rbase[rt] = deposit((uint32_t)(g0 + 16 * rt));
+  for (int rt = 0; rt < RT; ++rt)
+    rbase[rt] = dw ^ ((rt & 1) ? (uint32_t)tb[4] : 0u) ^ ((rt & 2) ? (uint32_t)tb[5] : 0u);
   const int h = (g0 >> nfree) & ((1 << s.h) - 1);   // the wave's groups share their batch bits
   const int so = s.soff + h * K * K + (16 * nb + li) * K + lk;
   T bre[KB], bim[KB];
@@ -1846,50 +1839,97 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
     bim[kb] = v.y;
   }
   const uint32_t arow = scatter4(li);
-  v4 cre[RT], cim[RT];
-#pragma unroll
-  for (int rt = 0; rt < RT; ++rt) cre[rt] = v4{0, 0, 0, 0}, cim[rt] = v4{0, 0, 0, 0};
-  // k blocks outer, row blocks inner: the 2 RT accumulators are independent, so
-  // consecutive matrix-core ops never wait for each other's results
-#pragma unroll
-  for (int kb = 0; kb < KB; ++kb) {
-    V a[RT];
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
-      a[rt] = tile[tile_cmb<SWZ>(tile_cmb<SWZ>(rbase[rt], arow), koff(4 * kb + lk))];
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) cre[rt] = M::mma(a[rt].x, bre[kb], cre[rt]);
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) cim[rt] = M::mma(a[rt].x, bim[kb], cim[rt]);
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) cre[rt] = M::mma(-a[rt].y, bim[kb], cre[rt]);
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) cim[rt] = M::mma(a[rt].y, bre[kb], cim[rt]);
-  }
   // waves of other nu blocks read these groups: all reads before any write (with
   // one nu block every wave owns its groups alone)
-  if (NB > 1) lds_barrier();
-  const uint32_t ocol = koff(16 * nb + li);
-#pragma unroll
-  for (int rt = 0; rt < RT; ++rt)
+  auto store = [&](int rt, const v4 &re, const v4 &im) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       V o;
-      o.x = cre[rt][r];
-      o.y = cim[rt][r];
-      tile[tile_cmb<SWZ>(tile_cmb<SWZ>(rbase[rt], scatter4(M::row(lk, r))), ocol)] = o;
+      o.x = re[r];
+      o.y = im[r];
+      tile[rbase[rt] ^ arow ^ koff(16 * nb + M::row(lk, r))] = o;
     }
+  };
+  if constexpr (!G3) {
+    v4 cre[RT], cim[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) cre[rt] = v4{0, 0, 0, 0}, cim[rt] = v4{0, 0, 0, 0};
+    // k blocks outer, row blocks inner: the 2 RT accumulators are independent, so
+    // consecutive matrix-core ops never wait for each other's results
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      V a[RT];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) a[rt] = tile[rbase[rt] ^ arow ^ koff(4 * kb + lk)];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) cre[rt] = M::mma(bre[kb], a[rt].x, cre[rt]);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) cim[rt] = M::mma(bim[kb], a[rt].x, cim[rt]);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) cre[rt] = M::mma(bim[kb], -a[rt].y, cre[rt]);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) cim[rt] = M::mma(bre[kb], a[rt].y, cim[rt]);
+    }
+    if (NB > 1) lds_barrier();
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) store(rt, cre[rt], cim[rt]);
+  } else {
+    static_assert(NB == 1, "three-product form for one nu block");
+    // two row blocks at a time (three accumulators each keep the registers of the
+    // four-product form): t1, t2, t3 as above, 6 independent accumulators per k block
+#pragma unroll
+    for (int half = 0; half < RT / 2; ++half) {
+      v4 c1[2], c2[2], c3[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) c1[u] = c2[u] = c3[u] = v4{0, 0, 0, 0};
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        const T bd = bim[kb] - bre[kb], bs = bre[kb] + bim[kb];
+        V a[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) a[u] = tile[rbase[2 * half + u] ^ arow ^ koff(4 * kb + lk)];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) c1[u] = M::mma(bre[kb], a[u].x + a[u].y, c1[u]);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) c2[u] = M::mma(bd, a[u].x, c2[u]);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) c3[u] = M::mma(bs, a[u].y, c3[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) store(2 * half + u, c1[u] - c3[u], c1[u] + c2[u]);
+    }
+  }
   lds_barrier();
 }
 
 // Gather every S_j of a chain into the dense [h][nu][kappa] blocks the chain
 // kernel reads: step j = blockIdx.y, elements over blockIdx.x (at most kChainMaxS).
+// The first workgroup of each step also writes the step's image table (at
+// a.tab_off elements, kChainTabInts int32 per step) for the launch's layout.
 template <typename T>
 __global__ void k_chain_pack(typename Vec<T>::type *__restrict__ Sp, ChainArgs a) {
   using V = typename Vec<T>::type;
   {
     const int j = blockIdx.y;
     const ChainStepArgs &s = a.st[j];
+    if (blockIdx.x == 0 && threadIdx.x < kChainTabInts) {
+      const int e = threadIdx.x;
+      uint32_t used = 0;
+      for (int i = 0; i < s.k; ++i) used |= 1u << s.q[i];
+      for (int i = 0; i < s.h; ++i) used |= 1u << s.hq[i];
+      int bit = -1;
+      if (e < kChainTileBits - s.k - s.h) {   // e-th tile bit outside q and hq
+        int c = 0;
+        for (int b = 0; b < kChainTileBits && bit < 0; ++b)
+          if (!((used >> b) & 1) && c++ == e) bit = b;
+      } else if (e < kChainTileBits - s.k) {
+        bit = s.hq[e - (kChainTileBits - s.k - s.h)];
+      } else if (e >= 16 && e < 16 + s.k) {
+        bit = s.q[e - 16];
+      }
+      const int32_t img = bit < 0 ? 0 : a.swizzled ? a.swz[bit] : (int32_t)(1u << bit);
+      reinterpret_cast<int32_t *>(Sp + a.tab_off)[kChainTabInts * j + e] = img;
+    }
     const V *S = (const V *)a.S[j];
     const int kk = 1 << s.k, tot = (1 << s.h) * kk * kk;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += gridDim.x * blockDim.x) {
@@ -1906,8 +1946,9 @@ __global__ void k_chain_pack(typename Vec<T>::type *__restrict__ Sp, ChainArgs a
   }
 }
 
-// SWZ: LDS addresses through the host's swizzle table (a separate instance: the
-// plain one keeps the cheaper addressing).
+// SWZ: tile loads and stores address LDS through the host's swizzle table (a separate
+// instance: the plain one keeps the cheaper addressing); the steps take their
+// addresses from the image tables, which k_chain_pack writes for the launch's layout.
 template <typename T, bool SWZ>
 __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename Vec<T>::type *__restrict__ X0,
                                                                   typename Vec<T>::type *__restrict__ Xm,
@@ -1918,6 +1959,7 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   __shared__ V sS[kChainSLds<T>];   // packed S_j of every step, when they fit
   const int t = threadIdx.x;
   const bool s_lds = a.s_elems <= kChainSLds<T>;
+  const int32_t *__restrict__ tab = reinterpret_cast<const int32_t *>(Sp + a.tab_off);
   if (s_lds)
     for (int e = t; e < a.s_elems; e += kChainThreads) sS[e] = Sp[e];
   const int64_t ntiles = int64_t(1) << a.nouter;
@@ -2009,24 +2051,29 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
         for (int q = 0; q < 4; ++q)
           if (q * n / 4 == j) load_quarter(q, bn);
       const ChainStepArgs &s = a.st[j];
+      const int32_t *tb = tab + kChainTabInts * j;
       switch (s.k) {
-        case 0: chain_apply<T, 1, SWZ>(tile, Sp, s, a.swz); break;
-        case 1: chain_apply<T, 2, SWZ>(tile, Sp, s, a.swz); break;
-        case 2: chain_apply<T, 4, SWZ>(tile, Sp, s, a.swz); break;
-        case 3: chain_apply<T, 8, SWZ>(tile, Sp, s, a.swz); break;
+        case 0: chain_apply<T, 1>(tile, Sp, s, tb); break;
+        case 1: chain_apply<T, 2>(tile, Sp, s, tb); break;
+        case 2: chain_apply<T, 4>(tile, Sp, s, tb); break;
+        case 3: chain_apply<T, 8>(tile, Sp, s, tb); break;
         case 4:
+          // mfma16: 0 vector FMAs, 1 matrix cores (four real products), 2 matrix cores
+          // (three real products, when the gate blocks are staged in LDS)
           if (!a.mfma16)
-            chain_apply<T, 16, SWZ>(tile, Sp, s, a.swz);
+            chain_apply<T, 16>(tile, Sp, s, tb);
+          else if (a.mfma16 == 2 && s_lds)
+            chain_apply_mfma<T, 16, true, true>(tile, Sp, sS, s, tb);
           else if (s_lds)
-            chain_apply_mfma<T, 16, true, SWZ>(tile, Sp, sS, s, a.swz);
+            chain_apply_mfma<T, 16, true>(tile, Sp, sS, s, tb);
           else
-            chain_apply_mfma<T, 16, false, SWZ>(tile, Sp, sS, s, a.swz);
+            chain_apply_mfma<T, 16, false>(tile, Sp, sS, s, tb);
           break;
         default:
           if (s_lds)
-            chain_apply_mfma<T, 32, true, SWZ>(tile, Sp, sS, s, a.swz);
+            chain_apply_mfma<T, 32, true>(tile, Sp, sS, s, tb);
           else
-            chain_apply_mfma<T, 32, false, SWZ>(tile, Sp, sS, s, a.swz);
+            chain_apply_mfma<T, 32, false>(tile, Sp, sS, s, tb);
           break;
       }
     }

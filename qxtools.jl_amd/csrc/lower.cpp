@@ -542,10 +542,13 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
     for (int j = 0; j < st.chain.nsteps; ++j) k16 |= st.chain.st[j].k == 4;
     bool swz = false;   // a non-trivial swizzle exists
     for (int b = 0; b < kChainTileBits; ++b) swz |= st.chain.swz[b] != (1 << b);
+    // m: steps with 16 contracted values on vector FMAs (0) or matrix cores with four
+    // (1) or three (2, gate blocks staged in LDS) real products per complex product
+    const bool staged = st.chain.s_elems * (precision == QX_C128 ? 16 : 8) <= (32 << 10);
     for (int grid : {256, 512, 1024})
-      for (int m : {0, 1})
+      for (int m : {0, 1, 2})
         for (int z : {0, 1}) {
-          if ((m && !k16) || (z && !swz)) continue;
+          if ((m && !k16) || (m == 2 && !staged) || (z && !swz)) continue;
           if (grid == st.chain.grid && m == st.chain.mfma16 && z == st.chain.swizzled) continue;
           Step v = st;
           v.chain.grid = grid;
@@ -855,6 +858,8 @@ Step chain_step(const Program &p, const Lowered &L, const ChainBuild &cb) {
     elems += (double)value_extent(p, L.values[s.small]);
   }
   a.s_elems = (int)st.ws_elems;
+  a.tab_off = (int)((st.ws_elems + kChainTabElems - 1) / kChainTabElems * kChainTabElems);
+  st.ws_elems = a.tab_off + (int64_t)kChainTabElems * a.nsteps;
   elems += (double)value_extent(p, L.values[cb.x0]) + (double)last.c_elems;
   st.chain_flops = flops;
   st.chain_elems = elems;
@@ -956,9 +961,9 @@ std::vector<int> form_chains(const Program &p, Lowered &L) {
 // bank group of i is the linear map L(i) = i_low ^ sum G[b] i_b over GF(2), and a
 // cycle's lanes, which differ in nb tile bits, are conflict-free when L maps those bits
 // to independent vectors.  A cycle's lanes differ in: the low tile bits (loads), the
-// fastest X_m tile bits (stores), each step's lowest free tile bits (its groups) and,
-// for steps with 16 or 32 contracted values, their first contracted bits (matrix-core
-// result columns).  G is 0 on the bits of a lane's element index r (tile bits 9-12 for
+// fastest X_m tile bits (stores) and each step's lowest free tile bits (its groups, in
+// both the vector and the matrix-core step, whose result columns are groups).  G is 0
+// on the bits of a lane's element index r (tile bits 9-12 for
 // loads, the last four X_m tile bits for stores) so those offsets stay additive; it
 // minimises the summed conflict degree (weights: 2 per step, 1 per load / store pass)
 // by hill climbing from fixed seeds.
@@ -981,7 +986,6 @@ void chain_swizzle(ChainArgs &a, int elem_bytes) {
     for (int b = 0; b < kChainTileBits && (int)v.size() < nb; ++b)
       if (!((used >> b) & 1)) v.push_back(b);
     sets.push_back({v, 2});
-    if (c.k >= 4) sets.push_back({std::vector<int>(c.q, c.q + std::min(nb, c.k)), 2});
   }
   std::vector<char> fixed(kChainTileBits, 0);   // G must stay 0
   for (int b = 0; b < kChainTileBits; ++b) fixed[b] = b < nb || b >= kChainTileBits - 4;

@@ -146,6 +146,11 @@ constexpr int kChainTileBits = 13;
 constexpr int kChainMaxSteps = 32;
 constexpr int kChainMinPassive = 3;   // untouched fast legs kept in the tile
 constexpr int kChainMaxS = 1024;      // elements of one staged small operand
+// Per-step image tables in the workspace after the packed gate blocks: kChainTabInts
+// int32 per step (LDS images of the step's group and contracted bits), reserved as
+// kChainTabElems workspace elements per step (128 bytes: 16 complex64 or 8 complex128).
+constexpr int kChainTabInts = 32;
+constexpr int kChainTabElems = 16;
 
 // The fields the chain kernel reads are 32-bit: it indexes steps with a uniform loop
 // counter, and dword-aligned fields load through the scalar unit (narrower ones would
@@ -160,8 +165,10 @@ struct ChainStepArgs {
 struct ChainArgs {
   int nsteps = 0, nouter = 0;
   int grid = 256;          // workgroups (each walks tiles); tuned
-  int mfma16 = 0;          // steps with 16 contracted values on matrix cores (else vector FMAs)
+  int mfma16 = 0;          // steps with 16 contracted values: 0 vector FMAs, 1 matrix cores,
+                           // 2 matrix cores with three real products per complex product
   int s_elems = 0;         // elements of all packed S_j (staged in LDS when they fit)
+  int tab_off = 0;         // workspace element offset of the per-step image tables
   int64_t win[kChainTileBits] = {0}, wout[kChainTileBits] = {0};   // tile bit -> X_0 / X_m stride
   int32_t pout[kChainTileBits] = {0};  // store order: i-th fastest X_m tile bit -> tile bit
   // LDS address of tile bit b: element i sits at the XOR of swz[b] over its set bits.

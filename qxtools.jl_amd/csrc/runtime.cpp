@@ -105,6 +105,8 @@ struct qx_plan {
   int64_t opt_chains = 1;                   // fold runs of gate-block steps into chain passes
   int64_t opt_batch_last = 0;               // batched leg slowest in bitstring-dependent values
   int64_t opt_small_walk = 0;               // small sections' levels in one launch (k_small_walk)
+  int64_t opt_chain_variant = -1;           // >= 0: chain passes' 16-value path (bits 0-1)
+                                            // and swizzled tiles (bit 2), for testing
   bool finalized = false;
 
   std::vector<DevBuf> data;                 // per key
@@ -181,6 +183,18 @@ struct qx_plan {
 
 namespace {
 
+// The chain-pass variant the `chain_variant` option names: the path of steps with 16
+// contracted values (bits 0-1: vector FMAs, matrix cores with four or with three real
+// products) and the swizzled tile addressing (bit 2, where the pass has a non-trivial
+// swizzle).
+void force_chain_variant(qx::Step &st, int64_t v) {
+  if (st.kern != qx::Kern::Chain) return;
+  st.chain.mfma16 = (int)(v & 3);
+  bool swz = false;
+  for (int b = 0; b < qx::kChainTileBits; ++b) swz |= st.chain.swz[b] != (1 << b);
+  st.chain.swizzled = (v & 4) && swz ? 1 : 0;
+}
+
 const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch, uint64_t fused = 0) {
   auto key = std::make_pair(batch, fused);
   auto it = P->lowered.find(key);
@@ -189,6 +203,8 @@ const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch, uint64_t fused = 0)
   if (P->opt_kernel >= 0)
     for (auto &st : L.steps)
       if (qx::force_kernel(st, P->prec, (int)P->opt_kernel)) L.ws_elems = std::max(L.ws_elems, st.ws_elems);
+  if (P->opt_chain_variant >= 0)
+    for (auto &st : L.steps) force_chain_variant(st, P->opt_chain_variant);
   size_t need = (size_t)L.arena_elems * P->esize();
   if (P->arena_cap_bytes > 0 && (int64_t)need > P->arena_cap_bytes)
     throw qx::Error(QX_ERR_MEMORY, "arena of " + std::to_string(need >> 20) + " MiB for batch " +
@@ -547,7 +563,7 @@ std::string cand_label(const qx::Step &st) {
   std::string v = kern_variant(st);
   if (st.swapped) v += "/swap";
   if (st.kern == qx::Kern::Chain)
-    v += "/g" + std::to_string(st.chain.grid) + (st.chain.mfma16 ? "/mfma" : "/fma") + (st.chain.swizzled ? "/swz" : "");
+    v += "/g" + std::to_string(st.chain.grid) + (st.chain.mfma16 == 2 ? "/mfma3" : st.chain.mfma16 ? "/mfma" : "/fma") + (st.chain.swizzled ? "/swz" : "");
   return v;
 }
 
@@ -1196,6 +1212,8 @@ const qx::VjpLowered &ensure_vjp(qx_plan *P, int64_t batch, uint64_t fused) {
     for (auto &sec : V.bsteps)
       for (auto &b : sec) force(b.step);
   }
+  if (P->opt_chain_variant >= 0)
+    for (auto &st : V.fwd.steps) force_chain_variant(st, P->opt_chain_variant);
   const size_t es = P->esize();
   size_t need = (size_t)(V.fwd.arena_elems + V.cot_elems + V.temp_elems) * es;
   if (P->arena_cap_bytes > 0 && (int64_t)need > P->arena_cap_bytes)
@@ -1689,6 +1707,10 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
     } else if (n == "small_walk") {
       plan->opt_small_walk = value;
       plan->drop_graphs();
+    } else if (n == "chain_variant") {
+      if (value < -1 || value > 7 || (value & 3) == 3)
+        throw qx::Error(QX_ERR_ARGUMENT, "chain_variant must be -1 (heuristic), 0..2 or 4..6");
+      plan->opt_chain_variant = value;
     } else if (n == "kernel") {
       if (value < -1 || value > 11) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..11");
       plan->opt_kernel = value;

@@ -568,10 +568,14 @@ void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const C
     packed = std::max(packed, (long double)t.soff + std::ldexp(1.0L, t.h + 2 * t.k));
   }
   if (packed > a.s_elems) violation("chain: packed gate blocks beyond s_elems");
+  if (a.tab_off < a.s_elems) violation("chain: step tables overlap the packed gate blocks");
+  if ((int64_t)kChainTabInts * 4 > (int64_t)kChainTabElems * (int64_t)es)
+    violation("chain: step table larger than its workspace share");
   launch(stream, [=]() {
     need(X0, rin, "chain X0");
     need(Xm, rout, "chain Xm");
     need(ws, packed * es, "chain packed gate blocks");
+    need((const char *)ws + (size_t)a.tab_off * es, (long double)kChainTabInts * 4 * a.nsteps, "chain step tables");
     for (auto &p : s) need(p.first, p.second, "chain gate block");
   });
 }

@@ -80,3 +80,22 @@ def test_chain_gate_runs(name, precision, tol):
     plain = Executor(dsl, data, precision=precision, batch=1, chains=0)
     outp = np.asarray(plain.amplitudes([""])).reshape(-1)
     assert np.linalg.norm(out - outp) / np.linalg.norm(outp) < tol
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 4, 5, 6])
+@pytest.mark.parametrize("precision,tol", [("c128", 1e-12), ("c64_4m", 2e-5)])
+def test_chain_variants(variant, precision, tol):
+    """Every path of the 16-value steps (vector FMAs, matrix cores with four or three
+    real products per complex product) with plain and swizzled tile addressing, on
+    overlapping 4-leg blocks and on mixed leg counts, against the oracle."""
+    for name in ("k4 x3 overlapping", "k1-k5 mixed"):
+        dsl, data = qxtest.gate_sequence_program(16, GATE_RUNS[name], seed=len(name))
+        ex = Executor(dsl, data, precision=precision, batch=1, chain_variant=variant)
+        out = np.asarray(ex.amplitudes([""])).reshape(-1)
+        ref = O.amplitudes(dsl, data, [""])[0].reshape(-1, order="F")
+        assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol, name
+        act = [b for b in ex.stats()["batches"] if b["active"]][0]
+        chains = [s for s in act["steps"] if s["kernel"] == "chain"]
+        parts = [s["config"].split("/") for s in chains]
+        assert chains and all(("fma", "mfma", "mfma3")[variant & 3] in p for p in parts)
+        assert all(("swz" in p) == bool(variant & 4) for p in parts)
