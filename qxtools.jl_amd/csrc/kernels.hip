@@ -1662,7 +1662,7 @@ __device__ __forceinline__ void static_for(F &&f) {
 constexpr int kChainTile = 1 << kChainTileBits;
 // LDS beside the tile for the packed gate blocks: 32 KiB (160 KiB per workgroup)
 template <typename T>
-constexpr int kChainSLds = (32 << 10) / (int)sizeof(typename Vec<T>::type);
+constexpr int kChainSLds = kChainSLdsBytes / (int)sizeof(typename Vec<T>::type);
 constexpr int kChainThreads = 512;
 constexpr int kChainPer = kChainTile / kChainThreads;   // tile elements per lane (16)
 

@@ -544,7 +544,7 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
     for (int b = 0; b < kChainTileBits; ++b) swz |= st.chain.swz[b] != (1 << b);
     // m: steps with 16 contracted values on vector FMAs (0) or matrix cores with four
     // (1) or three (2, gate blocks staged in LDS) real products per complex product
-    const bool staged = st.chain.s_elems * (precision == QX_C128 ? 16 : 8) <= (32 << 10);
+    const bool staged = st.chain.s_elems * (precision == QX_C128 ? 16 : 8) <= kChainSLdsBytes;
     for (int grid : {256, 512, 1024})
       for (int m : {0, 1, 2})
         for (int z : {0, 1}) {
@@ -1389,7 +1389,16 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
   if (chains && !keep_all) {
     for (int a : form_chains(p, L)) absorbed[a] = 1;
     for (Step &st : L.steps) {
-      if (st.kern == Kern::Chain) chain_swizzle(st.chain, precision == QX_C128 ? 16 : 8);
+      if (st.kern == Kern::Chain) {
+        chain_swizzle(st.chain, precision == QX_C128 ? 16 : 8);
+        // steps with 16 contracted values on matrix cores, three real products per
+        // complex product where the gate blocks are staged in LDS (the fastest form in
+        // measurement; the autotuner also times the others)
+        bool k16 = false;
+        for (int j = 0; j < st.chain.nsteps; ++j) k16 |= st.chain.st[j].k == 4;
+        const bool staged = st.chain.s_elems * (precision == QX_C128 ? 16 : 8) <= kChainSLdsBytes;
+        if (k16) st.chain.mfma16 = staged ? 2 : 1;
+      }
       L.ws_elems = std::max(L.ws_elems, st.ws_elems);
     }
   }

@@ -146,6 +146,7 @@ constexpr int kChainTileBits = 13;
 constexpr int kChainMaxSteps = 32;
 constexpr int kChainMinPassive = 3;   // untouched fast legs kept in the tile
 constexpr int kChainMaxS = 1024;      // elements of one staged small operand
+constexpr int kChainSLdsBytes = 32 << 10;   // LDS for the packed gate blocks (beside the tile)
 // Per-step image tables in the workspace after the packed gate blocks: kChainTabInts
 // int32 per step (LDS images of the step's group and contracted bits), reserved as
 // kChainTabElems workspace elements per step (128 bytes: 16 complex64 or 8 complex128).

@@ -1175,6 +1175,10 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
         for (int i = 0; i < qx::kChainTileBits; ++i) os << (i ? "," : "") << a.pout[i];
         os << "],\"swz\":[";
         for (int i = 0; i < qx::kChainTileBits; ++i) os << (i ? "," : "") << a.swz[i];
+        os << "],\"win\":[";
+        for (int i = 0; i < qx::kChainTileBits; ++i) os << (i ? "," : "") << a.win[i];
+        os << "],\"wout\":[";
+        for (int i = 0; i < qx::kChainTileBits; ++i) os << (i ? "," : "") << a.wout[i];
         os << "],\"q\":[";
         for (int j = 0; j < a.nsteps; ++j) {
           os << (j ? "," : "") << "[";
