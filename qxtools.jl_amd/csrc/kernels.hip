@@ -1674,6 +1674,21 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// End of a chain step: a workgroup barrier, or, when every wave only touches tile
+// elements of its own (wave-local chains: the wave index sits on three tile bits no
+// step contracts or batches), an ordering point for the wave alone (one wave's LDS
+// operations complete in issue order; this keeps the compiler from moving LDS
+// accesses across it).
+__device__ __forceinline__ void step_end(bool wave_local) {
+  if (wave_local) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+  } else {
+    lds_barrier();
+  }
+}
+
 // Step j's S, packed densely as [h][nu][kappa] at Sp + a.st[j].soff, is read with
 // wave-uniform indices (the batch bits h are placed on wave-uniform group bits), so
 // its values come through the scalar cache rather than LDS.
@@ -1706,7 +1721,7 @@ __device__ __forceinline__ uint32_t tile_cmb(uint32_t x, uint32_t y) {
 // into the lane part D(t) and the uniform part of i.
 template <typename T, int K>
 __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const typename Vec<T>::type *__restrict__ Sp,
-                                            const ChainStepArgs &s, const int32_t *__restrict__ tb) {
+                                            const ChainStepArgs &s, const int32_t *__restrict__ tb, bool wl) {
   using V = typename Vec<T>::type;
   constexpr int LK = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : K == 16 ? 4 : 5;
   constexpr int TB = kChainTileBits - 4;   // group bits from the lane (kChainThreads = 2^TB)
@@ -1763,7 +1778,7 @@ __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const t
       tile[base ^ koff(nu)] = o;
     }
   }
-  lds_barrier();
+  step_end(wl);
 }
 
 // Steps with 16 or 32 contracted values on matrix cores: out[g][nu] = sum_kappa
@@ -1785,7 +1800,7 @@ template <typename T, int K, bool SL, bool G3 = false>
 __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
                                                  const typename Vec<T>::type *__restrict__ Sp,
                                                  const typename Vec<T>::type *sS, const ChainStepArgs &s,
-                                                 const int32_t *__restrict__ tb) {
+                                                 const int32_t *__restrict__ tb, bool wl) {
   using V = typename Vec<T>::type;
   using M = Mfma16<T>;
   using v4 = typename M::v4;
@@ -1899,7 +1914,7 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
       for (int u = 0; u < 2; ++u) store(2 * half + u, c1[u] - c3[u], c1[u] + c2[u]);
     }
   }
-  lds_barrier();
+  step_end(NB == 1 && wl);
 }
 
 // Gather every S_j of a chain into the dense [h][nu][kappa] blocks the chain
@@ -1918,10 +1933,23 @@ __global__ void k_chain_pack(typename Vec<T>::type *__restrict__ Sp, ChainArgs a
       for (int i = 0; i < s.k; ++i) used |= 1u << s.q[i];
       for (int i = 0; i < s.h; ++i) used |= 1u << s.hq[i];
       int bit = -1;
-      if (e < kChainTileBits - s.k - s.h) {   // e-th tile bit outside q and hq
-        int c = 0;
-        for (int b = 0; b < kChainTileBits && bit < 0; ++b)
-          if (!((used >> b) & 1) && c++ == e) bit = b;
+      const int nfree = kChainTileBits - s.k - s.h;
+      if (e < nfree) {
+        // group bits 6-8 on the wave bits of a wave-local chain, the others on the
+        // remaining free tile bits in increasing order
+        int order[kChainTileBits];
+        int n = 0;
+        uint32_t taken = used;
+        if (a.wave_local)
+          for (int i = 0; i < 3; ++i) taken |= 1u << a.wbits[i];
+        for (int b = 0; b < kChainTileBits; ++b) {
+          if (a.wave_local && n == 6)
+            for (int i = 0; i < 3; ++i) order[n++] = a.wbits[i];
+          if (!((taken >> b) & 1)) order[n++] = b;
+        }
+        if (a.wave_local && n == 6)
+          for (int i = 0; i < 3; ++i) order[n++] = a.wbits[i];
+        bit = order[e];
       } else if (e < kChainTileBits - s.k) {
         bit = s.hq[e - (kChainTileBits - s.k - s.h)];
       } else if (e >= 16 && e < 16 + s.k) {
@@ -1960,6 +1988,7 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   const int t = threadIdx.x;
   const bool s_lds = a.s_elems <= kChainSLds<T>;
   const int32_t *__restrict__ tab = reinterpret_cast<const int32_t *>(Sp + a.tab_off);
+  const bool wl = a.wave_local != 0;
   if (s_lds)
     for (int e = t; e < a.s_elems; e += kChainThreads) sS[e] = Sp[e];
   const int64_t ntiles = int64_t(1) << a.nouter;
@@ -2053,30 +2082,32 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
       const ChainStepArgs &s = a.st[j];
       const int32_t *tb = tab + kChainTabInts * j;
       switch (s.k) {
-        case 0: chain_apply<T, 1>(tile, Sp, s, tb); break;
-        case 1: chain_apply<T, 2>(tile, Sp, s, tb); break;
-        case 2: chain_apply<T, 4>(tile, Sp, s, tb); break;
-        case 3: chain_apply<T, 8>(tile, Sp, s, tb); break;
+        case 0: chain_apply<T, 1>(tile, Sp, s, tb, wl); break;
+        case 1: chain_apply<T, 2>(tile, Sp, s, tb, wl); break;
+        case 2: chain_apply<T, 4>(tile, Sp, s, tb, wl); break;
+        case 3: chain_apply<T, 8>(tile, Sp, s, tb, wl); break;
         case 4:
           // mfma16: 0 vector FMAs, 1 matrix cores (four real products), 2 matrix cores
           // (three real products, when the gate blocks are staged in LDS)
           if (!a.mfma16)
-            chain_apply<T, 16>(tile, Sp, s, tb);
+            chain_apply<T, 16>(tile, Sp, s, tb, wl);
           else if (a.mfma16 == 2 && s_lds)
-            chain_apply_mfma<T, 16, true, true>(tile, Sp, sS, s, tb);
+            chain_apply_mfma<T, 16, true, true>(tile, Sp, sS, s, tb, wl);
           else if (s_lds)
-            chain_apply_mfma<T, 16, true>(tile, Sp, sS, s, tb);
+            chain_apply_mfma<T, 16, true>(tile, Sp, sS, s, tb, wl);
           else
-            chain_apply_mfma<T, 16, false>(tile, Sp, sS, s, tb);
+            chain_apply_mfma<T, 16, false>(tile, Sp, sS, s, tb, wl);
           break;
         default:
           if (s_lds)
-            chain_apply_mfma<T, 32, true>(tile, Sp, sS, s, tb);
+            chain_apply_mfma<T, 32, true>(tile, Sp, sS, s, tb, wl);
           else
-            chain_apply_mfma<T, 32, false>(tile, Sp, sS, s, tb);
+            chain_apply_mfma<T, 32, false>(tile, Sp, sS, s, tb, wl);
           break;
       }
     }
+    // the stores read the tile in store order, across waves
+    if (wl) lds_barrier();
     int64_t bo = 0;
 #pragma unroll 4
     for (int j = 0; j < a.nouter; ++j)

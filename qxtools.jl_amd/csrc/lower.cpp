@@ -543,19 +543,23 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
     bool swz = false;   // a non-trivial swizzle exists
     for (int b = 0; b < kChainTileBits; ++b) swz |= st.chain.swz[b] != (1 << b);
     // m: steps with 16 contracted values on vector FMAs (0) or matrix cores with four
-    // (1) or three (2, gate blocks staged in LDS) real products per complex product
+    // (1) or three (2, gate blocks staged in LDS) real products per complex product;
+    // wl: wave-local steps or a workgroup barrier after every step
     const bool staged = st.chain.s_elems * (precision == QX_C128 ? 16 : 8) <= kChainSLdsBytes;
+    const int wl0 = st.chain.wave_local;
     for (int grid : {256, 512, 1024})
       for (int m : {0, 1, 2})
-        for (int z : {0, 1}) {
-          if ((m && !k16) || (m == 2 && !staged) || (z && !swz)) continue;
-          if (grid == st.chain.grid && m == st.chain.mfma16 && z == st.chain.swizzled) continue;
-          Step v = st;
-          v.chain.grid = grid;
-          v.chain.mfma16 = m;
-          v.chain.swizzled = z;
-          out.push_back(std::move(v));
-        }
+        for (int z : {0, 1})
+          for (int wl = wl0; wl >= 0; --wl) {
+            if ((m && !k16) || (m == 2 && !staged) || (z && !swz)) continue;
+            if (grid == st.chain.grid && m == st.chain.mfma16 && z == st.chain.swizzled && wl == wl0) continue;
+            Step v = st;
+            v.chain.grid = grid;
+            v.chain.mfma16 = m;
+            v.chain.swizzled = z;
+            v.chain.wave_local = wl;
+            out.push_back(std::move(v));
+          }
     return out;
   }
   if (st.kern == Kern::Radix) return out;
@@ -955,18 +959,51 @@ std::vector<int> form_chains(const Program &p, Lowered &L) {
 
 }  // namespace
 
+// Wave-local chains: when every step contracts and batches at most four tile bits
+// together, the wave index (group bits 6-8 of every step) can sit on the three lowest
+// tile bits that no step touches, so that each wave reads and writes only its own
+// tile elements from step to step.
+void chain_wave_local(ChainArgs &a) {
+  uint32_t used = 0;
+  bool ok = true;
+  for (int j = 0; j < a.nsteps; ++j) {
+    const ChainStepArgs &c = a.st[j];
+    ok &= c.k + c.h <= 4;
+    for (int i = 0; i < c.k; ++i) used |= 1u << c.q[i];
+    for (int i = 0; i < c.h; ++i) used |= 1u << c.hq[i];
+  }
+  int n = 0;
+  for (int b = 0; b < kChainTileBits && n < 3; ++b)
+    if (!((used >> b) & 1)) a.wbits[n++] = b;
+  a.wave_local = ok && n == 3 ? 1 : 0;
+}
+
+// A step's free tile bits other than the wave bits of a wave-local chain, in increasing
+// order: its group bits 0-3 (the lanes of one access cycle) sit on the first four.
+std::vector<int> step_lane_choices(const ChainArgs &a, const ChainStepArgs &c) {
+  uint32_t skip = 0;
+  for (int i = 0; i < c.k; ++i) skip |= 1u << c.q[i];
+  for (int i = 0; i < c.h; ++i) skip |= 1u << c.hq[i];
+  if (a.wave_local)
+    for (int i = 0; i < 3; ++i) skip |= 1u << a.wbits[i];
+  std::vector<int> v;
+  for (int b = 0; b < kChainTileBits; ++b)
+    if (!((skip >> b) & 1)) v.push_back(b);
+  return v;
+}
+
 // LDS bank groups: one access cycle serves kLdsCycleBytes, i.e. 2^nb lanes of
 // elem_bytes each, and those lanes collide unless their addresses differ in the low nb
 // element bits.  Element i sits at P(i) = i ^ sum_{b >= nb} G[b] i_b (G[b] < 2^nb): the
 // bank group of i is the linear map L(i) = i_low ^ sum G[b] i_b over GF(2), and a
 // cycle's lanes, which differ in nb tile bits, are conflict-free when L maps those bits
 // to independent vectors.  A cycle's lanes differ in: the low tile bits (loads), the
-// fastest X_m tile bits (stores) and each step's lowest free tile bits (its groups, in
-// both the vector and the matrix-core step, whose result columns are groups).  G is 0
-// on the bits of a lane's element index r (tile bits 9-12 for
-// loads, the last four X_m tile bits for stores) so those offsets stay additive; it
-// minimises the summed conflict degree (weights: 2 per step, 1 per load / store pass)
-// by hill climbing from fixed seeds.
+// fastest X_m tile bits (stores) and each step's group bits 0-3, its lowest free tile
+// bits other than the wave bits of a wave-local chain (in both the vector and the
+// matrix-core step, whose result columns are groups).  G is 0 on the bits of a lane's
+// element index r (tile bits 9-12 for loads, the last four X_m tile bits for stores)
+// so those offsets stay additive; it minimises the summed conflict degree (weights: 2
+// per step, 1 per load / store pass) by hill climbing from fixed seeds.
 void chain_swizzle(ChainArgs &a, int elem_bytes) {
   int nb = 0;
   while ((elem_bytes << (nb + 1)) <= kLdsCycleBytes) ++nb;
@@ -978,13 +1015,8 @@ void chain_swizzle(ChainArgs &a, int elem_bytes) {
     sets.push_back({w, 1});
   }
   for (int j = 0; j < a.nsteps; ++j) {
-    const ChainStepArgs &c = a.st[j];
-    uint32_t used = 0;
-    for (int i = 0; i < c.k; ++i) used |= 1u << c.q[i];
-    for (int i = 0; i < c.h; ++i) used |= 1u << c.hq[i];
-    std::vector<int> v;
-    for (int b = 0; b < kChainTileBits && (int)v.size() < nb; ++b)
-      if (!((used >> b) & 1)) v.push_back(b);
+    std::vector<int> v = step_lane_choices(a, a.st[j]);
+    if ((int)v.size() > nb) v.resize(nb);
     sets.push_back({v, 2});
   }
   std::vector<char> fixed(kChainTileBits, 0);   // G must stay 0
@@ -1390,6 +1422,7 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
     for (int a : form_chains(p, L)) absorbed[a] = 1;
     for (Step &st : L.steps) {
       if (st.kern == Kern::Chain) {
+        chain_wave_local(st.chain);
         chain_swizzle(st.chain, precision == QX_C128 ? 16 : 8);
         // steps with 16 contracted values on matrix cores, three real products per
         // complex product where the gate blocks are staged in LDS (the fastest form in

@@ -170,6 +170,11 @@ struct ChainArgs {
                            // 2 matrix cores with three real products per complex product
   int s_elems = 0;         // elements of all packed S_j (staged in LDS when they fit)
   int tab_off = 0;         // workspace element offset of the per-step image tables
+  // wave-local chains (every step has k + h <= 4): the wave index (group bits 6-8 of
+  // every step) sits on three tile bits no step touches, so each wave keeps to its own
+  // tile elements and the steps need no workgroup barriers
+  int wave_local = 0;
+  int32_t wbits[3] = {0, 0, 0};
   int64_t win[kChainTileBits] = {0}, wout[kChainTileBits] = {0};   // tile bit -> X_0 / X_m stride
   int32_t pout[kChainTileBits] = {0};  // store order: i-th fastest X_m tile bit -> tile bit
   // LDS address of tile bit b: element i sits at the XOR of swz[b] over its set bits.

@@ -105,8 +105,8 @@ struct qx_plan {
   int64_t opt_chains = 1;                   // fold runs of gate-block steps into chain passes
   int64_t opt_batch_last = 0;               // batched leg slowest in bitstring-dependent values
   int64_t opt_small_walk = 0;               // small sections' levels in one launch (k_small_walk)
-  int64_t opt_chain_variant = -1;           // >= 0: chain passes' 16-value path (bits 0-1)
-                                            // and swizzled tiles (bit 2), for testing
+  int64_t opt_chain_variant = -1;           // >= 0: chain passes' 16-value path (bits 0-1),
+                                            // swizzled tiles (bit 2), barriers (bit 3); tests
   bool finalized = false;
 
   std::vector<DevBuf> data;                 // per key
@@ -185,14 +185,16 @@ namespace {
 
 // The chain-pass variant the `chain_variant` option names: the path of steps with 16
 // contracted values (bits 0-1: vector FMAs, matrix cores with four or with three real
-// products) and the swizzled tile addressing (bit 2, where the pass has a non-trivial
-// swizzle).
+// products), the swizzled tile addressing (bit 2, where the pass has a non-trivial
+// swizzle) and a workgroup barrier after every step (bit 3, else wave-local steps
+// where the pass allows them).
 void force_chain_variant(qx::Step &st, int64_t v) {
   if (st.kern != qx::Kern::Chain) return;
   st.chain.mfma16 = (int)(v & 3);
   bool swz = false;
   for (int b = 0; b < qx::kChainTileBits; ++b) swz |= st.chain.swz[b] != (1 << b);
   st.chain.swizzled = (v & 4) && swz ? 1 : 0;
+  if (v & 8) st.chain.wave_local = 0;
 }
 
 const qx::Lowered &ensure_lowered(qx_plan *P, int64_t batch, uint64_t fused = 0) {
@@ -563,7 +565,8 @@ std::string cand_label(const qx::Step &st) {
   std::string v = kern_variant(st);
   if (st.swapped) v += "/swap";
   if (st.kern == qx::Kern::Chain)
-    v += "/g" + std::to_string(st.chain.grid) + (st.chain.mfma16 == 2 ? "/mfma3" : st.chain.mfma16 ? "/mfma" : "/fma") + (st.chain.swizzled ? "/swz" : "");
+    v += "/g" + std::to_string(st.chain.grid) + (st.chain.mfma16 == 2 ? "/mfma3" : st.chain.mfma16 ? "/mfma" : "/fma") + (st.chain.swizzled ? "/swz" : "") +
+         (st.chain.wave_local ? "/wl" : "");
   return v;
 }
 
@@ -1171,7 +1174,8 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
       os << "]";
       if (st.kern == qx::Kern::Chain) {   // tile layout: store order, LDS swizzle, each step's tile bits
         const qx::ChainArgs &a = st.chain;
-        os << ",\"chain_tile\":{\"swizzled\":" << a.swizzled << ",\"pout\":[";
+        os << ",\"chain_tile\":{\"swizzled\":" << a.swizzled << ",\"wave_local\":" << a.wave_local << ",\"wbits\":["
+           << a.wbits[0] << "," << a.wbits[1] << "," << a.wbits[2] << "],\"pout\":[";
         for (int i = 0; i < qx::kChainTileBits; ++i) os << (i ? "," : "") << a.pout[i];
         os << "],\"swz\":[";
         for (int i = 0; i < qx::kChainTileBits; ++i) os << (i ? "," : "") << a.swz[i];
@@ -1712,8 +1716,8 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
       plan->opt_small_walk = value;
       plan->drop_graphs();
     } else if (n == "chain_variant") {
-      if (value < -1 || value > 7 || (value & 3) == 3)
-        throw qx::Error(QX_ERR_ARGUMENT, "chain_variant must be -1 (heuristic), 0..2 or 4..6");
+      if (value < -1 || value > 15 || (value & 3) == 3)
+        throw qx::Error(QX_ERR_ARGUMENT, "chain_variant must be -1 (heuristic) or 0..15 with bits 0-1 below 3");
       plan->opt_chain_variant = value;
     } else if (n == "kernel") {
       if (value < -1 || value > 11) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..11");

@@ -82,12 +82,13 @@ def test_chain_gate_runs(name, precision, tol):
     assert np.linalg.norm(out - outp) / np.linalg.norm(outp) < tol
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 4, 5, 6, 8, 9, 10, 14])
 @pytest.mark.parametrize("precision,tol", [("c128", 1e-12), ("c64_4m", 2e-5)])
 def test_chain_variants(variant, precision, tol):
     """Every path of the 16-value steps (vector FMAs, matrix cores with four or three
-    real products per complex product) with plain and swizzled tile addressing, on
-    overlapping 4-leg blocks and on mixed leg counts, against the oracle."""
+    real products per complex product) with plain and swizzled tile addressing, with
+    wave-local steps and with a workgroup barrier after every step, on overlapping
+    4-leg blocks and on mixed leg counts, against the oracle."""
     for name in ("k4 x3 overlapping", "k1-k5 mixed"):
         dsl, data = qxtest.gate_sequence_program(16, GATE_RUNS[name], seed=len(name))
         ex = Executor(dsl, data, precision=precision, batch=1, chain_variant=variant)
@@ -99,3 +100,5 @@ def test_chain_variants(variant, precision, tol):
         parts = [s["config"].split("/") for s in chains]
         assert chains and all(("fma", "mfma", "mfma3")[variant & 3] in p for p in parts)
         assert all(("swz" in p) == bool(variant & 4) for p in parts)
+        if variant & 8:
+            assert not any("wl" in p for p in parts)
