@@ -1747,6 +1747,50 @@ __device__ __forceinline__ void chain_apply(typename Vec<T>::type *tile, const t
     return Sp + s.soff + h * K * K;
   };
   static_assert(K <= kChainPer, "larger steps run on matrix cores");
+  constexpr int NGL = kChainPer / K;   // groups per lane
+  if (K >= 4 && NGL > 1 && s.h == 0) {
+    // one S for every group: each S value is loaded once and feeds the lane's NGL
+    // groups (4 NGL independent FMA chains; with 1 or 2 contracted values the 8 or 16
+    // groups' accumulators would not fit beside the prefetch registers)
+    uint32_t base[NGL];
+    V in[NGL][K];
+#pragma unroll
+    for (int i = 0; i < NGL; ++i) {
+      base[i] = dt;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if ((i >> b) & 1) base[i] ^= (uint32_t)tb[TB + b];
+#pragma unroll
+      for (int kap = 0; kap < K; ++kap) in[i][kap] = tile[base[i] ^ koff(kap)];
+    }
+    const V *Sr = Sp + s.soff;
+#pragma unroll
+    for (int nu = 0; nu < K; ++nu) {
+      T xa[NGL], xb[NGL], ya[NGL], yb[NGL];
+#pragma unroll
+      for (int i = 0; i < NGL; ++i) xa[i] = xb[i] = ya[i] = yb[i] = T(0);
+#pragma unroll
+      for (int kap = 0; kap < K; ++kap) {
+        const V sv = Sr[nu * K + kap];
+#pragma unroll
+        for (int i = 0; i < NGL; ++i) {
+          xa[i] = fma(sv.x, in[i][kap].x, xa[i]);
+          xb[i] = fma(sv.y, in[i][kap].y, xb[i]);
+          ya[i] = fma(sv.x, in[i][kap].y, ya[i]);
+          yb[i] = fma(sv.y, in[i][kap].x, yb[i]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NGL; ++i) {
+        V o;
+        o.x = xa[i] - xb[i];
+        o.y = ya[i] + yb[i];
+        tile[base[i] ^ koff(nu)] = o;
+      }
+    }
+    step_end(wl);
+    return;
+  }
   // whole groups per lane, updated in place
 #pragma unroll
   for (int i = 0; i < kChainPer / K; ++i) {
@@ -1897,12 +1941,17 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
       v4 c1[2], c2[2], c3[2];
 #pragma unroll
       for (int u = 0; u < 2; ++u) c1[u] = c2[u] = c3[u] = v4{0, 0, 0, 0};
+      // the half's A values are read up front, so that the reads of later k blocks
+      // are in flight during the matrix-core ops of earlier ones
+      V ah[KB][2];
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) ah[kb][u] = tile[rbase[2 * half + u] ^ arow ^ koff(4 * kb + lk)];
 #pragma unroll
       for (int kb = 0; kb < KB; ++kb) {
         const T bd = bim[kb] - bre[kb], bs = bre[kb] + bim[kb];
-        V a[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) a[u] = tile[rbase[2 * half + u] ^ arow ^ koff(4 * kb + lk)];
+        const V *a = ah[kb];
 #pragma unroll
         for (int u = 0; u < 2; ++u) c1[u] = M::mma(bre[kb], a[u].x + a[u].y, c1[u]);
 #pragma unroll
@@ -1912,6 +1961,9 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u) store(2 * half + u, c1[u] - c3[u], c1[u] + c2[u]);
+      // scheduling: the half's 2 KB LDS reads first, then its 6 KB matrix-core ops
+      __builtin_amdgcn_sched_group_barrier(0x100, 2 * KB, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 6 * KB, 0);
     }
   }
   step_end(NB == 1 && wl);
@@ -1989,8 +2041,6 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   const bool s_lds = a.s_elems <= kChainSLds<T>;
   const int32_t *__restrict__ tab = reinterpret_cast<const int32_t *>(Sp + a.tab_off);
   const bool wl = a.wave_local != 0;
-  if (s_lds)
-    for (int e = t; e < a.s_elems; e += kChainThreads) sS[e] = Sp[e];
   const int64_t ntiles = int64_t(1) << a.nouter;
   auto in_base = [&](int64_t w) {
     int64_t bi = 0;
@@ -2065,9 +2115,23 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   };
   static_assert(kChainPer == 16, "four quarters of four elements");
   int64_t w = blockIdx.x;
+  // the first tile's loads and the staged gate blocks' loads in flight together
   if (w < ntiles) {
     const int64_t bi = in_base(w);
     static_for<kChainPer>([&](auto r) { pf[r] = X0e[bi + rin(r)]; });
+  }
+  if (s_lds) {
+    // every lane loads (an index past the blocks reads the last element again), only
+    // indices inside are stored
+    constexpr int NS = kChainSLds<T> / kChainThreads;
+    V sv[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) sv[i] = Sp[min(t + kChainThreads * i, a.s_elems - 1)];
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+      if (t + kChainThreads * i < a.s_elems) sS[t + kChainThreads * i] = sv[i];
+  }
+  if (w < ntiles) {
     static_for<kChainPer>([&](auto r) { tile_e[rsi(r)] = pf[r]; });
     lds_barrier();
   }
