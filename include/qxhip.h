@@ -120,9 +120,16 @@ qx_status qx_plan_update_tensor(qx_plan *plan, const char *key, const double *z,
  *                 8 rocBLAS strided-batched GEMM (steps whose operands are plain
  *                 column-major matrices), 9 skinny contraction (K >= 2^12, at most 32
  *                 output values per batch element, N <= 2), 10 rocBLAS on gathered
- *                 plain copies (steps of >= 2^34 flops whose operands are not plain) =
- *                 use that kernel on every step it can run (testing; the autotuner
- *                 also considers 8-10) */
+ *                 plain copies (steps of >= 2^34 flops whose operands are not plain),
+ *                 11 the library's own plain matrix-core GEMM (steps whose operands
+ *                 are plain matrices) = use that kernel on every step it can run
+ *                 (testing; the autotuner also considers 8-11)
+ *   "chain_variant" -1 (default) = heuristic chain-pass form (tuned with "autotune");
+ *                 else bits 0-1 = path of steps with 16 contracted values (0 vector
+ *                 FMAs, 1 matrix cores with four real products, 2 with three), bit 2 =
+ *                 XOR-swizzled tile addressing where the pass has a swizzle, bit 3 = a
+ *                 workgroup barrier after every step instead of wave-local steps
+ *                 (testing) */
 qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value);
 
 /* Check that every data key is present, lower the program (label classes,
