@@ -122,8 +122,9 @@ qx_status qx_plan_update_tensor(qx_plan *plan, const char *key, const double *z,
  *                 output values per batch element, N <= 2), 10 rocBLAS on gathered
  *                 plain copies (steps of >= 2^34 flops whose operands are not plain),
  *                 11 the library's own plain matrix-core GEMM (steps whose operands
- *                 are plain matrices) = use that kernel on every step it can run
- *                 (testing; the autotuner also considers 8-11)
+ *                 are plain matrices), 12 the same with three real products per
+ *                 complex product (complex128) = use that kernel on every step it can
+ *                 run (testing; the autotuner also considers 8-12)
  *   "chain_variant" -1 (default) = heuristic chain-pass form (tuned with "autotune");
  *                 else bits 0-1 = path of steps with 16 contracted values (0 vector
  *                 FMAs, 1 matrix cores with four real products, 2 with three), bit 2 =

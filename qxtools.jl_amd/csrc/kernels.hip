@@ -1135,6 +1135,7 @@ void launch_contract_gemm(int prec, const void *A, const void *B, void *C, void 
 // their X panels in its L2.
 template <typename T, bool GAUSS> struct PlainTile;
 template <> struct PlainTile<double, false> { static constexpr int WM = 32, WN = 32, TK = 16; };
+template <> struct PlainTile<double, true> { static constexpr int WM = 32, WN = 32, TK = 16; };
 template <> struct PlainTile<float, false> { static constexpr int WM = 64, WN = 64, TK = 16; };
 template <> struct PlainTile<float, true> { static constexpr int WM = 64, WN = 32, TK = 16; };
 
@@ -2694,7 +2695,11 @@ void launch_plain_gemm(int prec, const void *X, const void *Y, void *C, void *ws
       default: hipLaunchKernelGGL((k_plain_gemm_bf16x3<1, 1>), g, dim3(256), 0, s, (const float2 *)X, (const float2 *)Y, (float2 *)C, (float2 *)ws, b); break;
     }
   } else if (prec == QX_C128) {
-    QX_PLAIN_OPS(double, double2, false)
+    if (b.gauss) {
+      QX_PLAIN_OPS(double, double2, true)
+    } else {
+      QX_PLAIN_OPS(double, double2, false)
+    }
   } else if (prec == QX_C64_3M) {
     QX_PLAIN_OPS(float, float2, true)
   } else {

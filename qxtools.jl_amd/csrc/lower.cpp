@@ -131,7 +131,7 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
 
 // Kernel configurations a power-of-two step can run with.
 enum Choice { kBits = 0, kRows = 1, kGemm0 = 2, kGemm1 = 3, kGemm2 = 4, kMfma = 5, kReduce = 6, kMfma32 = 7,
-              kBlas = 8, kSkinny = 9, kTtgt = 10, kPlain = 11, kNumChoices = 12 };
+              kBlas = 8, kSkinny = 9, kTtgt = 10, kPlain = 11, kPlain3 = 12, kNumChoices = 13 };
 
 // K split of the hand-written plain GEMM: enough work units to give every CU two
 // workgroups, each K range at least 8 stages (128 values) long.
@@ -305,9 +305,13 @@ bool configure_kernel(Step &st, int precision, int choice) {
       while (st.c_elems * b.ksplit < 8192 && (int64_t(1) << nk) / b.ksplit >= 128) b.ksplit *= 2;
       break;
     }
+    case kPlain3:   // complex128 as three real products per complex product
+      if (precision != QX_C128) return false;
+      [[fallthrough]];
     case kPlain: {
       BlasArgs ba;
       if (!plain_gemm(st, ba)) return false;
+      ba.gauss = choice == kPlain3;
       // complex64 steps with M, N, K >= 1024 under QX_C64_BF16X3: the split bf16 product
       ba.split = precision == QX_C64_BF16X3 && ba.r >= 1024 && ba.s >= 1024 && ba.k >= 1024;
       plain_split(ba, precision);
@@ -572,7 +576,7 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
   if (sb.nk >= 12 && sb.nm + sb.nn <= 5 && sb.nn <= 1) alts.push_back(kSkinny);
   if (sb.nk >= 4 && sb.nn >= 4 && sb.nm >= 4 && 8.0 * st.H * st.M * st.N * st.K >= double(int64_t(1) << 34))
     alts.push_back(kTtgt);
-  if (sb.nk >= 4 && sb.nn >= 4 && sb.nm >= 4) alts.insert(alts.end(), {kPlain, kBlas});
+  if (sb.nk >= 4 && sb.nn >= 4 && sb.nm >= 4) alts.insert(alts.end(), {kPlain, kPlain3, kBlas});
   if (sb.nk <= 6 || st.c_elems <= (int64_t(1) << 16)) alts.push_back(kBits);
   const int def = default_choice(st, precision);
   for (int c : alts) {

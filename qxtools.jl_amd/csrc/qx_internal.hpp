@@ -230,8 +230,10 @@ struct BlasArgs {
   int64_t ga[kMaxBits] = {0}, gb[kMaxBits] = {0}, gc[kMaxBits] = {0};
   int64_t ta = 0, tb = 0;   // elements of the A and B copies (the result copy: r s batch)
   // own = 1: the hand-written matrix-core GEMM (k_plain_gemm) instead of the vendor
-  // BLAS; its K split over ksplit workgroups with dense partials in the workspace
-  int own = 0, ksplit = 1;
+  // BLAS; its K split over ksplit workgroups with dense partials in the workspace;
+  // gauss = 1: complex128 as three real products per complex product (complex64 takes
+  // the form from its precision)
+  int own = 0, ksplit = 1, gauss = 0;
 };
 
 enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows, Chain, Blas };

@@ -1119,8 +1119,8 @@ std::string kern_variant(const qx::Step &st) {
       if (st.blas.own)
         return std::string(st.blas.split ? "k_plain_gemm_bf16x3<" : "k_plain_gemm<") + (st.blas.opx ? "T" : "N") +
                (st.blas.opy ? "T" : "N") +
-               (st.blas.swap ? ",BA" : ",AB") + (st.blas.ksplit > 1 ? ",ks" + std::to_string(st.blas.ksplit) : "") +
-               ">" + (st.blas.ttgt ? "+gather" : "");
+               (st.blas.swap ? ",BA" : ",AB") + (st.blas.gauss ? ",3m" : "") +
+               (st.blas.ksplit > 1 ? ",ks" + std::to_string(st.blas.ksplit) : "") + ">" + (st.blas.ttgt ? "+gather" : "");
       return std::string("rocblas_gemm_strided_batched<") + (st.blas.opx ? "T" : "N") + (st.blas.opy ? "T" : "N") +
              (st.blas.swap ? ",BA>" : ",AB>") + (st.blas.ttgt ? "+gather" : "");
     default: return "k_contract_bits";
@@ -1720,7 +1720,7 @@ qx_status qx_plan_set_option(qx_plan *plan, const char *name, int64_t value) {
         throw qx::Error(QX_ERR_ARGUMENT, "chain_variant must be -1 (heuristic) or 0..15 with bits 0-1 below 3");
       plan->opt_chain_variant = value;
     } else if (n == "kernel") {
-      if (value < -1 || value > 11) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..11");
+      if (value < -1 || value > 12) throw qx::Error(QX_ERR_ARGUMENT, "kernel must be -1 (heuristic) or 0..12");
       plan->opt_kernel = value;
     } else if (n == "profile") {
       // may be toggled at any time; every set clears the accumulated step times

@@ -239,6 +239,22 @@ def test_plain_gemm_layouts(shape, layout, prec, tol):
     assert err(out, ref) < tol
 
 
+@pytest.mark.parametrize("layout", range(8))
+@pytest.mark.parametrize("shape", PLAIN_SHAPES + [(0, 4, 4, 12), (0, 5, 4, 13)])
+def test_plain_gemm_complex128_three_products(shape, layout):
+    """The complex128 plain GEMM as three real products per complex product (kernel
+    12) on every layout: normwise 1e-12 against the oracle, and per element within
+    1e-12 of the largest element (the Gauss form's rounding scales with |a| |b|)."""
+    dsl, data, ref = plain_step(960 + layout + 10 * sum(shape), *shape, layout & 1, layout >> 1 & 1, layout >> 2)
+    ex = Executor(dsl, data, precision="c128", kernel=12)
+    out = np.asarray(ex.amplitudes([""])).reshape(-1)
+    st = next(b for b in ex.stats()["batches"] if b["active"])["steps"][0]
+    ex.close()
+    assert st["kernel"] == "plain_gemm" and ",3m" in st["variant"], st["variant"]
+    assert err(out, ref) < 1e-12
+    assert np.max(np.abs(out - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
 def test_plain_gemm_split_k_in_graph_replays():
     """A plain GEMM whose K range is split over workgroups (partials summed by a second
     kernel) captured into the call's graph: every replay equals the eager result bit
