@@ -2012,7 +2012,7 @@ __global__ void k_chain_pack(typename Vec<T>::type *__restrict__ Sp, ChainArgs a
       reinterpret_cast<int32_t *>(Sp + a.tab_off)[kChainTabInts * j + e] = img;
     }
     const V *S = (const V *)a.S[j];
-    const int kk = 1 << s.k, tot = (1 << s.h) * kk * kk;
+    const int kk = 1 << s.k, tot = s.merged ? 0 : (1 << s.h) * kk * kk;   // merged: k_chain_merge
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += gridDim.x * blockDim.x) {
       const int kap = e % kk, nu = (e / kk) % kk, h = e / (kk * kk);
       int64_t o = 0;
@@ -2025,6 +2025,55 @@ __global__ void k_chain_pack(typename Vec<T>::type *__restrict__ Sp, ChainArgs a
       Sp[s.soff + e] = S[o];
     }
   }
+}
+
+// Merged steps: the product of the members' gate blocks, each acting on some of the
+// step's bits (identity on the others), first member applied first.  Workgroup g
+// holds the running product [nu][kappa] in LDS (lane nu K + kappa), stages each
+// member's block in LDS and applies it: new[nu][kappa] = sum over the member's input
+// bits s of S[nu's member bits][s] cur[nu with its member bits set to s][kappa].
+// Written packed as [nu][kappa] like the single blocks.
+template <typename T>
+__global__ __launch_bounds__(256) void k_chain_merge(typename Vec<T>::type *__restrict__ Sp, ChainMergeArgs m) {
+  using V = typename Vec<T>::type;
+  __shared__ V cur[256], nxt[256], blk[256];
+  const ChainMergeArgs::Group &g = m.g[blockIdx.x];
+  const int K = 1 << g.k, t = threadIdx.x, nu = t / K, kap = t % K;
+  const bool on = t < K * K;
+  if (on) cur[t] = V{nu == kap ? T(1) : T(0), T(0)};
+  for (int c = 0; c < g.count; ++c) {
+    const ChainMergeArgs::Member &mm = m.m[g.first + c];
+    const int kk = 1 << mm.kc;
+    __syncthreads();
+    if (t < kk * kk) {   // blk[so kk + si] = S[si -> sk strides, so -> sn strides]
+      const int so = t / kk, si = t % kk;
+      int64_t o = 0;
+      for (int i = 0; i < mm.kc; ++i) {
+        if ((si >> i) & 1) o += mm.sk[i];
+        if ((so >> i) & 1) o += mm.sn[i];
+      }
+      blk[t] = ((const V *)mm.S)[o];
+    }
+    __syncthreads();
+    if (on) {
+      int sub_out = 0, rest = nu;
+      for (int i = 0; i < mm.kc; ++i) {
+        sub_out |= ((nu >> mm.pos[i]) & 1) << i;
+        rest &= ~(1 << mm.pos[i]);
+      }
+      V acc{0, 0};
+      for (int si = 0; si < kk; ++si) {
+        int x = rest;
+        for (int i = 0; i < mm.kc; ++i)
+          if ((si >> i) & 1) x |= 1 << mm.pos[i];
+        cmac<T>(acc, blk[sub_out * kk + si], cur[x * K + kap], 0, 0);
+      }
+      nxt[t] = acc;
+    }
+    __syncthreads();
+    if (on) cur[t] = nxt[t];
+  }
+  if (on) Sp[g.soff + t] = cur[t];
 }
 
 // SWZ: tile loads and stores address LDS through the host's swizzle table (a separate
@@ -2639,13 +2688,16 @@ void launch_reject(const double *amps, const char *cand, int64_t nb, int nch, ui
   check_launch("reject");
 }
 
-void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args, void *stream) {
+void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args,
+                           const ChainMergeArgs &merge, void *stream) {
   const int64_t ntiles = int64_t(1) << args.nouter;
   const int g = (int)(ntiles < args.grid ? ntiles : args.grid);
   hipStream_t s = (hipStream_t)stream;
   const dim3 pack_grid(kChainMaxS / 256, args.nsteps);
   if (prec == QX_C128) {
     hipLaunchKernelGGL(k_chain_pack<double>, pack_grid, dim3(256), 0, s, (double2 *)ws, args);
+    if (merge.ngroups > 0)
+      hipLaunchKernelGGL(k_chain_merge<double>, dim3(merge.ngroups), dim3(256), 0, s, (double2 *)ws, merge);
     if (args.swizzled)
       hipLaunchKernelGGL((k_contract_chain<double, true>), dim3(g), dim3(kChainThreads), 0, s, (const double2 *)X0,
                          (double2 *)Xm, (const double2 *)ws, args);
@@ -2654,6 +2706,8 @@ void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const C
                          (double2 *)Xm, (const double2 *)ws, args);
   } else {
     hipLaunchKernelGGL(k_chain_pack<float>, pack_grid, dim3(256), 0, s, (float2 *)ws, args);
+    if (merge.ngroups > 0)
+      hipLaunchKernelGGL(k_chain_merge<float>, dim3(merge.ngroups), dim3(256), 0, s, (float2 *)ws, merge);
     if (args.swizzled)
       hipLaunchKernelGGL((k_contract_chain<float, true>), dim3(g), dim3(kChainThreads), 0, s, (const float2 *)X0,
                          (float2 *)Xm, (const float2 *)ws, args);

@@ -95,8 +95,23 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
         if (x) throw Error(QX_ERR_STATE, "internal: chain tile layout is not a bijection");
       }
       if (seen != (1u << kChainTileBits) - 1) throw Error(QX_ERR_STATE, "internal: chain store order");
+      for (const auto &cm : st.chain_merge) {
+        int64_t ms = 0;
+        for (int i = 0; i < cm.kc; ++i) {
+          ms += cm.sk[i] + cm.sn[i];
+          if (cm.pos[i] < 0 || cm.pos[i] >= a.st[cm.step].k || !a.st[cm.step].merged)
+            throw Error(QX_ERR_STATE, "internal: chain merge member bits");
+        }
+        auto es = extent(cm.small);
+        if (es.first + ms >= es.second)
+          throw Error(QX_ERR_STATE, "internal: chain '" + p.nodes[st.node].name + "' merge member out of range");
+      }
       for (int j = 0; j < a.nsteps; ++j) {
         const ChainStepArgs &c = a.st[j];
+        if (c.merged) {
+          if (c.h != 0 || c.k > 4) throw Error(QX_ERR_STATE, "internal: chain merged step");
+          continue;
+        }
         int64_t ms = 0;
         for (int i = 0; i < c.k; ++i) ms += c.sk[i] + c.sn[i];
         for (int i = 0; i < c.h; ++i) ms += c.sh[i];
@@ -849,21 +864,108 @@ Step chain_step(const Program &p, const Lowered &L, const ChainBuild &cb) {
   a.nouter = (int)outer.size();
   if (a.nouter > kMaxBits) throw Error(QX_ERR_UNSUPPORTED, "chain: too many legs");
   for (int j = 0; j < a.nouter; ++j) a.oin[j] = cb.x0_stride[outer[j]], a.oout[j] = xm_stride[outer[j]];
-  a.nsteps = (int)cb.steps.size();
-  double flops = 0, elems = 0;
+  // Merge gate blocks: a step without batch legs joins an earlier group without batch
+  // legs when their contracted slots together are at most four (one LDS pass instead
+  // of two), moving back past groups whose slots (contracted or batch) it does not
+  // touch.
+  struct Group {
+    std::vector<int> members, slots;   // slots: the union, sorted
+    bool batch = false;
+    std::set<int> touch;               // contracted and batch slots
+  };
+  std::vector<Group> groups;
   for (size_t j = 0; j < cb.steps.size(); ++j) {
     const auto &s = cb.steps[j];
-    ChainStepArgs &c = a.st[j];
-    c.k = (int32_t)s.kslot.size();
-    c.h = (int32_t)s.hslot.size();
-    for (int i = 0; i < c.k; ++i) c.q[i] = pos_of[s.kslot[i]], c.sk[i] = (int16_t)s.sk[i], c.sn[i] = (int16_t)s.sn[i];
-    for (int i = 0; i < c.h; ++i) c.hq[i] = pos_of[s.hslot[i]], c.sh[i] = (int16_t)s.sh[i];
-    c.soff = (int32_t)st.ws_elems;
-    st.ws_elems += (int64_t(1) << (c.h + 2 * c.k));
-    st.chain_small.push_back(s.small);
+    std::set<int> ks(s.kslot.begin(), s.kslot.end());
+    bool placed = false;
+    if (s.hslot.empty() && s.kslot.size() <= 4) {
+      // the reachable groups it could join: best one that it adds no slot to, else one
+      // it completes to four slots, else the smallest union
+      int best = -1, best_rank = 1 << 30;
+      for (int g = (int)groups.size() - 1; g >= 0; --g) {
+        const Group &G = groups[g];
+        if (!G.batch && (int)G.members.size() < kChainMaxMerged) {
+          std::set<int> u(G.slots.begin(), G.slots.end());
+          u.insert(ks.begin(), ks.end());
+          if (u.size() <= 4) {
+            const int rank = u.size() == G.slots.size() ? 0 : u.size() == 4 ? 1 : 1 + (int)u.size();
+            if (rank < best_rank) best_rank = rank, best = g;
+          }
+        }
+        bool disjoint = true;
+        for (int x : ks) disjoint &= !G.touch.count(x);
+        if (!disjoint) break;
+      }
+      if (best >= 0) {
+        Group &G = groups[best];
+        std::set<int> u(G.slots.begin(), G.slots.end());
+        u.insert(ks.begin(), ks.end());
+        G.members.push_back((int)j);
+        G.slots.assign(u.begin(), u.end());
+        G.touch.insert(ks.begin(), ks.end());
+        placed = true;
+      }
+    }
+    if (!placed) {
+      Group G;
+      G.members = {(int)j};
+      G.slots = s.kslot;
+      std::sort(G.slots.begin(), G.slots.end());
+      G.batch = !s.hslot.empty();
+      G.touch.insert(s.kslot.begin(), s.kslot.end());
+      G.touch.insert(s.hslot.begin(), s.hslot.end());
+      groups.push_back(std::move(G));
+    }
+  }
+  int nmerged = 0;
+  for (auto &G : groups) nmerged += G.members.size() > 1 ? (int)G.members.size() : 0;
+  if (nmerged > kChainMaxMerged) {   // more members than the merge kernel takes: no merging
+    groups.clear();
+    for (size_t j = 0; j < cb.steps.size(); ++j) {
+      Group G;
+      G.members = {(int)j};
+      G.slots = cb.steps[j].kslot;
+      groups.push_back(std::move(G));
+    }
+  }
+  a.nsteps = (int)groups.size();
+  double flops = 0, elems = 0;
+  for (size_t j = 0; j < cb.steps.size(); ++j) {
     const Step &m = L.steps[cb.members[j]];
     flops += 8.0 * m.H * m.M * m.N * m.K;
-    elems += (double)value_extent(p, L.values[s.small]);
+    elems += (double)value_extent(p, L.values[cb.steps[j].small]);
+  }
+  for (size_t g = 0; g < groups.size(); ++g) {
+    const Group &G = groups[g];
+    ChainStepArgs &c = a.st[g];
+    const auto &s0 = cb.steps[G.members[0]];
+    if (G.members.size() == 1) {
+      c.k = (int32_t)s0.kslot.size();
+      c.h = (int32_t)s0.hslot.size();
+      for (int i = 0; i < c.k; ++i) c.q[i] = pos_of[s0.kslot[i]], c.sk[i] = (int16_t)s0.sk[i], c.sn[i] = (int16_t)s0.sn[i];
+      for (int i = 0; i < c.h; ++i) c.hq[i] = pos_of[s0.hslot[i]], c.sh[i] = (int16_t)s0.sh[i];
+    } else {
+      c.k = (int32_t)G.slots.size();
+      c.h = 0;
+      c.merged = 1;
+      for (int i = 0; i < c.k; ++i) c.q[i] = pos_of[G.slots[i]];
+      for (int mj : G.members) {
+        const auto &s = cb.steps[mj];
+        Step::ChainMember cm;
+        cm.step = (int)g;
+        cm.small = s.small;
+        cm.kc = (int)s.kslot.size();
+        for (int i = 0; i < cm.kc; ++i) {
+          cm.pos[i] = (int)(std::find(G.slots.begin(), G.slots.end(), s.kslot[i]) - G.slots.begin());
+          cm.sk[i] = (int)s.sk[i];
+          cm.sn[i] = (int)s.sn[i];
+        }
+        st.chain_merge.push_back(cm);
+      }
+    }
+    c.soff = (int32_t)st.ws_elems;
+    st.ws_elems += (int64_t(1) << (c.h + 2 * c.k));
+    st.chain_small.push_back(s0.small);
   }
   a.s_elems = (int)st.ws_elems;
   a.tab_off = (int)((st.ws_elems + kChainTabElems - 1) / kChainTabElems * kChainTabElems);
@@ -1497,6 +1599,7 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
   for (size_t s = 0; s < L.steps.size(); ++s) {
     std::vector<int> ins = {L.steps[s].in_a, L.steps[s].in_b};
     ins.insert(ins.end(), L.steps[s].chain_small.begin(), L.steps[s].chain_small.end());
+    for (const auto &cm : L.steps[s].chain_merge) ins.push_back(cm.small);
     for (int in : ins) {
       int r = root[in];
       if (r < 0) continue;

@@ -161,7 +161,28 @@ struct ChainStepArgs {
   int32_t q[5] = {0}, hq[3] = {0};   // tile bit of each contracted / batch leg
   int32_t soff = 0;             // offset of S packed as [h][nu][kappa] in the workspace
   int16_t sk[5] = {0}, sn[5] = {0}, sh[3] = {0};   // strides in S of contracted, new, batch legs
+  int32_t merged = 0;           // 1: S is the product of several gate blocks (k_chain_merge)
 };
+
+// Gate blocks of a chain merged into one step (several consecutive steps, or steps
+// moved past steps on other tile bits, whose contracted bits together are at most
+// four): the merged step's S is the product of its members' blocks, each acting on
+// some of the merged step's bits (identity on the others), first member rightmost.
+constexpr int kChainMaxMerged = 32;   // members of merged steps in one chain
+struct ChainMergeArgs {
+  int ngroups = 0;
+  struct Group {
+    int32_t step = 0, k = 0, first = 0, count = 0;   // merged step index, its bits, members
+    int32_t soff = 0;                                 // its packed block in the workspace
+  } g[kChainMaxSteps];
+  struct Member {
+    const void *S = nullptr;   // the member's gate block, set at launch
+    int32_t kc = 0;            // its contracted legs
+    int32_t pos[4] = {0};      // index of each among the merged step's bits
+    int32_t sk[4] = {0}, sn[4] = {0};   // strides in S of its contracted and new legs
+  } m[kChainMaxMerged];
+};
+static_assert(sizeof(ChainMergeArgs) <= 4096, "merge arguments must fit the kernel argument segment");
 
 struct ChainArgs {
   int nsteps = 0, nouter = 0;
@@ -256,7 +277,12 @@ struct Step {
   RadixArgs radix;         // Kern::Radix
   BlasArgs blas;           // Kern::Blas
   ChainArgs chain;         // Kern::Chain: in_a = X_0, node = X_m
-  std::vector<int> chain_small;   // Kern::Chain: node of each S_j
+  std::vector<int> chain_small;   // Kern::Chain: node of each S_j (a merged step: its first member's)
+  struct ChainMember {
+    int step = 0, small = -1, kc = 0;   // merged step, gate block node, its contracted legs
+    int pos[4] = {0}, sk[4] = {0}, sn[4] = {0};
+  };
+  std::vector<ChainMember> chain_merge;   // Kern::Chain: members of merged steps, in order
   double chain_flops = 0, chain_elems = 0;   // Kern::Chain: work and compulsory elements moved
 };
 
@@ -390,7 +416,10 @@ void launch_gather(int prec, const void *src, void *dst, int nb, const int64_t *
 void launch_planar_merge(const float *re, const float *im, int64_t n, void *dst, void *stream);
 // d_ranges: n (offset, length) pairs in 16-byte units from base; one workgroup each.
 void launch_zero_batch(void *base, const int64_t *d_ranges, int n, void *stream);
-void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args, void *stream);
+// One chain pass: k_chain_pack (gate blocks and step tables into the workspace), for
+// merged steps k_chain_merge (their members' product), then k_contract_chain.
+void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args,
+                           const ChainMergeArgs &merge, void *stream);
 // C = op(X) op(Y) by the hand-written matrix-core GEMM (b.own): plain column-major
 // operands as for the vendor BLAS; with b.ksplit > 1, `ws` holds ksplit * batch * r * s
 // elements of partials.  plain_gemm_units: C tiles of one K range.

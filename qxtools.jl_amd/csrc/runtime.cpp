@@ -183,6 +183,14 @@ struct qx_plan {
 
 namespace {
 
+// Contraction steps a lowered step performs (a chain pass: its members, merged or not).
+int fused_steps(const qx::Step &st) {
+  if (st.kern != qx::Kern::Chain) return 1;
+  int n = st.chain.nsteps;
+  for (int j = 0; j < st.chain.nsteps; ++j) n -= st.chain.st[j].merged;
+  return n + (int)st.chain_merge.size();
+}
+
 // The chain-pass variant the `chain_variant` option names: the path of steps with 16
 // contracted values (bits 0-1: vector FMAs, matrix cores with four or with three real
 // products), the swizzled tile addressing (bit 2, where the pass has a non-trivial
@@ -376,7 +384,24 @@ void launch_step(qx_plan *P, const qx::Lowered &L, const qx::Step &st, void *A, 
         const qx::Value &v = L.values[st.chain_small[j]];
         a.S[j] = bs ? value_ptr(P, v, varval, *bs) : value_ptr(P, v, varval);
       }
-      qx::launch_contract_chain(P->prec, A, C, P->ws.p, a, stream);
+      qx::ChainMergeArgs m;
+      for (size_t i = 0; i < st.chain_merge.size(); ++i) {
+        const auto &cm = st.chain_merge[i];
+        if (m.ngroups == 0 || m.g[m.ngroups - 1].step != cm.step) {
+          auto &g = m.g[m.ngroups++];
+          g.step = cm.step;
+          g.k = st.chain.st[cm.step].k;
+          g.soff = st.chain.st[cm.step].soff;
+          g.first = (int32_t)i;
+        }
+        m.g[m.ngroups - 1].count++;
+        auto &mm = m.m[i];
+        const qx::Value &v = L.values[cm.small];
+        mm.S = bs ? value_ptr(P, v, varval, *bs) : value_ptr(P, v, varval);
+        mm.kc = cm.kc;
+        for (int q = 0; q < 4; ++q) mm.pos[q] = cm.pos[q], mm.sk[q] = cm.sk[q], mm.sn[q] = cm.sn[q];
+      }
+      qx::launch_contract_chain(P->prec, A, C, P->ws.p, a, m, stream);
       break;
     }
     case qx::Kern::Radix: qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, st.radix, stream); break;
@@ -1150,7 +1175,7 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
     double bytes = (double)es * st.H * (st.M * st.K + st.K * st.N + st.M * st.N);
     if (st.kern == qx::Kern::Chain) flops = st.chain_flops, bytes = (double)es * st.chain_elems;
     os << "{\"name\":\"" << prog.nodes[st.node].name << "\",\"section\":" << st.section
-       << ",\"fused_steps\":" << (st.kern == qx::Kern::Chain ? st.chain.nsteps : 1) << ",\"B\":" << st.H
+       << ",\"fused_steps\":" << fused_steps(st) << ",\"B\":" << st.H
        << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K << ",\"flops\":" << flops
        << ",\"bytes\":" << bytes << ",\"kernel\":\"" << kern_name(st) << "\",\"variant\":\""
        << kern_variant(st) << "\",\"config\":\"" << cand_label(st) << "\",\"ws_elems\":" << st.ws_elems;
@@ -1545,7 +1570,7 @@ std::string stats_json(const qx_plan *P) {
         if (it != P->small.end() && it->second.walk_levels[st.section] > 0) variant = "k_small_walk";
       }
       os << "{\"name\":\"" << P->prog.nodes[st.node].name << "\",\"section\":" << st.section
-         << ",\"fused_steps\":" << (st.kern == qx::Kern::Chain ? st.chain.nsteps : 1)
+         << ",\"fused_steps\":" << fused_steps(st)
          << ",\"B\":" << st.H << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K
          << ",\"flops\":" << flops << ",\"bytes\":" << bytes << ",\"kernel\":\""
          << kern_name(st) << "\",\"variant\":\"" << variant << "\",\"config\":\"" << cand_label(st)

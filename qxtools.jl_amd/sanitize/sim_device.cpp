@@ -548,7 +548,8 @@ void launch_zero_batch(void *base, const int64_t *d_ranges, int n, void *stream)
   });
 }
 
-void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &a, void *stream) {
+void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &a,
+                           const ChainMergeArgs &merge, void *stream) {
   const size_t es = esize(prec);
   if (a.nsteps < 1 || a.nsteps > kChainMaxSteps || a.nouter < 0 || a.nouter > kMaxBits)
     violation("chain arguments out of range");
@@ -561,13 +562,36 @@ void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const C
     if (t.k < 0 || t.k > 5 || t.h < 0 || t.h > 3 || t.k + t.h < 1)
       violation("chain step " + std::to_string(j) + " out of range: k " + std::to_string(t.k) + ", h " +
                 std::to_string(t.h));
+    if (t.merged && (t.h != 0 || t.k > 4)) violation("chain merged step with batch legs or over 4 bits");
     long double r = 1;
     for (int i = 0; i < t.k; ++i) r += (long double)t.sk[i] + t.sn[i];
     for (int i = 0; i < t.h; ++i) r += t.sh[i];
-    s.emplace_back(a.S[j], r * es);
+    if (!t.merged) s.emplace_back(a.S[j], r * es);
     packed = std::max(packed, (long double)t.soff + std::ldexp(1.0L, t.h + 2 * t.k));
   }
   if (packed > a.s_elems) violation("chain: packed gate blocks beyond s_elems");
+  // merged steps: every group names a merged step, covers its members, and writes its
+  // K x K block inside the packed blocks; every member's reach inside its gate block
+  if (merge.ngroups < 0 || merge.ngroups > a.nsteps) violation("chain merge: group count");
+  int merged_steps = 0;
+  for (int j = 0; j < a.nsteps; ++j) merged_steps += a.st[j].merged;
+  if (merge.ngroups != merged_steps) violation("chain merge: groups do not match the merged steps");
+  for (int gi = 0; gi < merge.ngroups; ++gi) {
+    const auto &g = merge.g[gi];
+    if (g.step < 0 || g.step >= a.nsteps || !a.st[g.step].merged || g.k != a.st[g.step].k || g.soff != a.st[g.step].soff)
+      violation("chain merge: group " + std::to_string(gi) + " does not name its merged step");
+    if (g.first < 0 || g.count < 2 || g.first + g.count > kChainMaxMerged) violation("chain merge: member range");
+    for (int c = g.first; c < g.first + g.count; ++c) {
+      const auto &mm = merge.m[c];
+      if (mm.kc < 1 || mm.kc > 4) violation("chain merge: member legs");
+      long double r = 1;
+      for (int i = 0; i < mm.kc; ++i) {
+        if (mm.pos[i] < 0 || mm.pos[i] >= g.k) violation("chain merge: member bit outside its step");
+        r += (long double)mm.sk[i] + mm.sn[i];
+      }
+      s.emplace_back(mm.S, r * es);
+    }
+  }
   if (a.tab_off < a.s_elems) violation("chain: step tables overlap the packed gate blocks");
   if ((int64_t)kChainTabInts * 4 > (int64_t)kChainTabElems * (int64_t)es)
     violation("chain: step table larger than its workspace share");

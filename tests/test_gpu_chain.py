@@ -62,7 +62,15 @@ GATE_RUNS = {
     "k4 x3 overlapping": [(10, 11, 12, 13), (13, 14, 15, 0), (0, 1, 2, 3)],
     "k5 x2 high legs": [(11, 12, 13, 14, 15), (6, 7, 8, 9, 10)],
     "k2 x4 low legs": [(0, 1), (1, 2), (2, 3), (0, 3)],
+    # merged gate blocks: two 2-leg blocks into one 4-leg step, a 1-leg block absorbed
+    # into a 4-leg one, a block moved back past a block on other legs, three blocks
+    "merge pair and absorb": [(0, 1), (2, 3), (6, 7, 8, 9), (8,)],
+    "merge past other legs": [(0, 5), (6, 7, 8, 9), (3, 4), (0, 3)],
+    "merge three": [(0, 3), (5, 0), (6, 3), (9, 10, 11, 12), (12,), (1, 2)],
 }
+
+# chain steps after merging (the kernel's passes over the tile) for the runs above
+MERGED_STEPS = {"merge pair and absorb": 2, "merge past other legs": 2, "merge three": 3, "k2 x4 low legs": 1}
 
 
 @pytest.mark.parametrize("name", sorted(GATE_RUNS))
@@ -80,6 +88,10 @@ def test_chain_gate_runs(name, precision, tol):
     plain = Executor(dsl, data, precision=precision, batch=1, chains=0)
     outp = np.asarray(plain.amplitudes([""])).reshape(-1)
     assert np.linalg.norm(out - outp) / np.linalg.norm(outp) < tol
+    if name in MERGED_STEPS:
+        ch = [s for s in info["steps"] if s["kernel"] == "chain"]
+        assert sum(len(s["chain_tile"]["q"]) for s in ch) == MERGED_STEPS[name], [s["chain_tile"]["q"] for s in ch]
+        assert sum(s["fused_steps"] for s in ch) == len(GATE_RUNS[name])
 
 
 @pytest.mark.parametrize("variant", [0, 1, 2, 4, 5, 6, 8, 9, 10, 14])

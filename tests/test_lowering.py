@@ -270,8 +270,8 @@ def test_chain_lds_swizzle_is_a_bijection_that_spreads_banks():
     (16 lanes of 16-byte elements, differing in the step's lowest four free tile bits
     outside the wave bits of a wave-local chain, in the vector and the matrix-core
     steps alike) reach no fewer bank groups than the plain layout, and more on average
-    (8.2 of 16 against 4.2 in the 5x5 plan, whose chains are all wave-local: the wave
-    bits take three of the low free bits)."""
+    (7.5 of 16 against 3.5 in the 5x5 plan with its merged gate blocks, whose chains are
+    all wave-local: the wave bits take three of the low free bits)."""
     dsl, data, bits = workloads.WORKLOADS["rqc_5x5_d20"].load()
     info = _native.inspect(dsl, batch=256)
     chains = [s for s in info["steps"] if s["kernel"] == "chain"]
@@ -302,4 +302,4 @@ def test_chain_lds_swizzle_is_a_bijection_that_spreads_banks():
             groups.append(got)
             plain_groups.append(ref)
     assert all(c["chain_tile"]["wave_local"] for c in chains)
-    assert sum(groups) / len(groups) >= 8 > 2 * sum(plain_groups) / len(plain_groups) - 1
+    assert sum(groups) / len(groups) >= 7 and sum(groups) > 1.6 * sum(plain_groups)
