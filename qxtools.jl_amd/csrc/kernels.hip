@@ -1690,6 +1690,13 @@ __device__ __forceinline__ void step_end(bool wave_local) {
   }
 }
 
+// Slot of packed gate-block element e in the staged copy in LDS: the low four index
+// bits XOR the next four, so that the 16 rows of a 16- or 32-value block, which the
+// matrix-core steps read one element per row across lanes (rows 256 or 512 bytes
+// apart in complex128), land in distinct banks instead of one.  A bijection on every
+// aligned run of 16 elements.
+__device__ __forceinline__ int ss_slot(int e) { return e ^ ((e >> 4) & 15); }
+
 // Step j's S, packed densely as [h][nu][kappa] at Sp + a.st[j].soff, is read with
 // wave-uniform indices (the batch bits h are placed on wave-uniform group bits), so
 // its values come through the scalar cache rather than LDS.
@@ -1894,7 +1901,7 @@ __device__ __forceinline__ void chain_apply_mfma(typename Vec<T>::type *tile,
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) {
     // from LDS when staged: a global load here would wait for the prefetch too
-    const V v = SL ? sS[so + 4 * kb] : Sp[so + 4 * kb];
+    const V v = SL ? sS[ss_slot(so + 4 * kb)] : Sp[so + 4 * kb];
     bre[kb] = v.x;
     bim[kb] = v.y;
   }
@@ -2086,8 +2093,9 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
                                                                   ChainArgs a) {
   using V = typename Vec<T>::type;
   __shared__ V tile[kChainTile];
-  __shared__ V sS[kChainSLds<T>];   // packed S_j of every step, when they fit
+  __shared__ V sS[kChainSLds<T>];   // packed S_j of every step (at ss_slot), when they fit
   const int t = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(t) >> 6;
   const bool s_lds = a.s_elems <= kChainSLds<T>;
   const int32_t *__restrict__ tab = reinterpret_cast<const int32_t *>(Sp + a.tab_off);
   const bool wl = a.wave_local != 0;
@@ -2179,7 +2187,7 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
     for (int i = 0; i < NS; ++i) sv[i] = Sp[min(t + kChainThreads * i, a.s_elems - 1)];
 #pragma unroll
     for (int i = 0; i < NS; ++i)
-      if (t + kChainThreads * i < a.s_elems) sS[t + kChainThreads * i] = sv[i];
+      if (t + kChainThreads * i < a.s_elems) sS[ss_slot(t + kChainThreads * i)] = sv[i];
   }
   if (w < ntiles) {
     static_for<kChainPer>([&](auto r) { tile_e[rsi(r)] = pf[r]; });
@@ -2193,6 +2201,15 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
       if (more)
         for (int q = 0; q < 4; ++q)
           if (q * n / 4 == j) load_quarter(q, bn);
+      // wave-local steps: the two waves of a SIMD (waves w and w + 4) take turns at the
+      // higher issue priority, step by step, so that neither runs far ahead and the
+      // other is left to finish its steps alone before the barrier of the stores
+      if (wl) {
+        if (((wave >> 2) ^ j) & 1)
+          __builtin_amdgcn_s_setprio(1);
+        else
+          __builtin_amdgcn_s_setprio(0);
+      }
       const ChainStepArgs &s = a.st[j];
       const int32_t *tb = tab + kChainTabInts * j;
       switch (s.k) {
@@ -2221,7 +2238,10 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
       }
     }
     // the stores read the tile in store order, across waves
-    if (wl) lds_barrier();
+    if (wl) {
+      __builtin_amdgcn_s_setprio(0);
+      lds_barrier();
+    }
     int64_t bo = 0;
 #pragma unroll 4
     for (int j = 0; j < a.nouter; ++j)
