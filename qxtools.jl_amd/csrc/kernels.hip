@@ -2100,12 +2100,27 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   const int32_t *__restrict__ tab = reinterpret_cast<const int32_t *>(Sp + a.tab_off);
   const bool wl = a.wave_local != 0;
   const int64_t ntiles = int64_t(1) << a.nouter;
-  auto in_base = [&](int64_t w) {
-    int64_t bi = 0;
+  // X_0 / X_m offsets of tile w: the outer bits' strides summed over w's set bits
+  auto tile_base = [&](int64_t w, const int64_t *o) {
+    int64_t b = 0;
 #pragma unroll 4
     for (int j = 0; j < a.nouter; ++j)
-      if ((w >> j) & 1) bi += a.oin[j];
-    return bi;
+      if ((w >> j) & 1) b += o[j];
+    return b;
+  };
+  // the same for tile w + gridDim.x from tile w's offset b, when w + gridDim.x is a
+  // tile: with a power-of-two grid (2^gb workgroups) the low gb bits stay and the high
+  // part counts up by one (its lowest zero bit c sets, the c ones below clear), so a
+  // workgroup's next offset costs about two stride loads instead of one per outer bit
+  const int gsz = (int)gridDim.x;
+  const bool grid_pow2 = (gsz & (gsz - 1)) == 0;
+  const int gb = __builtin_ctz((unsigned)gsz);
+  auto next_base = [&](int64_t w, int64_t b, const int64_t *o) {
+    if (!grid_pow2) return tile_base(w + gsz, o);
+    const int c = __builtin_ctzll(~((uint64_t)w >> gb));
+    b += o[gb + c];
+    for (int j = 0; j < c; ++j) b -= o[gb + j];
+    return b;
   };
   // lane t holds tile elements t + kChainThreads * r: the low tile bits come from t,
   // the top four from r
@@ -2173,9 +2188,11 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   };
   static_assert(kChainPer == 16, "four quarters of four elements");
   int64_t w = blockIdx.x;
+  int64_t bi = 0, bo = 0;   // the current tile's X_0 and X_m offsets
   // the first tile's loads and the staged gate blocks' loads in flight together
   if (w < ntiles) {
-    const int64_t bi = in_base(w);
+    bi = tile_base(w, a.oin);
+    bo = tile_base(w, a.oout);
     static_for<kChainPer>([&](auto r) { pf[r] = X0e[bi + rin(r)]; });
   }
   if (s_lds) {
@@ -2195,7 +2212,7 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   }
   for (; w < ntiles; w += gridDim.x) {
     const bool more = w + gridDim.x < ntiles;
-    const int64_t bn = more ? in_base(w + gridDim.x) : 0;
+    const int64_t bn = more ? next_base(w, bi, a.oin) : 0;
     const int n = a.nsteps;
     for (int j = 0; j < n; ++j) {
       if (more)
@@ -2242,12 +2259,10 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
       __builtin_amdgcn_s_setprio(0);
       lds_barrier();
     }
-    int64_t bo = 0;
-#pragma unroll 4
-    for (int j = 0; j < a.nouter; ++j)
-      if ((w >> j) & 1) bo += a.oout[j];
     E *Xme = reinterpret_cast<E *>(Xm);
     static_for<kChainPer>([&](auto r) { Xme[bo + rout(r)] = tile_e[rso(r)]; });
+    if (more) bo = next_base(w, bo, a.oout);
+    bi = bn;
     lds_barrier();
     if (more) {
       static_for<kChainPer>([&](auto r) { tile_e[rsi(r)] = pf[r]; });
