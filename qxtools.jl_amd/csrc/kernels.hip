@@ -1062,6 +1062,73 @@ void rows_launch_k(int prec, const void *A, const void *B, void *C, const RowArg
 
 }  // namespace
 
+// Runs of outer products (OuterArgs): lane per output element, the product of the
+// factors in the run's order, each factor's offset the sum of its strides over the
+// element's set bits.  A workgroup covers 256 consecutive elements per trip: the lowest
+// 8 bits are the lane's own (their part of every factor's offset is summed once per
+// lane), the bits above are the same for all lanes (summed once per trip, one lane per
+// factor); both through LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_outer_run(typename Vec<T>::type *__restrict__ C, int64_t n, OuterArgs a) {
+  using V = typename Vec<T>::type;
+  __shared__ int sbit[kOuterMaxTerms];
+  __shared__ int64_t sw[kOuterMaxTerms];
+  __shared__ int64_t shi[kOuterMaxFactors];
+  const int t = threadIdx.x;
+  const int nf = a.nf, nt = a.first[nf];
+  for (int k = t; k < nt; k += 256) {
+    sbit[k] = a.bit[k];
+    sw[k] = a.w[k];
+  }
+  __syncthreads();
+  // the lane's own low-bit offsets, one per factor, in LDS (each lane reads only its own)
+  __shared__ int64_t slo[kOuterMaxFactors * 256];
+  for (int f = 0; f < nf; ++f) {
+    int64_t o = 0;
+    for (int k = a.first[f]; k < a.first[f + 1]; ++k)
+      if (sbit[k] < 8 && ((t >> sbit[k]) & 1)) o += sw[k];
+    slo[f * 256 + t] = o;
+  }
+  for (int64_t base = (int64_t)blockIdx.x * 256; base < n; base += (int64_t)gridDim.x * 256) {
+    if (t < nf) {
+      int64_t h = 0;
+      for (int k = a.first[t]; k < a.first[t + 1]; ++k)
+        if (sbit[k] >= 8 && ((base >> sbit[k]) & 1)) h += sw[k];
+      shi[t] = h;
+    }
+    __syncthreads();
+    const int64_t e = base + t;
+    if (e < n) {
+      // every factor's value loaded before the products (one memory latency per trip)
+      V xs[kOuterMaxFactors];
+#pragma unroll
+      for (int f = 0; f < kOuterMaxFactors; ++f)
+        if (f < nf) xs[f] = reinterpret_cast<const V *>(a.F[f])[slo[f * 256 + t] + shi[f]];
+      V v = xs[0];
+#pragma unroll
+      for (int f = 1; f < kOuterMaxFactors; ++f)
+        if (f < nf) {
+          V r{0, 0};
+          if (a.prev_a[f]) cmac<T>(r, v, xs[f], 0, 0);
+          else cmac<T>(r, xs[f], v, 0, 0);
+          v = r;
+        }
+      C[e] = v;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_outer(int prec, void *C, int64_t c_elems, const OuterArgs &args, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const int g = grid_for(c_elems, 256) < 1024 ? grid_for(c_elems, 256) : 1024;
+  if (prec == QX_C128)
+    hipLaunchKernelGGL(k_outer_run<double>, dim3(g), dim3(256), 0, s, (double2 *)C, c_elems, args);
+  else
+    hipLaunchKernelGGL(k_outer_run<float>, dim3(g), dim3(256), 0, s, (float2 *)C, c_elems, args);
+  check_launch("outer_run");
+}
+
 void launch_contract_rows(int prec, const void *A, const void *B, void *C, const RowArgs &args, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   switch (args.nk) {

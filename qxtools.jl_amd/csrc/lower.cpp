@@ -125,6 +125,26 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
       }
       continue;
     }
+    if (st.kern == Kern::Outer) {
+      const OuterArgs &a = st.outer;
+      if (a.nf != (int)st.outer_in.size() || a.nf < 2 || a.nf > kOuterMaxFactors)
+        throw Error(QX_ERR_STATE, "internal: outer-product run factors");
+      for (int f = 0; f < a.nf; ++f) {
+        int64_t mf = 0;
+        for (int t = a.first[f]; t < a.first[f + 1]; ++t) {
+          if (a.bit[t] < 0 || (int64_t(1) << a.bit[t]) >= st.c_elems || a.w[t] <= 0)
+            throw Error(QX_ERR_STATE, "internal: outer-product run term");
+          mf += a.w[t];
+        }
+        auto ef = extent(st.outer_in[f]);
+        if (ef.first + mf >= ef.second)
+          throw Error(QX_ERR_STATE, "internal: outer-product run '" + p.nodes[st.node].name + "' factor out of range");
+      }
+      auto ec = extent(st.node);
+      if (ec.first + st.c_elems - 1 >= ec.second || !pow2(st.c_elems))
+        throw Error(QX_ERR_STATE, "internal: outer-product run '" + p.nodes[st.node].name + "' output out of range");
+      continue;
+    }
     int64_t ma = 0, mb = 0, mc = 0;
     for (auto &g : st.cl) {
       ma += (g.dim - 1) * g.sa;
@@ -535,7 +555,7 @@ static Step swap_operands(const Step &st) {
 }
 
 bool force_kernel(Step &st, int precision, int choice) {
-  if (st.kern == Kern::Radix || st.kern == Kern::Chain) return false;
+  if (st.kern == Kern::Radix || st.kern == Kern::Chain || st.kern == Kern::Outer) return false;
   Step src = st;
   Step v = src;
   if (!configure_kernel(v, precision, choice)) {
@@ -581,7 +601,7 @@ std::vector<Step> kernel_candidates(const Step &st, int precision) {
           }
     return out;
   }
-  if (st.kern == Kern::Radix) return out;
+  if (st.kern == Kern::Radix || st.kern == Kern::Outer) return out;
   const StepBits sb = step_bits(st);
   std::vector<int> alts;
   if (sb.nk >= 2 && sb.nn >= 2 && sb.nm >= 4) alts.insert(alts.end(), {kGemm0, kGemm1, kGemm2, kMfma});
@@ -1063,6 +1083,167 @@ std::vector<int> form_chains(const Program &p, Lowered &L) {
   return absorbed;
 }
 
+bool small_step(const Step &st, bool wide);
+
+// A step that only multiplies: no contracted leg, power-of-two legs.
+bool outer_product(const Step &st) {
+  if (st.kern == Kern::Chain || st.kern == Kern::Outer || st.kern == Kern::Radix || !st.kl.empty() || st.K != 1)
+    return false;
+  for (const Leg &g : st.cl)
+    if (!pow2(g.dim)) return false;
+  return true;
+}
+
+// The Kern::Outer step of a run of outer products (members in order, each reading the
+// previous one's output), or false when the run does not fit OuterArgs.  Walks back
+// from the last member: every leg of the final C is followed through the running
+// value's operand to the step that brought it in; each factor gets, for every bit of
+// every leg it carries, that leg's stride in it.
+bool outer_step(const Lowered &L, const std::vector<size_t> &run, Step &out) {
+  const int n = (int)run.size();
+  const Step &last = L.steps[run.back()];
+  struct Pend {
+    int64_t dim, c_bit;   // the final leg: dimension and its lowest bit in C
+    int64_t s;            // its stride in the current member's output
+  };
+  std::vector<Pend> pend;
+  for (const Leg &g : last.cl) pend.push_back({g.dim, ilog2(g.sc), g.sc});
+  // factor terms per member: member i >= 1 adds one factor, member 0 two (A then B)
+  std::vector<std::vector<std::pair<int, int64_t>>> terms(n + 1);
+  std::vector<int> fnode(n + 1, -1);
+  std::vector<int8_t> prev_a(n + 1, 0);
+  for (int i = n - 1; i >= 0; --i) {
+    const Step &st = L.steps[run[i]];
+    if ((int)pend.size() != (int)st.cl.size()) return false;
+    std::map<int64_t, const Leg *> by_sc;
+    for (const Leg &g : st.cl) by_sc[g.sc] = &g;
+    const bool run_a = i > 0 && st.in_a == L.steps[run[i - 1]].node;
+    const bool run_b = i > 0 && !run_a && st.in_b == L.steps[run[i - 1]].node;
+    if (i > 0 && !run_a && !run_b) return false;
+    std::vector<Pend> next;
+    auto add_terms = [&](int f, const Pend &pd, int64_t stride) {
+      for (int64_t j = 0; (int64_t(1) << j) < pd.dim; ++j) terms[f].push_back({(int)(pd.c_bit + j), stride << j});
+    };
+    for (const Pend &pd : pend) {
+      auto it = by_sc.find(pd.s);
+      if (it == by_sc.end() || it->second->dim != pd.dim) return false;
+      const Leg &g = *it->second;
+      if (i == 0) {
+        if (g.sa) add_terms(0, pd, g.sa);
+        if (g.sb) add_terms(1, pd, g.sb);
+        continue;
+      }
+      const int64_t s_run = run_a ? g.sa : g.sb, s_fac = run_a ? g.sb : g.sa;
+      if (s_run) next.push_back({pd.dim, pd.c_bit, s_run});
+      if (s_fac) add_terms(i + 1, pd, s_fac);
+    }
+    if (i == 0) {
+      fnode[0] = st.in_a;
+      fnode[1] = st.in_b;
+      prev_a[1] = 1;
+    } else {
+      fnode[i + 1] = run_a ? st.in_b : st.in_a;
+      prev_a[i + 1] = run_a ? 1 : 0;
+      pend = std::move(next);
+    }
+  }
+  int nterms = 0;
+  for (auto &t : terms) nterms += (int)t.size();
+  if (n + 1 > kOuterMaxFactors || nterms > kOuterMaxTerms) return false;
+  out = Step();
+  out.kern = Kern::Outer;
+  out.node = last.node;
+  out.in_a = fnode[0];
+  out.in_b = fnode[1];
+  out.outer_in = fnode;
+  out.section = last.section;
+  out.cl = last.cl;
+  out.c_elems = last.c_elems;
+  out.M = last.c_elems;
+  OuterArgs &a = out.outer;
+  a.nf = n + 1;
+  a.members = n;
+  int k = 0;
+  for (int f = 0; f <= n; ++f) {
+    a.first[f] = k;
+    a.prev_a[f] = prev_a[f];
+    for (auto &t : terms[f]) {
+      a.bit[k] = (int8_t)t.first;
+      a.w[k] = t.second;
+      ++k;
+    }
+  }
+  a.first[n + 1] = k;
+  double flops = 0, elems = (double)last.c_elems;
+  for (size_t m : run) flops += 6.0 * L.steps[m].c_elems;   // one complex product per output element
+  for (int f = 0; f <= n; ++f) {
+    int64_t fe = 1;   // elements the factor's terms reach
+    for (int t = a.first[f]; t < a.first[f + 1]; ++t) fe += a.w[t];
+    elems += (double)fe;
+  }
+  out.chain_flops = flops;
+  out.chain_elems = elems;
+  return true;
+}
+
+// Runs of at least two outer products, each member's output read only by the next
+// member (same section, not the saved result), become one Kern::Outer step placed
+// where the run's last step was (one launch and one write of the final product
+// instead of a launch and an intermediate per member).  Returns the nodes whose
+// values are no longer materialised.
+std::vector<int> form_outer_runs(const Program &p, Lowered &L) {
+  const size_t nn = p.nodes.size();
+  std::vector<int> uses(nn, 0);
+  for (const Node &n : p.nodes) {
+    if (n.op == Op::Ncon) uses[n.ia]++, uses[n.ib]++;
+    if (n.op == Op::View || n.op == Op::Save) uses[n.ia]++;
+  }
+  std::vector<std::vector<size_t>> runs;
+  std::map<int, size_t> open;   // output node of a run's last member -> run
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    const Step &st = L.steps[s];
+    if (!outer_product(st)) continue;
+    int from = -1;
+    for (int in : {st.in_a, st.in_b}) {
+      auto it = open.find(in);
+      if (it != open.end() && uses[in] == 1 && in != p.save && L.steps[runs[it->second].back()].section == st.section) {
+        from = (int)it->second;
+        open.erase(it);
+        break;
+      }
+    }
+    if (from < 0) {
+      from = (int)runs.size();
+      runs.push_back({});
+    }
+    runs[from].push_back(s);
+    open[st.node] = (size_t)from;
+  }
+  std::vector<int> absorbed;
+  std::vector<char> drop(L.steps.size(), 0);
+  std::map<size_t, Step> place;
+  for (auto &run : runs) {
+    // runs that end in a small step stay: the small steps of a section run batched by
+    // dependency level, many per launch
+    if (run.size() < 2 || small_step(L.steps[run.back()], false)) continue;
+    Step f;
+    if (!outer_step(L, run, f)) continue;
+    for (size_t j = 0; j + 1 < run.size(); ++j) {
+      drop[run[j]] = 1;
+      absorbed.push_back(L.steps[run[j]].node);
+    }
+    place[run.back()] = std::move(f);
+  }
+  std::vector<Step> out;
+  for (size_t s = 0; s < L.steps.size(); ++s) {
+    if (drop[s]) continue;
+    auto it = place.find(s);
+    out.push_back(it != place.end() ? std::move(it->second) : std::move(L.steps[s]));
+  }
+  L.steps = std::move(out);
+  return absorbed;
+}
+
 }  // namespace
 
 // Wave-local chains: when every step contracts and batches at most four tile bits
@@ -1188,7 +1369,7 @@ namespace {
 // ranges into a few elements): then up to 2^16 MACs with any split.  (The forward
 // keeps the limit: its larger steps run faster as multi-workgroup launches.)
 bool small_step(const Step &st, bool wide = false) {
-  if (st.section == 1 || st.kern == Kern::Chain || st.kern == Kern::Radix) return false;
+  if (st.section == 1 || st.kern == Kern::Chain || st.kern == Kern::Radix || st.kern == Kern::Outer) return false;
   const int64_t macs = st.c_elems * st.K;
   if (macs > (int64_t(1) << 18)) return false;
   if ((st.c_elems > 4096 || st.K > 4096) && (!wide || macs > (int64_t(1) << 16))) return false;
@@ -1526,6 +1707,7 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
   std::vector<char> absorbed(nn, 0);
   if (chains && !keep_all) {
     for (int a : form_chains(p, L)) absorbed[a] = 1;
+    for (int a : form_outer_runs(p, L)) absorbed[a] = 1;
     for (Step &st : L.steps) {
       if (st.kern == Kern::Chain) {
         chain_wave_local(st.chain);
@@ -1600,6 +1782,7 @@ Lowered lower(const Program &p, int64_t batch, bool keep_all, int precision, uin
     std::vector<int> ins = {L.steps[s].in_a, L.steps[s].in_b};
     ins.insert(ins.end(), L.steps[s].chain_small.begin(), L.steps[s].chain_small.end());
     for (const auto &cm : L.steps[s].chain_merge) ins.push_back(cm.small);
+    ins.insert(ins.end(), L.steps[s].outer_in.begin(), L.steps[s].outer_in.end());
     for (int in : ins) {
       int r = root[in];
       if (r < 0) continue;

@@ -257,7 +257,25 @@ struct BlasArgs {
   int own = 0, ksplit = 1, gauss = 0;
 };
 
-enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows, Chain, Blas };
+// Runs of outer products (steps with no contracted leg, each output feeding only the
+// next step of the run): C = F_0 x F_1 x ... over the factors the run reads, legs two
+// factors share (the bitstring batch leg) multiplied elementwise.  Output element e
+// (C dense, e its offset) is the product of F_f[sum over the set bits b of e of the
+// factor's stride for bit b], taken in the run's order: v = F_0, then for each later
+// factor v = v F_f (the running value was that step's A operand) or F_f v.
+constexpr int kOuterMaxFactors = 16, kOuterMaxTerms = 192;
+struct OuterArgs {
+  int nf = 0;                            // factors
+  int members = 0;                       // steps of the run
+  int first[kOuterMaxFactors + 1] = {0}; // factor f's terms: [first[f], first[f + 1])
+  int8_t prev_a[kOuterMaxFactors] = {0}; // factor f >= 1: the running value was its step's A
+  int8_t bit[kOuterMaxTerms] = {0};      // output bit of each term
+  int64_t w[kOuterMaxTerms] = {0};       // the factor's stride for that bit
+  const void *F[kOuterMaxFactors] = {nullptr};   // set at launch
+};
+static_assert(sizeof(OuterArgs) <= 4096, "outer-product arguments must fit the kernel argument segment");
+
+enum class Kern : int8_t { Bits, Radix, Gemm, Reduce, Rows, Chain, Blas, Outer };
 
 struct Step {
   int node = -1;           // ncon node
@@ -283,7 +301,9 @@ struct Step {
     int pos[4] = {0}, sk[4] = {0}, sn[4] = {0};
   };
   std::vector<ChainMember> chain_merge;   // Kern::Chain: members of merged steps, in order
-  double chain_flops = 0, chain_elems = 0;   // Kern::Chain: work and compulsory elements moved
+  double chain_flops = 0, chain_elems = 0;   // Kern::Chain / Outer: work and compulsory elements moved
+  OuterArgs outer;                 // Kern::Outer: in_a, in_b = its first two factors
+  std::vector<int> outer_in;       // Kern::Outer: node of each factor
 };
 
 struct ArenaSlot {
@@ -418,6 +438,7 @@ void launch_planar_merge(const float *re, const float *im, int64_t n, void *dst,
 void launch_zero_batch(void *base, const int64_t *d_ranges, int n, void *stream);
 // One chain pass: k_chain_pack (gate blocks and step tables into the workspace), for
 // merged steps k_chain_merge (their members' product), then k_contract_chain.
+void launch_outer(int prec, void *C, int64_t c_elems, const OuterArgs &args, void *stream);
 void launch_contract_chain(int prec, const void *X0, void *Xm, void *ws, const ChainArgs &args,
                            const ChainMergeArgs &merge, void *stream);
 // C = op(X) op(Y) by the hand-written matrix-core GEMM (b.own): plain column-major

@@ -185,6 +185,7 @@ namespace {
 
 // Contraction steps a lowered step performs (a chain pass: its members, merged or not).
 int fused_steps(const qx::Step &st) {
+  if (st.kern == qx::Kern::Outer) return st.outer.members;
   if (st.kern != qx::Kern::Chain) return 1;
   int n = st.chain.nsteps;
   for (int j = 0; j < st.chain.nsteps; ++j) n -= st.chain.st[j].merged;
@@ -402,6 +403,15 @@ void launch_step(qx_plan *P, const qx::Lowered &L, const qx::Step &st, void *A, 
         for (int q = 0; q < 4; ++q) mm.pos[q] = cm.pos[q], mm.sk[q] = cm.sk[q], mm.sn[q] = cm.sn[q];
       }
       qx::launch_contract_chain(P->prec, A, C, P->ws.p, a, m, stream);
+      break;
+    }
+    case qx::Kern::Outer: {
+      qx::OuterArgs a = st.outer;
+      for (int f = 0; f < a.nf; ++f) {
+        const qx::Value &v = L.values[st.outer_in[f]];
+        a.F[f] = bs ? value_ptr(P, v, varval, *bs) : value_ptr(P, v, varval);
+      }
+      qx::launch_outer(P->prec, C, st.c_elems, a, stream);
       break;
     }
     case qx::Kern::Radix: qx::launch_contract_radix(P->prec, A, B, C, st.c_elems, st.radix, stream); break;
@@ -1115,6 +1125,7 @@ const char *kern_name(const qx::Step &st) {
     case qx::Kern::Reduce: return "reduce";
     case qx::Kern::Rows: return "rows";
     case qx::Kern::Chain: return "chain";
+    case qx::Kern::Outer: return "outer";
     case qx::Kern::Blas: return st.blas.own ? "plain_gemm" : "blas";
     default: return "bits";
   }
@@ -1140,6 +1151,7 @@ std::string kern_variant(const qx::Step &st) {
       return "k_contract_reduce";
     case qx::Kern::Radix: return "k_contract_radix";
     case qx::Kern::Chain: return "k_contract_chain";
+    case qx::Kern::Outer: return "k_outer_run";
     case qx::Kern::Blas:
       if (st.blas.own)
         return std::string(st.blas.split ? "k_plain_gemm_bf16x3<" : "k_plain_gemm<") + (st.blas.opx ? "T" : "N") +
@@ -1173,7 +1185,7 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
     if (s) os << ",";
     double flops = 8.0 * st.H * st.M * st.N * st.K;
     double bytes = (double)es * st.H * (st.M * st.K + st.K * st.N + st.M * st.N);
-    if (st.kern == qx::Kern::Chain) flops = st.chain_flops, bytes = (double)es * st.chain_elems;
+    if (st.kern == qx::Kern::Chain || st.kern == qx::Kern::Outer) flops = st.chain_flops, bytes = (double)es * st.chain_elems;
     os << "{\"name\":\"" << prog.nodes[st.node].name << "\",\"section\":" << st.section
        << ",\"fused_steps\":" << fused_steps(st) << ",\"B\":" << st.H
        << ",\"M\":" << st.M << ",\"N\":" << st.N << ",\"K\":" << st.K << ",\"flops\":" << flops
@@ -1186,6 +1198,7 @@ void lowered_json(std::ostringstream &os, const qx::Program &prog, const qx::Low
       };
       os << ",\"level\":" << st.level << ",\"reads\":[\"" << root_name(st.in_a) << "\",\"" << root_name(st.in_b) << "\"";
       for (int c : st.chain_small) os << ",\"" << root_name(c) << "\"";
+      for (size_t f = 2; f < st.outer_in.size(); ++f) os << ",\"" << root_name(st.outer_in[f]) << "\"";
       os << "]";
     }
     if (with_slots) {   // leg strides (elements) in A, B and C: free legs in C order, then contracted legs
@@ -1563,7 +1576,7 @@ std::string stats_json(const qx_plan *P) {
       if (s) os << ",";
       double flops = 8.0 * st.H * st.M * st.N * st.K;
       double bytes = (double)P->esize() * st.H * (st.M * st.K + st.K * st.N + st.M * st.N);
-      if (st.kern == qx::Kern::Chain) flops = st.chain_flops, bytes = (double)P->esize() * st.chain_elems;
+      if (st.kern == qx::Kern::Chain || st.kern == qx::Kern::Outer) flops = st.chain_flops, bytes = (double)P->esize() * st.chain_elems;
       std::string variant = kern_variant(st);
       if (st.level >= 0 && P->opt_small_walk) {   // levels walked by one workgroup
         auto it = P->small.find(&L);

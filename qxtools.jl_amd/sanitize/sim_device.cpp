@@ -387,6 +387,24 @@ void launch_contract_rows(int prec, const void *A, const void *B, void *C, const
   });
 }
 
+void launch_outer(int prec, void *C, int64_t c_elems, const OuterArgs &a, void *stream) {
+  const size_t es = esize(prec);
+  if (a.nf < 2 || a.nf > kOuterMaxFactors) violation("outer run: factor count");
+  std::vector<std::pair<const void *, long double>> reach;
+  for (int f = 0; f < a.nf; ++f) {
+    long double r = 1;
+    for (int k = a.first[f]; k < a.first[f + 1]; ++k) {
+      if (a.bit[k] < 0 || (int64_t(1) << a.bit[k]) >= c_elems) violation("outer run: term bit outside C");
+      r += (long double)a.w[k];
+    }
+    reach.push_back({a.F[f], r * es});
+  }
+  launch(stream, [=]() {
+    for (auto &x : reach) need(x.first, x.second, "outer run factor");
+    need(C, (long double)c_elems * es, "outer run C");
+  });
+}
+
 void launch_contract_radix(int prec, const void *A, const void *B, void *C, int64_t c_elems, const RadixArgs &r,
                            void *stream) {
   const size_t es = esize(prec);

@@ -80,7 +80,11 @@ save output t11
 """
     info = _native.inspect(dsl, batch=4)
     names = [s["name"] for s in info["steps"]]
-    assert names == ["I2", "t8", "t9", "t10", "I1", "t11"]
+    # t8 (outer product of t5_s and I2) and t9 (t8 times t3_s on a shared leg) form
+    # one outer-product run
+    assert names == ["I2", "t9", "t10", "I1", "t11"]
+    t9 = info["steps"][1]
+    assert t9["kernel"] == "outer" and t9["fused_steps"] == 2
     assert info["result_elems"] == 1
 
 
@@ -175,6 +179,22 @@ def test_chains_fold_gate_block_runs():
     assert len(big) <= 20                      # 72 passes without chains or stages
     for s in chains:   # launch configurations of the one chain kernel
         assert all(c.startswith("k_contract_chain/") for c in s["candidates"]) and len(s["candidates"]) >= 3
+
+
+def test_outer_product_runs_fold_into_one_step():
+    """The output vectors of the 5x5 depth-20 plan (13 and 12 qubits' bitstring
+    vectors, one outer product per qubit) are built by two outer-product runs, each
+    one launch writing only the final 256 x 2^13 / 2^12 product; the factors it reads
+    are the per-qubit vectors and the run's first operand."""
+    dsl, data, bits = workloads.WORKLOADS["rqc_5x5_d20"].load()
+    info = _native.inspect(dsl, batch=256)
+    runs = [s for s in info["steps"] if s["kernel"] == "outer"]
+    assert sorted(s["M"] for s in runs) == [256 << 12, 256 << 13]
+    for s in runs:
+        assert s["section"] == 2 and s["fused_steps"] >= 10 and len(s["reads"]) == s["fused_steps"] + 1
+        assert s["candidates"] == ["k_outer_run"]
+    assert not any(s["kernel"] == "rows" and s["section"] == 2 for s in info["steps"])
+    _check_schedule(info)
 
 
 def test_batched_small_levels_do_not_share_memory():
