@@ -2273,17 +2273,24 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
     for (int i = 0; i < NS; ++i)
       if (t + kChainThreads * i < a.s_elems) sS[ss_slot(t + kChainThreads * i)] = sv[i];
   }
+  // offset of the next tile; its first quarter of loads is issued right after the
+  // current tile's refill, so that those loads run during the barrier and not during
+  // the tile's first step
+  int64_t bn = 0;
   if (w < ntiles) {
     static_for<kChainPer>([&](auto r) { tile_e[rsi(r)] = pf[r]; });
+    if (w + gridDim.x < ntiles) {
+      bn = next_base(w, bi, a.oin);
+      load_quarter(0, bn);
+    }
     lds_barrier();
   }
   for (; w < ntiles; w += gridDim.x) {
     const bool more = w + gridDim.x < ntiles;
-    const int64_t bn = more ? next_base(w, bi, a.oin) : 0;
     const int n = a.nsteps;
     for (int j = 0; j < n; ++j) {
       if (more)
-        for (int q = 0; q < 4; ++q)
+        for (int q = 1; q < 4; ++q)
           if (q * n / 4 == j) load_quarter(q, bn);
       // wave-local steps: the two waves of a SIMD (waves w and w + 4) take turns at the
       // higher issue priority, step by step, so that neither runs far ahead and the
@@ -2333,6 +2340,10 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
     lds_barrier();
     if (more) {
       static_for<kChainPer>([&](auto r) { tile_e[rsi(r)] = pf[r]; });
+      if (w + 2 * (int64_t)gridDim.x < ntiles) {
+        bn = next_base(w + gridDim.x, bi, a.oin);
+        load_quarter(0, bn);
+      }
       lds_barrier();
     }
   }
