@@ -107,8 +107,8 @@ qx_status qx_plan_update_tensor(qx_plan *plan, const char *key, const double *z,
  *                 every step (same results), times them with HIP events and keeps
  *                 the fastest per step for later calls; default 0
  *   "chains"      1 (default) = fold runs of gate-block steps on one large tensor
- *                 into single passes and batch small steps per dependency level;
- *                 0 = one launch per step
+ *                 into single passes, runs of outer products into one launch each,
+ *                 and batch small steps per dependency level; 0 = one launch per step
  *   "batch_last"  1 = lay the bitstring batch leg out slowest in bitstring-dependent
  *                 intermediates (default 0: fastest)
  *   "small_walk"  1 = a section whose batched small steps total at most 2^22
