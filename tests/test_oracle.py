@@ -234,3 +234,32 @@ def test_users_guide_dsl_example():
     s2 = 1 / np.sqrt(2)
     assert np.allclose(out[[0, 1, 4, 5]], [s2, 0, 0, s2], rtol=0, atol=1e-16)   # 00 01 10 11
     assert len(O.slice_vars(O.parse(dsl))) == 2
+
+
+def test_step_shapes_hand_counted():
+    """The oracle's per-step (B, M, N, K) on a program whose classes are counted by hand:
+    label 2 is in A, B and C (batch, 4), label 1 only in A and C (M, 2), label 3 only in
+    B and C (N, 16), label 4 in A and B only (K, 8); the second step sums labels 3 and 2
+    (K = 64) and keeps label 1 (M = 2).  Exchanging M and N, or K and B, fails the
+    tuple; the library's lowering (an independent classification in C++) agrees on B
+    and K and on the pair {M, N} (it may exchange the operands)."""
+    from qxhip import _native
+    dsl = """# version: 0.4.0
+load a data_1 2,4,8
+load b data_2 8,4,16
+ncon c 1,3,2 a 1,2,4 b 4,2,3
+load f data_3 16,4
+ncon e 1 c 1,3,2 f 3,2
+save output e
+"""
+    shapes = O.step_shapes(dsl)
+    assert shapes == [(4, 2, 16, 8), (1, 2, 1, 64)]
+    # multiply-adds by brute force over every label assignment of the first step
+    dims = {1: 2, 2: 4, 3: 16, 4: 8}
+    macs = sum(1 for _ in itertools.product(*(range(dims[l]) for l in (1, 2, 3, 4))))
+    B, M, N, K = shapes[0]
+    assert macs == B * M * N * K
+    info = _native.inspect(dsl, batch=1)
+    lowered = [(s["B"], s["M"], s["N"], s["K"]) for s in info["steps"]]
+    for (b, m, n, k), (lb, lm, ln, lk) in zip(shapes, lowered):
+        assert (lb, lk) == (b, k) and sorted((lm, ln)) == sorted((m, n))
