@@ -82,10 +82,12 @@ def greedy_contraction_plan(tn, seed: int | None = None, alpha: float = 1.0,
     tick = 0
 
     def push_pairs(n):
+        # sorted: the pairs' order (heap ties, noise draws) must not depend on the
+        # interpreter's string hashing
         nonlocal tick
         seen = set()
-        for i in cur[n]:
-            for m in holders[i]:
+        for i in sorted(cur[n]):
+            for m in sorted(holders[i]):
                 if m != n and m not in seen:
                     seen.add(m)
                     tick += 1
