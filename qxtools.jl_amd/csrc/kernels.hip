@@ -2222,15 +2222,26 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
       if ((r >> i) & 1) o += a.wout[a.pout[kChainTileBits - RB + i]];
     return o;
   };
-  // the r bits are never swizzled (the host keeps their images plain powers of two):
-  // in load order they are the top tile bits, above every bit of sin, so their offset
-  // is a constant that folds into the LDS instructions
-  auto rsi = [&](auto r) { return sin + (uint32_t)(kChainThreads * r); };
+  // element r's LDS address: the lane's part combined with the images of the r bits
+  // (in load order the top tile bits; in store order the last four X_m tile bits).
+  // Plain, the load-order part is a constant above every bit of sin, which folds into
+  // the LDS instructions; swizzled, one XOR with a uniform value per element.
+  auto rsi = [&](auto r) {
+    if constexpr (!SWZ) {
+      return sin + (uint32_t)(kChainThreads * r);
+    } else {
+      uint32_t o = sin;
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+        if ((r >> i) & 1) o ^= (uint32_t)a.swz[kChainTileBits - RB + i];
+      return o;
+    }
+  };
   auto rso = [&](auto r) {
     uint32_t o = sout;
 #pragma unroll
     for (int i = 0; i < RB; ++i)
-      if ((r >> i) & 1) o = tile_cmb<SWZ>(o, 1u << a.pout[kChainTileBits - RB + i]);
+      if ((r >> i) & 1) o = tile_cmb<SWZ>(o, tile_img<SWZ>(a.swz, a.pout[kChainTileBits - RB + i]));
     return o;
   };
   // prefetch registers as native two-element vectors (a struct copy would keep them

@@ -1287,10 +1287,13 @@ std::vector<int> step_lane_choices(const ChainArgs &a, const ChainStepArgs &c) {
 // to independent vectors.  A cycle's lanes differ in: the low tile bits (loads), the
 // fastest X_m tile bits (stores) and each step's group bits 0-3, its lowest free tile
 // bits other than the wave bits of a wave-local chain (in both the vector and the
-// matrix-core step, whose result columns are groups).  G is 0 on the bits of a lane's
-// element index r (tile bits 9-12 for loads, the last four X_m tile bits for stores)
-// so those offsets stay additive; it minimises the summed conflict degree (weights: 2
-// per step, 1 per load / store pass) by hill climbing from fixed seeds.
+// matrix-core step, whose result columns are groups).  Every tile bit from nb up may
+// carry a pattern, the bits of a lane's element index r included (tile bits 9-12 for
+// loads, the last four X_m tile bits for stores: their images then combine by XOR, one
+// vector op per element of a refill or a store; kept plain, they would leave eight of
+// the thirteen bits without a pattern and many steps' lanes on a few bank groups).
+// G minimises the summed conflict degree (weights: 2 per step, 1 per load / store
+// pass) by hill climbing from fixed seeds.
 void chain_swizzle(ChainArgs &a, int elem_bytes) {
   int nb = 0;
   while ((elem_bytes << (nb + 1)) <= kLdsCycleBytes) ++nb;
@@ -1307,8 +1310,7 @@ void chain_swizzle(ChainArgs &a, int elem_bytes) {
     sets.push_back({v, 2});
   }
   std::vector<char> fixed(kChainTileBits, 0);   // G must stay 0
-  for (int b = 0; b < kChainTileBits; ++b) fixed[b] = b < nb || b >= kChainTileBits - 4;
-  for (int i = kChainTileBits - 4; i < kChainTileBits; ++i) fixed[a.pout[i]] = 1;
+  for (int b = 0; b < nb; ++b) fixed[b] = 1;
   std::vector<uint32_t> G(kChainTileBits, 0);
   auto image = [&](int b) { return b < nb ? 1u << b : G[b]; };
   auto cost = [&]() {

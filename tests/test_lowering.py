@@ -285,13 +285,13 @@ def _lds_groups(swz, bits, nb):
 
 def test_chain_lds_swizzle_is_a_bijection_that_spreads_banks():
     """Every chain tile's swizzle maps the 2^13 tile elements onto distinct LDS slots
-    (brute force), keeps the element-index bits (9-12, the last four store bits) and
-    the low four tile bits plain, and lets the lanes of each step's first access cycle
+    (brute force), keeps the low four tile bits plain, and lets the lanes of each
+    step's first access cycle
     (16 lanes of 16-byte elements, differing in the step's lowest four free tile bits
     outside the wave bits of a wave-local chain, in the vector and the matrix-core
-    steps alike) reach no fewer bank groups than the plain layout, and more on average
-    (7.5 of 16 against 3.5 in the 5x5 plan with its merged gate blocks, whose chains are
-    all wave-local: the wave bits take three of the low free bits)."""
+    steps alike) reach no fewer bank groups than the plain layout, and all 16 in the
+    5x5 plan with its merged gate blocks (the plain layout: 3.5 on average; its chains
+    are all wave-local, so the wave bits take three of the low free bits)."""
     dsl, data, bits = workloads.WORKLOADS["rqc_5x5_d20"].load()
     info = _native.inspect(dsl, batch=256)
     chains = [s for s in info["steps"] if s["kernel"] == "chain"]
@@ -309,7 +309,7 @@ def test_chain_lds_swizzle_is_a_bijection_that_spreads_banks():
                     a ^= swz[b]
             addr.add(a)
         assert addr == set(range(1 << 13))
-        for b in list(range(nb)) + list(range(9, 13)) + pout[9:]:
+        for b in range(nb):
             assert swz[b] == 1 << b
         plain = [1 << b for b in range(13)]
         used = swz if t["swizzled"] else plain
@@ -322,4 +322,4 @@ def test_chain_lds_swizzle_is_a_bijection_that_spreads_banks():
             groups.append(got)
             plain_groups.append(ref)
     assert all(c["chain_tile"]["wave_local"] for c in chains)
-    assert sum(groups) / len(groups) >= 7 and sum(groups) > 1.6 * sum(plain_groups)
+    assert min(groups) == 1 << nb and sum(groups) > 4 * sum(plain_groups)
