@@ -2222,23 +2222,30 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
       if ((r >> i) & 1) o += a.wout[a.pout[kChainTileBits - RB + i]];
     return o;
   };
-  // element r's LDS address: the lane's part combined with the images of the r bits
-  // (in load order the top tile bits; in store order the last four X_m tile bits).
-  // Plain, the load-order part is a constant above every bit of sin, which folds into
-  // the LDS instructions; swizzled, one XOR with a uniform value per element.
-  auto rsi = [&](auto r) {
+  // element r's LDS address: the lane's part (`li`: sin, `lo`: sout) combined with the
+  // images of the r bits (in load order the top tile bits; in store order the last four
+  // X_m tile bits).  Plain, the load-order part is a constant above every bit of sin,
+  // which folds into the LDS instructions; swizzled, one XOR with a uniform value per
+  // element.  The lane parts pass through `fresh` at each use, so that the sixteen
+  // addresses are formed next to their accesses rather than hoisted out of the tile
+  // loop, where they would stay live across the steps (and spill).
+  auto fresh = [](uint32_t x) {
+    asm volatile("" : "+v"(x));
+    return x;
+  };
+  auto rsi = [&](uint32_t li, auto r) {
     if constexpr (!SWZ) {
-      return sin + (uint32_t)(kChainThreads * r);
+      return li + (uint32_t)(kChainThreads * r);
     } else {
-      uint32_t o = sin;
+      uint32_t o = li;
 #pragma unroll
       for (int i = 0; i < RB; ++i)
         if ((r >> i) & 1) o ^= (uint32_t)a.swz[kChainTileBits - RB + i];
       return o;
     }
   };
-  auto rso = [&](auto r) {
-    uint32_t o = sout;
+  auto rso = [&](uint32_t lo, auto r) {
+    uint32_t o = lo;
 #pragma unroll
     for (int i = 0; i < RB; ++i)
       if ((r >> i) & 1) o = tile_cmb<SWZ>(o, tile_img<SWZ>(a.swz, a.pout[kChainTileBits - RB + i]));
@@ -2289,7 +2296,7 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
   // the tile's first step
   int64_t bn = 0;
   if (w < ntiles) {
-    static_for<kChainPer>([&](auto r) { tile_e[rsi(r)] = pf[r]; });
+    static_for<kChainPer>([&](auto r) { tile_e[rsi(sin, r)] = pf[r]; });
     if (w + gridDim.x < ntiles) {
       bn = next_base(w, bi, a.oin);
       load_quarter(0, bn);
@@ -2345,12 +2352,16 @@ __global__ __launch_bounds__(kChainThreads) void k_contract_chain(const typename
       lds_barrier();
     }
     E *Xme = reinterpret_cast<E *>(Xm);
-    static_for<kChainPer>([&](auto r) { Xme[bo + rout(r)] = tile_e[rso(r)]; });
+    {
+      const uint32_t lo = fresh(sout);
+      static_for<kChainPer>([&](auto r) { Xme[bo + rout(r)] = tile_e[rso(lo, r)]; });
+    }
     if (more) bo = next_base(w, bo, a.oout);
     bi = bn;
     lds_barrier();
     if (more) {
-      static_for<kChainPer>([&](auto r) { tile_e[rsi(r)] = pf[r]; });
+      const uint32_t li = fresh(sin);
+      static_for<kChainPer>([&](auto r) { tile_e[rsi(li, r)] = pf[r]; });
       if (w + 2 * (int64_t)gridDim.x < ntiles) {
         bn = next_base(w + gridDim.x, bi, a.oin);
         load_quarter(0, bn);
