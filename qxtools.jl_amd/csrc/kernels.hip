@@ -1064,37 +1064,52 @@ void rows_launch_k(int prec, const void *A, const void *B, void *C, const RowArg
 
 // Runs of outer products (OuterArgs): lane per output element, the product of the
 // factors in the run's order, each factor's offset the sum of its strides over the
-// element's set bits.  A workgroup covers 256 consecutive elements per trip: the lowest
-// 8 bits are the lane's own (their part of every factor's offset is summed once per
-// lane), the bits above are the same for all lanes (summed once per trip, one lane per
-// factor); both through LDS.
+// element's set bits.  The terms first go into a dense LDS table (factor, output bit)
+// -> stride (a factor has at most one term per bit), read without loop-carried waits.
+// A workgroup covers 256 consecutive elements per trip: the lowest 8 bits are the
+// lane's own (their part of every factor's offset is summed once per lane, kept in
+// LDS), the bits above are the same for all lanes (summed once per trip, 16 lanes per
+// factor).
 template <typename T>
 __global__ __launch_bounds__(256) void k_outer_run(typename Vec<T>::type *__restrict__ C, int64_t n, OuterArgs a) {
   using V = typename Vec<T>::type;
-  __shared__ int sbit[kOuterMaxTerms];
-  __shared__ int64_t sw[kOuterMaxTerms];
+  __shared__ int64_t tw[kOuterMaxFactors * kOuterBits];
   __shared__ int64_t shi[kOuterMaxFactors];
+  __shared__ int64_t slo[kOuterMaxFactors * 256];
   const int t = threadIdx.x;
   const int nf = a.nf, nt = a.first[nf];
+  for (int i = t; i < kOuterMaxFactors * kOuterBits; i += 256) tw[i] = 0;
+  __syncthreads();
   for (int k = t; k < nt; k += 256) {
-    sbit[k] = a.bit[k];
-    sw[k] = a.w[k];
+    int f = 0;
+#pragma unroll
+    for (int g = 1; g < kOuterMaxFactors; ++g)
+      if (g < nf && k >= a.first[g]) f = g;
+    tw[f * kOuterBits + a.bit[k]] = a.w[k];
   }
   __syncthreads();
-  // the lane's own low-bit offsets, one per factor, in LDS (each lane reads only its own)
-  __shared__ int64_t slo[kOuterMaxFactors * 256];
+  // the lane's own low-bit offsets, one per factor (each lane reads only its own)
   for (int f = 0; f < nf; ++f) {
     int64_t o = 0;
-    for (int k = a.first[f]; k < a.first[f + 1]; ++k)
-      if (sbit[k] < 8 && ((t >> sbit[k]) & 1)) o += sw[k];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      if ((t >> b) & 1) o += tw[f * kOuterBits + b];
     slo[f * 256 + t] = o;
   }
   for (int64_t base = (int64_t)blockIdx.x * 256; base < n; base += (int64_t)gridDim.x * 256) {
-    if (t < nf) {
-      int64_t h = 0;
-      for (int k = a.first[t]; k < a.first[t + 1]; ++k)
-        if (sbit[k] >= 8 && ((base >> sbit[k]) & 1)) h += sw[k];
-      shi[t] = h;
+    {
+      // 16 lanes per factor (f < kOuterMaxFactors = 256 / 16), each summing every
+      // 16th bit, then across the 16
+      static_assert(kOuterMaxFactors * 16 == 256, "16 lanes per factor");
+      const int f = t >> 4, j = t & 15;
+      long long h = 0;
+      if (f < nf) {
+#pragma unroll
+        for (int b = 8 + j; b < kOuterBits; b += 16) h += ((base >> b) & 1) ? tw[f * kOuterBits + b] : 0;
+      }
+#pragma unroll
+      for (int m = 8; m >= 1; m >>= 1) h += __shfl_xor(h, m, 16);
+      if (j == 0 && f < nf) shi[f] = h;
     }
     __syncthreads();
     const int64_t e = base + t;

@@ -132,8 +132,10 @@ static void check_bounds(const Program &p, const Lowered &L, const std::vector<i
       for (int f = 0; f < a.nf; ++f) {
         int64_t mf = 0;
         for (int t = a.first[f]; t < a.first[f + 1]; ++t) {
-          if (a.bit[t] < 0 || (int64_t(1) << a.bit[t]) >= st.c_elems || a.w[t] <= 0)
+          if (a.bit[t] < 0 || a.bit[t] >= kOuterBits || (int64_t(1) << a.bit[t]) >= st.c_elems || a.w[t] <= 0)
             throw Error(QX_ERR_STATE, "internal: outer-product run term");
+          for (int u = a.first[f]; u < t; ++u)
+            if (a.bit[u] == a.bit[t]) throw Error(QX_ERR_STATE, "internal: outer-product run bit named twice");
           mf += a.w[t];
         }
         auto ef = extent(st.outer_in[f]);

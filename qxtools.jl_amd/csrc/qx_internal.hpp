@@ -264,6 +264,7 @@ struct BlasArgs {
 // factor's stride for bit b], taken in the run's order: v = F_0, then for each later
 // factor v = v F_f (the running value was that step's A operand) or F_f v.
 constexpr int kOuterMaxFactors = 16, kOuterMaxTerms = 192;
+constexpr int kOuterBits = 48;   // output bits an outer-product run's terms may name
 struct OuterArgs {
   int nf = 0;                            // factors
   int members = 0;                       // steps of the run

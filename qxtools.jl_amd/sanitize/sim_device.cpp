@@ -394,7 +394,10 @@ void launch_outer(int prec, void *C, int64_t c_elems, const OuterArgs &a, void *
   for (int f = 0; f < a.nf; ++f) {
     long double r = 1;
     for (int k = a.first[f]; k < a.first[f + 1]; ++k) {
-      if (a.bit[k] < 0 || (int64_t(1) << a.bit[k]) >= c_elems) violation("outer run: term bit outside C");
+      if (a.bit[k] < 0 || a.bit[k] >= kOuterBits || (int64_t(1) << a.bit[k]) >= c_elems)
+        violation("outer run: term bit outside C");
+      for (int u = a.first[f]; u < k; ++u)
+        if (a.bit[u] == a.bit[k]) violation("outer run: a factor names a bit twice (its LDS table slot)");
       r += (long double)a.w[k];
     }
     reach.push_back({a.F[f], r * es});
