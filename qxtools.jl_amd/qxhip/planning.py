@@ -463,10 +463,24 @@ class PlanCost:
         return t * 2.0 ** sum(lg[i] for i in sliced)
 
 
+def plan_depth(plan: list) -> int:
+    """Levels of the plan's dependency tree: the longest chain of steps each reading the
+    previous one's result."""
+    lvl: dict = {}
+    for a, b, c in plan:
+        lvl[c] = 1 + max(lvl.get(a, 0), lvl.get(b, 0))
+    return max(lvl.values(), default=0)
+
+
 def contraction_plan(tn, trials: int = 32, seed: int = 42, max_log2_size: float | None = None,
-                     open_indices=(), min_fill_trials: int = 0) -> list:
-    """Best plan by total flops (rank cap optional) over `trials` randomised greedy
-    passes and `min_fill_trials` noisy min-fill line-graph elimination orders."""
+                     open_indices=(), min_fill_trials: int = 0, level_seconds: float = 5e-6) -> list:
+    """Best plan (rank cap optional) over `trials` randomised greedy passes and
+    `min_fill_trials` noisy min-fill line-graph elimination orders, by modelled device
+    time: the roofline time of its steps (`PlanCost.roofline_seconds`) plus
+    `level_seconds` per level of its dependency tree (`plan_depth`: the executor runs a
+    level's small steps in one launch, and a launch costs a few microseconds however
+    little it does).  Large plans are ranked by their arithmetic, small ones by their
+    depth; total flops break ties."""
     open_indices = _open_of(tn, open_indices)
     tn = _as_tn(tn)
     rng = np.random.default_rng(seed)
@@ -488,9 +502,10 @@ def contraction_plan(tn, trials: int = 32, seed: int = 42, max_log2_size: float 
         else:
             order = elimination_order(adj, seed=sd, heuristic="min_fill", noise=a1)
             plan = order_to_contraction_plan(order, tn, open_indices)
-        flops, mx = PlanCost(tn, plan, open_indices).cost()
+        pc = PlanCost(tn, plan, open_indices)
+        flops, mx = pc.cost()
         over = 0.0 if max_log2_size is None else max(0.0, mx - max_log2_size)
-        key = (over, flops, mx)
+        key = (over, pc.roofline_seconds() + level_seconds * plan_depth(plan), flops, mx)
         if best_key is None or key < best_key:
             best, best_key = plan, key
     return best
@@ -594,7 +609,7 @@ def contraction_scheme(tn, num: int, trials: int = 32, seed: int = 42,
     groups = expand_slice_bonds_to_hyperindices(tn, sliced)
     meta = OrderedDict()
     meta["Determination of contraction plan"] = OrderedDict([
-        ("Method used", "randomised greedy (best of trials by flops)"),
+        ("Method used", "randomised greedy (best of trials by roofline time plus launch levels)"),
         ("Trials", trials), ("Seed used", seed),
         ("log2 flops", round(flops0, 3)), ("log2 largest intermediate", round(mx0, 3)),
         ("Hypergraph used", True),
