@@ -480,7 +480,7 @@ def contraction_plan(tn, trials: int = 32, seed: int = 42, max_log2_size: float 
     `level_seconds` per level of its dependency tree (`plan_depth`: the executor runs a
     level's small steps in one launch, and a launch costs a few microseconds however
     little it does).  Large plans are ranked by their arithmetic, small ones by their
-    depth; total flops break ties."""
+    depth; total flops break ties.  level_seconds 0: by total flops alone."""
     open_indices = _open_of(tn, open_indices)
     tn = _as_tn(tn)
     rng = np.random.default_rng(seed)
@@ -505,7 +505,10 @@ def contraction_plan(tn, trials: int = 32, seed: int = 42, max_log2_size: float 
         pc = PlanCost(tn, plan, open_indices)
         flops, mx = pc.cost()
         over = 0.0 if max_log2_size is None else max(0.0, mx - max_log2_size)
-        key = (over, pc.roofline_seconds() + level_seconds * plan_depth(plan), flops, mx)
+        if level_seconds > 0:
+            key = (over, pc.roofline_seconds() + level_seconds * plan_depth(plan), flops, mx)
+        else:
+            key = (over, flops, mx)
         if best_key is None or key < best_key:
             best, best_key = plan, key
     return best
@@ -583,19 +586,21 @@ def expand_slice_bonds_to_hyperindices(tn, bonds: list) -> list:
 def contraction_scheme(tn, num: int, trials: int = 32, seed: int = 42,
                        max_log2_size: float | None = None, min_fill_trials: int = 0,
                        slicer: str = "width", element_bytes: int = 16,
-                       max_log2_slice_size: float | None = None) -> tuple:
+                       max_log2_slice_size: float | None = None, level_seconds: float = 5e-6) -> tuple:
     """(bond_groups, plan, metadata) as `contraction_scheme` returns
     (`src/contraction_planning.jl:219-299`).
 
     slicer: "width" (`greedy_slicing`, flops as the tie-break) or "roofline"
     (`roofline_slicing` with `element_bytes` per element and an optional cap on the
-    largest intermediate of one slice)."""
+    largest intermediate of one slice).  level_seconds: see `contraction_plan` (0 ranks
+    the candidates by flops alone)."""
     if slicer not in ("width", "roofline"):
         raise ValueError(f"unknown slicer {slicer!r}")
     open_indices = _open_of(tn, ())
     tn = _as_tn(tn)
     plan = contraction_plan(tn, trials=trials, seed=seed, max_log2_size=max_log2_size,
-                            open_indices=open_indices, min_fill_trials=min_fill_trials)
+                            open_indices=open_indices, min_fill_trials=min_fill_trials,
+                            level_seconds=level_seconds)
     pc = PlanCost(tn, plan, open_indices)
     flops0, mx0 = pc.cost()
     if num <= 0:
