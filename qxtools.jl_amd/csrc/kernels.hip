@@ -1389,8 +1389,9 @@ __global__ void k_plain_sum(typename Vec<T>::type *__restrict__ C, const typenam
 // complex product (Re = Xr.Yr - Xi.Yi with the Xi parts negated, Im = Xr.Yi + Xi.Yr).
 // The split happens while a K stage is staged in LDS: each lane loads two complex64
 // values adjacent in k, rounds them, and stores the packed pairs into four planes
-// (re hi, re lo, im hi, im lo) laid out [m][k] / [n][k] with 8 values of padding per row,
-// so a lane's fragment (8 consecutive k values of one row) is one 16-byte LDS read.
+// (re hi, re lo, im hi, im lo) laid out [m][k] / [n][k], a row's 16-byte chunks
+// XOR-permuted by the row, so a lane's fragment (8 consecutive k values of one row) is
+// one 16-byte LDS read.
 // Tiles of 64 x 64 of C per workgroup (2 x 2 waves of 2 x 2 blocks), K stages of 32,
 // the next stage fetched into registers during the products; K split and unit order as
 // in k_plain_gemm.  A lane's k slot is the same in both operands, so the products pair
@@ -1402,7 +1403,12 @@ __global__ __launch_bounds__(256) void k_plain_gemm_bf16x3(const float2 *__restr
                                                            BlasArgs b) {
   typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
   typedef float f4 __attribute__((ext_vector_type(4)));
-  constexpr int TM = 64, TN = 64, TK = 32, RW = (TK + 8) / 2;   // row: 20 dwords (16 used)
+  // a row of a plane: 16 dwords (32 bf16 values of k) in four 16-byte chunks, chunk c
+  // of row r stored at chunk c ^ ((r >> 2) & 3): the 16 rows a fragment read spans
+  // (one chunk each) and the 4 rows x 16 dwords a wave stores with lanes along k both
+  // fall on distinct banks
+  constexpr int TM = 64, TN = 64, TK = 32, RW = TK / 2;
+  auto slot = [](int row, int dw) { return dw ^ (((row >> 2) & 3) << 2); };
   constexpr int BI = 2, BJ = 2, NP = 4;                           // planes: rh, rl, ih, il
   __shared__ uint32_t sA[NP][TM][RW];
   __shared__ uint32_t sB[NP][TN][RW];
@@ -1468,10 +1474,11 @@ __global__ __launch_bounds__(256) void k_plain_gemm_bf16x3(const float2 *__restr
   // (hi, lo) planes of a pair of values adjacent in k, packed two per dword
   auto split_store = [&](uint32_t (*pl)[TM][RW], int row, int pair, float2 v0, float2 v1) {
     const uint16_t rh0 = bf16_bits(v0.x), rh1 = bf16_bits(v1.x), ih0 = bf16_bits(v0.y), ih1 = bf16_bits(v1.y);
-    pl[0][row][pair] = (uint32_t)rh0 | ((uint32_t)rh1 << 16);
-    pl[1][row][pair] = (uint32_t)bf16_bits(v0.x - bf16_value(rh0)) | ((uint32_t)bf16_bits(v1.x - bf16_value(rh1)) << 16);
-    pl[2][row][pair] = (uint32_t)ih0 | ((uint32_t)ih1 << 16);
-    pl[3][row][pair] = (uint32_t)bf16_bits(v0.y - bf16_value(ih0)) | ((uint32_t)bf16_bits(v1.y - bf16_value(ih1)) << 16);
+    const int d = slot(row, pair);
+    pl[0][row][d] = (uint32_t)rh0 | ((uint32_t)rh1 << 16);
+    pl[1][row][d] = (uint32_t)bf16_bits(v0.x - bf16_value(rh0)) | ((uint32_t)bf16_bits(v1.x - bf16_value(rh1)) << 16);
+    pl[2][row][d] = (uint32_t)ih0 | ((uint32_t)ih1 << 16);
+    pl[3][row][d] = (uint32_t)bf16_bits(v0.y - bf16_value(ih0)) | ((uint32_t)bf16_bits(v1.y - bf16_value(ih1)) << 16);
   };
   f4 accr[BI][BJ], acci[BI][BJ];
 #pragma unroll
@@ -1479,7 +1486,7 @@ __global__ __launch_bounds__(256) void k_plain_gemm_bf16x3(const float2 *__restr
 #pragma unroll
     for (int j = 0; j < BJ; ++j) accr[i][j] = f4{0, 0, 0, 0}, acci[i][j] = f4{0, 0, 0, 0};
   auto frag = [&](const uint32_t (*pl)[TM][RW], int plane, int row) {
-    const uint4 v = *reinterpret_cast<const uint4 *>(&pl[plane][row][4 * lq]);
+    const uint4 v = *reinterpret_cast<const uint4 *>(&pl[plane][row][slot(row, 4 * lq)]);
     return v;
   };
   auto as_bf8 = [](uint4 v) { return __builtin_bit_cast(bf8, v); };
